@@ -1,0 +1,181 @@
+/*
+ * oa_step.h -- C ABI of the MI355X (gfx950) CP electronic-step library
+ * (openatom_amd/liboa_step.so).
+ *
+ * The library replaces, on one GPU, the hot path of OpenAtom's Gamma-point
+ * electronic step.  Each entry point names the reference interface it stands
+ * in for (paths relative to the OpenAtom tree).  All arithmetic at the
+ * boundary is FP64; plain pointers and sizes only.
+ *
+ * Conventions (the reference's own):
+ *   - states carry norm 2 and occupation 1 (gen_wave.C:699-710,
+ *     parse.C:283-284), so S = Psi^H Psi is about 2I and Ortho works on S/2;
+ *   - G-space coefficients are complex doubles, interleaved (re, im),
+ *     [nstates][npts] in the order of the stored k-vectors: the kx=0 plane
+ *     first (its redundant half ky<0 or ky=0,kz<0 included), points of one
+ *     (kx,ky) line contiguous with kz ascending
+ *     (stateSlab.C:285-380, util.C:1364-1397);
+ *   - real-space arrays are [Nz][Ny][Nx] doubles, x fastest;
+ *   - G->R is psi(r) = sum_G c(G) exp(-i G.r) unnormalised
+ *     (fftCache.C:1133-1243); the force from V is
+ *     F(G) = -(2/N) sum_r V psi exp(+i G.r), doubled off the kx=0 plane
+ *     (CP_State_GSpacePlane.C:1607-1612, stateSlab.C:240-272).
+ *
+ * Ownership: host pointers are borrowed for the duration of a call
+ * (copy-in / copy-out).  Device state belongs to the context.  A context is
+ * bound to one HIP stream and is not thread-safe; many contexts may share a
+ * process and a GPU.  Errors are status codes, never aborts; the message of
+ * the last error is available from oa_last_error().
+ */
+#ifndef OA_STEP_H
+#define OA_STEP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oa_ctx oa_ctx;
+
+typedef enum {
+  OA_OK = 0,
+  OA_EINVAL = 1,   /* bad argument or layout (replaces the CkExit layout checks) */
+  OA_EHIP = 2,     /* HIP runtime error */
+  OA_ENOMEM = 3,   /* device allocation failed */
+  OA_ENOCONV = 4,  /* Newton-Schulz missed its tolerance (ortho.C:129-133) */
+  OA_ESTATE = 5    /* call out of order (the reference's "Flow of Control" checks) */
+} oa_status;
+
+typedef enum { OA_FP64 = 0, OA_MIXED = 1 } oa_precision;
+
+typedef struct {
+  double hmat[9];      /* cell matrix, bohr, PINY flat order (hmat[1..9]) */
+  int grid[3];         /* Nx, Ny, Nz (state FFT grid) */
+  double ecut_ry;      /* wavefunction cutoff: |G|^2 <= ecut_ry */
+  int nstates;
+  const double *occ;   /* [nstates] occupations or NULL (all 1) */
+  int npts;            /* stored points; 0 => enumerate from ecut/hmat */
+  const int *kx, *ky, *kz; /* [npts] stored k-vectors or NULL */
+} oa_system;
+
+typedef struct {
+  int precision;       /* oa_precision */
+  int ns_max_iter;     /* invsqr_max_iter (ortho.h:86-87) */
+  double ns_tol;       /* invsqr_tolerance */
+  double dt;           /* SD time step, a.u. (gentimeinfo->dt) */
+  double tau_mass;     /* cp_mass_tau_def, fs (default 25) */
+  double ecut_mass;    /* cp_mass_cut_def (default 2) */
+  int use_lambda;      /* project F with Lambda (cp_min_opt = 1) */
+  int want_resid;      /* compute ||Psi^H Psi/2 - I|| after each step */
+} oa_options;
+
+typedef struct {
+  double eke;          /* kinetic energy (cp_eke.C:128) */
+  double eloc;         /* (Omega/N) sum_r V rho */
+  double eelec;        /* eke + eloc */
+  double fnorm;        /* sqrt(sum |F|^2) after projection */
+  int ns_iters;        /* Newton-Schulz iterations used */
+  double ns_err;       /* final rms change of T (ortho.C:117) */
+  double resid_max;    /* max |(Psi^H Psi)/2 - I| after the step (if asked) */
+  double resid_rms;
+  double ms;           /* device time of the step, ms */
+} oa_step_out;
+
+/* default options: FP64, ns_tol 1e-15, ns_max_iter 30, dt 5, tau 25 fs,
+ * ecut_mass 2, Lambda on, residual on */
+void oa_default_options(oa_options *opt);
+
+/* Stored k-vectors of a cell and cutoff in the reference order:
+ * setkvec3d_sm (set_cp_ewald.C:523-680) + flip_data_set
+ * (para_grp_parse.C:178-285).  Pass NULL arrays to get the count. */
+int oa_kvecs(double ecut_ry, const double hmat[9], int *kx, int *ky, int *kz);
+
+/* Build a context: layout, FFT plans, device buffers.
+ * Replaces the FFTcache constructor (fftCache.C:37-449), the
+ * GStateSlab/RealStateSlab allocation (stateSlab.C:33-106) and
+ * create_line_decomp_descriptor (util.C:1650-1930). */
+oa_status oa_create(int device, const oa_system *sys, const oa_options *opt, oa_ctx **out);
+oa_status oa_destroy(oa_ctx *ctx);
+const char *oa_last_error(const oa_ctx *ctx);
+
+/* sizes of the context's layout: npts, nlines, nplane_x, nkx0 */
+oa_status oa_layout_info(const oa_ctx *ctx, int info[4]);
+/* the stored k-vectors in use */
+oa_status oa_get_kvecs(const oa_ctx *ctx, int *kx, int *ky, int *kz);
+
+/* coefficients in/out, [nstates][npts] complex (GStateSlab::packedPlaneData) */
+oa_status oa_psi_set(oa_ctx *ctx, const double *psi);
+oa_status oa_psi_get(oa_ctx *ctx, double *psi);
+/* V_KS(r) in [Nz][Ny][Nx] (the VksMsg payload, CP_State_RealSpacePlane.C:530-597) */
+oa_status oa_vks_set(oa_ctx *ctx, const double *v);
+
+/* G->R for all states + density.  Replaces doStpFFTGtoR_Gchare,
+ * doStpFFTGtoR_Rchare (fftCache.C:1133-1243), the G->R transpose
+ * (CP_State_GSpacePlane.C:1449-1518, CP_State_RealSpacePlane.C:186-271)
+ * and the density reduction (CP_State_RealSpacePlane.C:349-355, 450-517;
+ * CP_Rho_RealSpacePlane.C:398-447, 630-650). */
+oa_status oa_fft_g2r(oa_ctx *ctx);
+/* psi_s(r) of one state (RealStateSlab plane data) */
+oa_status oa_psir_get(oa_ctx *ctx, int state, double *psir);
+/* rho(r) [Nz][Ny][Nx] */
+oa_status oa_density_get(oa_ctx *ctx, double *rho);
+
+/* V psi + R->G + EKE for all states.  Replaces unpackVks
+ * (CP_State_RealSpacePlane.C:530-597), doStpFFTRtoG_Rchare/_Gchare
+ * (fftCache.C:1099-1123, 1252-1319), doIFFT (CP_State_GSpacePlane.C:1582-1627),
+ * GStateSlab::addForces (stateSlab.C:240-272) and CPNONLOCAL::CP_eke_calc
+ * (cp_eke.C:30-140).  *eke may be NULL. */
+oa_status oa_vpsi_r2g(oa_ctx *ctx, double *eke);
+oa_status oa_force_get(oa_ctx *ctx, double *f);
+oa_status oa_force_set(oa_ctx *ctx, const double *f);
+
+/* Lambda = Re Psi^H F and F -= w Psi Lambda (asymmetric PairCalculator
+ * forward/backward, ckPairCalculator.C:682-802, 1189-1510;
+ * CP_State_GSpacePlane.C:2045-2225).  lambda may be NULL. */
+oa_status oa_lambda(oa_ctx *ctx, double *lambda);
+
+/* psi += dt/m F and the kx=0 redundancy sync
+ * (CPINTEGRATE::CP_integrate_min_STD, cp_min_std.C:56-86; CP_create_mass,
+ * cp_control_integrate.C:136-203; CP_State_GSpacePlane.C:2862-3061) */
+oa_status oa_sd_update(oa_ctx *ctx, double *fnorm);
+
+/* S = Psi^H Psi (reference scaling, ~2I).  Replaces sendPsi's kx=0 scaling
+ * (CP_State_GSpacePlane.C:3118-3125) + symmetric
+ * PairCalculator::multiplyForward (ckPairCalculator.C:682-802) and the
+ * Ortho tile assembly (ortho.C:153-268).  S may be NULL. */
+oa_status oa_overlap(oa_ctx *ctx, double *S);
+
+/* T = (S/2)^{-1/2} by Newton-Schulz on the S of the last oa_overlap.
+ * Replaces Ortho::do_iteration/step_2/step_3/tolerance_check/collect_error
+ * (ortho.C:109-146, 862-985) and CPORTHOG::CP_orthoTransform
+ * (cp_lowdin.C:24-46, 180-265).  OA_ENOCONV if max_iter is reached. */
+oa_status oa_ortho_ns(oa_ctx *ctx, double tol, int max_iter, int *iters, double *err, double *T);
+/* load a T for oa_rotate (the PairCalculator's orthoT input) */
+oa_status oa_tmat_set(oa_ctx *ctx, const double *T);
+
+/* Psi <- Psi T.  Replaces PairCalculator::bwMultiplyHelper
+ * (ckPairCalculator.C:1189-1425) and unpackNewPsi/doNewPsi
+ * (CP_State_GSpacePlane.C:3185-3447). */
+oa_status oa_rotate(oa_ctx *ctx);
+
+/* One minimisation step in the order of GSpaceDriver (gspace.ci:106-190):
+ * G->R, rho, V psi, R->G, EKE and kinetic force, [Lambda], SD update,
+ * redundancy sync, S, Newton-Schulz, Psi T.  out may be NULL (no host
+ * synchronisation); oa_step_result() fetches it later. */
+oa_status oa_step(oa_ctx *ctx, oa_step_out *out);
+oa_status oa_sync(oa_ctx *ctx);
+oa_status oa_step_result(oa_ctx *ctx, oa_step_out *out);
+
+/* Device time (ms) of the last oa_step once oa_set_timing(ctx, 1), from HIP
+ * events on the context's stream: [0] z gather+FFT, [1] y-FFT, [2] x c2r +
+ * rho, [3] V psi + x r2c, [4] y-FFT back, [5] z-FFT + pack + kinetic force,
+ * [6] EKE/E_loc reductions + Lambda, [7] SD + redundancy sync, [8] S,
+ * [9] Newton-Schulz, [10] Psi T, [11] residual, [12] whole step.
+ * Fills min(n, OA_NTIMINGS) entries. */
+#define OA_NTIMINGS 13
+oa_status oa_timings(oa_ctx *ctx, double *ms, int n);
+oa_status oa_set_timing(oa_ctx *ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,120 @@
+"""ctypes binding of liboa_step.so (the C ABI declared in include/oa_step.h).
+
+The library is the product: there is no fallback.  If it has not been built
+the import of this module still succeeds, but every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboa_step.so")
+
+OA_OK, OA_EINVAL, OA_EHIP, OA_ENOMEM, OA_ENOCONV, OA_ESTATE = range(6)
+OA_FP64, OA_MIXED = 0, 1
+STATUS_NAMES = {0: "OA_OK", 1: "OA_EINVAL", 2: "OA_EHIP", 3: "OA_ENOMEM",
+                4: "OA_ENOCONV", 5: "OA_ESTATE"}
+
+# every symbol include/oa_step.h declares
+EXPORTS = (
+    "oa_default_options", "oa_kvecs", "oa_create", "oa_destroy", "oa_last_error",
+    "oa_layout_info", "oa_get_kvecs", "oa_psi_set", "oa_psi_get", "oa_vks_set",
+    "oa_fft_g2r", "oa_psir_get", "oa_density_get", "oa_vpsi_r2g", "oa_force_get",
+    "oa_force_set", "oa_lambda", "oa_sd_update", "oa_overlap", "oa_ortho_ns",
+    "oa_tmat_set", "oa_rotate", "oa_step", "oa_sync", "oa_step_result", "oa_timings",
+    "oa_set_timing",
+)
+
+
+class OaSystem(ctypes.Structure):
+    _fields_ = [("hmat", ctypes.c_double * 9), ("grid", ctypes.c_int * 3),
+                ("ecut_ry", ctypes.c_double), ("nstates", ctypes.c_int),
+                ("occ", ctypes.c_void_p), ("npts", ctypes.c_int),
+                ("kx", ctypes.c_void_p), ("ky", ctypes.c_void_p), ("kz", ctypes.c_void_p)]
+
+
+class OaOptions(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("ns_max_iter", ctypes.c_int),
+                ("ns_tol", ctypes.c_double), ("dt", ctypes.c_double),
+                ("tau_mass", ctypes.c_double), ("ecut_mass", ctypes.c_double),
+                ("use_lambda", ctypes.c_int), ("want_resid", ctypes.c_int)]
+
+
+class OaStepOut(ctypes.Structure):
+    _fields_ = [("eke", ctypes.c_double), ("eloc", ctypes.c_double),
+                ("eelec", ctypes.c_double), ("fnorm", ctypes.c_double),
+                ("ns_iters", ctypes.c_int), ("ns_err", ctypes.c_double),
+                ("resid_max", ctypes.c_double), ("resid_rms", ctypes.c_double),
+                ("ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class OaError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+_lib = None
+
+
+def load():
+    """Load liboa_step.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with "
+                           "`make -C openatom_amd/csrc` (or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    pd = ctypes.POINTER(ctypes.c_double)
+    L.oa_default_options.argtypes = [ctypes.POINTER(OaOptions)]
+    L.oa_default_options.restype = None
+    L.oa_kvecs.argtypes = [cd, pd, vp, vp, vp]
+    L.oa_kvecs.restype = ci
+    L.oa_create.argtypes = [ci, ctypes.POINTER(OaSystem), ctypes.POINTER(OaOptions),
+                            ctypes.POINTER(vp)]
+    L.oa_last_error.argtypes = [vp]
+    L.oa_last_error.restype = ctypes.c_char_p
+    for name, args in (("oa_destroy", [vp]), ("oa_layout_info", [vp, vp]),
+                       ("oa_get_kvecs", [vp, vp, vp, vp]), ("oa_psi_set", [vp, vp]),
+                       ("oa_psi_get", [vp, vp]), ("oa_vks_set", [vp, vp]),
+                       ("oa_fft_g2r", [vp]), ("oa_psir_get", [vp, ci, vp]),
+                       ("oa_density_get", [vp, vp]), ("oa_vpsi_r2g", [vp, vp]),
+                       ("oa_force_get", [vp, vp]), ("oa_force_set", [vp, vp]),
+                       ("oa_lambda", [vp, vp]), ("oa_sd_update", [vp, vp]),
+                       ("oa_overlap", [vp, vp]),
+                       ("oa_ortho_ns", [vp, cd, ci, vp, vp, vp]),
+                       ("oa_tmat_set", [vp, vp]), ("oa_rotate", [vp]),
+                       ("oa_step", [vp, vp]), ("oa_sync", [vp]),
+                       ("oa_step_result", [vp, vp]), ("oa_timings", [vp, vp, ci]),
+                       ("oa_set_timing", [vp, ci])):
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ci
+    _lib = L
+    return L
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def default_options():
+    o = OaOptions()
+    load().oa_default_options(ctypes.byref(o))
+    return o
+
+
+def kvecs(ecut_ry, hmat):
+    """Stored k-vectors (reference order) computed by the library."""
+    L = load()
+    h = (ctypes.c_double * 9)(*np.asarray(hmat, dtype=np.float64).reshape(9))
+    n = L.oa_kvecs(float(ecut_ry), h, None, None, None)
+    kx = np.zeros(n, np.int32); ky = np.zeros(n, np.int32); kz = np.zeros(n, np.int32)
+    L.oa_kvecs(float(ecut_ry), h, ptr(kx), ptr(ky), ptr(kz))
+    return kx, ky, kz

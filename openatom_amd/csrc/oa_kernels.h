@@ -1,0 +1,111 @@
+// oa_kernels.h -- host-side launch interface of the device kernels.
+#pragma once
+
+#include "oa_common.h"
+
+namespace oa {
+
+// Everything the FFT chain needs for one system (device pointers).
+struct FftCtx {
+  Geom g;
+  FftPlan px, py, pz;
+  int lpb;    // z-lines per workgroup in the z passes
+  int kc;     // kx columns per workgroup in the y passes
+  int npair;  // pairs of real rows per workgroup in the x passes
+  const double2* psi;     // [nst][ld] stored coefficients (FP64 master copy)
+  double2* force;         // [nst][ld]
+  const int* blk_pt0;     // [nblk+1] first point of each z-line block
+  const int* pt_code;     // [npts] (kx0 << 31) | (line_in_block << 12) | z
+  const int* line_dst;    // [nlines] kx * ny + (ky mod ny)
+  const int* kylo;        // [nplane_x] populated ky range per kx column
+  const int* kyhi;
+  const double2 *twx, *twy, *twz;  // exp(-2 pi i j / n)
+  const double* g2;       // [npts] |G|^2
+  const double* occ;      // [nst]
+  const double* vks;      // [N] V(r)
+  void* zbuf;             // [nst][nz][nplane_x][ny] complex T
+  void* xbuf;             // [nst][nz][ny][nplane_x] complex T
+  void* psir;             // [nst][nz][ny][nx] T
+  double* rho;            // [N]
+  double* eke_part;       // [nst][nblk]
+  double inv_vol, ecut_ry, scale;
+};
+
+// stage: 0 z-gather+FFT, 1 y-FFT fwd, 2 x c2r + rho, 3 V psi + x r2c,
+//        4 y-FFT bwd, 5 z-FFT + pack + kinetic force / EKE partials
+enum { FFT_ZFWD = 0, FFT_YFWD, FFT_XC2R, FFT_XR2C, FFT_YBWD, FFT_ZBWD, FFT_NSTAGE };
+hipError_t fft_stage(const FftCtx& c, bool fp32, int stage, hipStream_t st);
+size_t fft_max_smem(const FftCtx& c, bool fp32);
+
+// Device-resident per-step results (copied to the host once per step).
+struct StepDev {
+  double eke, eloc, fnorm2;
+  double ns_err;
+  int ns_iter;      // >0 converged at this iteration, <0 not converged
+  int pad;
+  double resid_max, resid_rms;
+};
+
+// ---- dense linear algebra (oa_gemm.hip) ----
+// part[sp][i][j] = sum_{k in slice sp} w(k) A[i][k] B[j][k]; A, B are [n][lda]
+// rows of K reals; w(k) = w_lo for k < k_lo else w_hi.  sym: B == A, only
+// upper tiles computed.
+struct GramArgs {
+  const double* A;
+  const double* B;
+  int n, lda, K, k_lo;
+  double w_lo, w_hi;
+  int sym;
+  int kslice;
+  double* part;
+};
+int gram_nsplit(int K, int kslice);
+hipError_t gram_f64(const GramArgs& a, hipStream_t st);
+// C = alpha * sum_sp part[sp]; mirrored from the upper triangle if sym
+hipError_t gram_reduce(const double* part, int nsplit, int n, int sym, double alpha,
+                       double* C, hipStream_t st);
+
+// Y[j][k] = beta * Y[j][k] + s(k) * sum_i M[i][j] X[i][k]; s(k) = s_lo for
+// k < k_lo else s_hi.  X, Y: [n][ldx] (Y may not alias X).
+struct RotArgs {
+  const double* X;
+  const double* M;
+  double* Y;
+  int n, ldx, K, k_lo;
+  double s_lo, s_hi, beta;
+};
+hipError_t rotate_f64(const RotArgs& a, hipStream_t st);
+
+// Newton-Schulz for T = (S/2)^{-1/2} with device-side convergence test
+// (ortho.C:862-985).  Work: X[2], T[2] (n*n each), errpart[(max_iter+1) * nwg].
+struct NsArgs {
+  const double* S;
+  double* X0; double* X1;
+  double* T0; double* T1;
+  double* C;
+  double* Tout;
+  double* errpart;
+  StepDev* res;
+  int n, max_iter;
+  double tol;
+};
+hipError_t ns_run(const NsArgs& a, hipStream_t st);
+
+// ---- pointwise G-space kernels (oa_gemm.hip) ----
+// psi += dtm[p] * F (cp_min_std.C:56-86); accumulates |F|^2 partials
+hipError_t sd_update(double2* psi, const double2* f, const double* dtm, int nst, int npts,
+                     int ld, double* fpart, int nfpart, hipStream_t st);
+// redundant kx=0 points <- conj(partner) (CP_State_GSpacePlane.C:2862-3061)
+hipError_t red_sync(double2* psi, const int* red, const int* partner, int nred, int nst, int ld,
+                    hipStream_t st);
+// sum of parts -> res field (deterministic single-workgroup reduction)
+hipError_t reduce_to(const double* part, int n, double scale, double* out, hipStream_t st);
+// sum_r V rho partials
+hipError_t dot_parts(const double* a, const double* b, size_t n, double* part, int nparts,
+                     hipStream_t st);
+// max|C/2 - I| and rms over an n*n matrix -> res->resid_max/resid_rms
+hipError_t resid_stats(const double* S, int n, StepDev* res, hipStream_t st);
+// zero-fill (graph safe replacement for memset)
+hipError_t fill_zero(void* p, size_t bytes, hipStream_t st);
+
+}  // namespace oa

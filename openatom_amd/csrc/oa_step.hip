@@ -1,0 +1,914 @@
+// oa_step.hip -- context, layout and the C ABI of the CP electronic-step
+// library.  Host code only; the kernels live in oa_fft.hip and oa_gemm.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "oa_kernels.h"
+#include "oa_step.h"
+
+using oa::FftCtx;
+using oa::StepDev;
+
+namespace {
+
+constexpr double kTimeConv = 0.0241888;  // piny_constants.h:58 (fs -> a.u.)
+constexpr double kCpEmagic = 0.00050;    // piny_constants.h:59
+
+double inv3(const double h[9], double hi[9]) {
+  const double det = h[0] * (h[4] * h[8] - h[5] * h[7]) - h[1] * (h[3] * h[8] - h[5] * h[6]) +
+                     h[2] * (h[3] * h[7] - h[4] * h[6]);
+  const double id = 1.0 / det;
+  hi[0] = (h[4] * h[8] - h[5] * h[7]) * id;
+  hi[1] = (h[2] * h[7] - h[1] * h[8]) * id;
+  hi[2] = (h[1] * h[5] - h[2] * h[4]) * id;
+  hi[3] = (h[5] * h[6] - h[3] * h[8]) * id;
+  hi[4] = (h[0] * h[8] - h[2] * h[6]) * id;
+  hi[5] = (h[2] * h[3] - h[0] * h[5]) * id;
+  hi[6] = (h[3] * h[7] - h[4] * h[6]) * id;
+  hi[7] = (h[1] * h[6] - h[0] * h[7]) * id;
+  hi[8] = (h[0] * h[4] - h[1] * h[3]) * id;
+  return det;
+}
+
+// |G|^2 for integer (a,b,c): G = 2 pi (a hi[0..2], ...) as in set_cp_ewald.C:563-566
+double gsq(const double* hi, int a, int b, int c) {
+  const double t = 2.0 * M_PI;
+  const double gx = (a * hi[0] + b * hi[1] + c * hi[2]) * t;
+  const double gy = (a * hi[3] + b * hi[4] + c * hi[5]) * t;
+  const double gz = (a * hi[6] + b * hi[7] + c * hi[8]) * t;
+  return gx * gx + gy * gy + gz * gz;
+}
+
+// Half-sphere enumeration of setkvec3d_sm (set_cp_ewald.C:523-680) with the
+// kmax of calc_cutoff (set_cp_ewald.C:176-197), then the kx=0 mirror of
+// flip_data_set (para_grp_parse.C:178-285).  The half-sphere test
+// |G|^2/2 <= ecut_Ha is |G|^2 <= ecut_ry.
+void enumerate_kvecs(double ecut_ry, const double hmat[9], std::vector<int>& kx,
+                     std::vector<int>& ky, std::vector<int>& kz) {
+  double hi[9];
+  inv3(hmat, hi);
+  int kmax[3];
+  for (int d = 0; d < 3; d++) {
+    const double len = std::sqrt(hi[d] * hi[d] + hi[d + 3] * hi[d + 3] + hi[d + 6] * hi[d + 6]);
+    const double t = std::sqrt(0.25 * ecut_ry) / (M_PI * len);
+    int it = (int)t;
+    if (t - it > 0.99) it++;
+    kmax[d] = it;
+  }
+  auto inside = [&](int a, int b, int c) { return gsq(hi, a, b, c) <= ecut_ry; };
+  std::vector<int> ha, hb, hc;
+  int amax = 0;
+  while (amax + 1 <= kmax[0] && inside(amax + 1, 0, 0)) amax++;
+  for (int a = 0; a <= amax; a++) {
+    int bmin = (a == 0) ? 0 : -kmax[1];
+    while (bmin < 0 && !inside(a, bmin, 0)) bmin++;
+    int bmax = 0;
+    while (bmax + 1 <= kmax[1] && inside(a, bmax + 1, 0)) bmax++;
+    for (int b = bmin; b <= bmax; b++) {
+      int cmin = (a == 0 && b == 0) ? 1 : -kmax[2];
+      while (cmin < 0 && !inside(a, b, cmin)) cmin++;
+      int cmax = 0;
+      while (cmax + 1 <= kmax[2] && inside(a, b, cmax + 1)) cmax++;
+      for (int c = cmin; c <= cmax; c++) {
+        ha.push_back(a);
+        hb.push_back(b);
+        hc.push_back(c);
+      }
+    }
+  }
+  // the G=0 term is appended after the half sphere; the mirror of the kx=0
+  // half plane is prepended in reverse order, then G=0, then the rest
+  int n0 = 0;
+  while (n0 < (int)ha.size() && ha[n0] == 0) n0++;
+  kx.clear();
+  ky.clear();
+  kz.clear();
+  for (int i = n0 - 1; i >= 0; i--) {
+    kx.push_back(0);
+    ky.push_back(-hb[i]);
+    kz.push_back(-hc[i]);
+  }
+  kx.push_back(0);
+  ky.push_back(0);
+  kz.push_back(0);
+  for (size_t i = 0; i < ha.size(); i++) {
+    kx.push_back(ha[i]);
+    ky.push_back(hb[i]);
+    kz.push_back(hc[i]);
+  }
+}
+
+bool make_plan(int n, FftPlan& p) {
+  p.n = n;
+  p.npass = 0;
+  int m = n;
+  const int order[4] = {4, 5, 3, 2};
+  for (int r : order)
+    while (m % r == 0) {
+      if (p.npass >= OA_MAXPASS) return false;
+      p.radix[p.npass++] = r;
+      m /= r;
+    }
+  return m == 1 && n > 1;
+}
+
+int imod(int a, int n) {
+  int r = a % n;
+  return r < 0 ? r + n : r;
+}
+
+}  // namespace
+
+struct oa_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  oa_options opt{};
+  bool fp32 = false;
+  int nx = 0, ny = 0, nz = 0, nst = 0, npts = 0, ld = 0, nlines = 0, nplane_x = 0, nkx0 = 0,
+      nred = 0;
+  double hmat[9]{}, hmati[9]{}, vol = 0.0, ecut_ry = 0.0;
+  std::vector<int> kx, ky, kz;
+  // device
+  std::vector<void*> allocs;
+  double2 *psi = nullptr, *psi2 = nullptr, *force = nullptr;
+  int *blk_pt0 = nullptr, *pt_code = nullptr, *line_dst = nullptr, *kylo = nullptr,
+      *kyhi = nullptr, *red = nullptr, *partner = nullptr;
+  double2 *twx = nullptr, *twy = nullptr, *twz = nullptr;
+  double *g2 = nullptr, *occ = nullptr, *dtm = nullptr, *vks = nullptr, *rho = nullptr;
+  void *zbuf = nullptr, *xbuf = nullptr, *psir = nullptr;
+  double* eke_part = nullptr;
+  int nblk = 0;
+  double *S = nullptr, *T = nullptr, *lam = nullptr, *gpart = nullptr, *S2 = nullptr;
+  int gsplit_max = 0, kslice = 0;
+  double *nsX0 = nullptr, *nsX1 = nullptr, *nsT0 = nullptr, *nsT1 = nullptr, *nsC = nullptr,
+         *errpart = nullptr;
+  double *fpart = nullptr, *dpart = nullptr;
+  int nfpart = 0, ndpart = 0;
+  StepDev* res = nullptr;
+  StepDev* res_host = nullptr;
+  FftCtx fc{};
+  bool timing = false;
+  // ring of per-step event sets so a run of steps can be timed without host
+  // synchronisation inside it
+  static constexpr int kRing = 64;
+  std::vector<hipEvent_t> ev;  // [kRing][OA_NTIMINGS] once timing is enabled
+  int ev_cur = 0, ev_count = 0;
+  bool have_S = false, have_T = false, have_v = false;
+};
+
+namespace {
+
+oa_status fail(oa_ctx* c, oa_status s, const std::string& m) {
+  if (c) c->err = m;
+  return s;
+}
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(c, OA_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+template <typename P>
+oa_status dalloc(oa_ctx* c, P** p, size_t bytes) {
+  void* q = nullptr;
+  if (bytes == 0) bytes = 16;
+  if (hipMalloc(&q, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, OA_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  c->allocs.push_back(q);
+  *p = reinterpret_cast<P*>(q);
+  return OA_OK;
+}
+
+#define TRY(expr)                     \
+  do {                                \
+    oa_status s_ = (expr);            \
+    if (s_ != OA_OK) return s_;       \
+  } while (0)
+
+oa_status upload(oa_ctx* c, void* dst, const void* src, size_t bytes) {
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return OA_OK;
+}
+
+std::vector<double2> twiddles(int n) {
+  std::vector<double2> t(n);
+  for (int j = 0; j < n; j++) {
+    const double a = 2.0 * M_PI * (double)j / (double)n;
+    t[j] = make_double2(std::cos(a), -std::sin(a));
+  }
+  return t;
+}
+
+// Validate the stored layout and build the device tables.
+oa_status build_layout(oa_ctx* c) {
+  const int n = c->npts;
+  const std::vector<int>&kx = c->kx, &ky = c->ky, &kz = c->kz;
+  // kx = 0 plane first (stateSlab.C:325-340), kx >= 0 (Gamma half space)
+  int nkx0 = 0;
+  while (nkx0 < n && kx[nkx0] == 0) nkx0++;
+  for (int i = nkx0; i < n; i++)
+    if (kx[i] <= 0) return fail(c, OA_EINVAL, "kx=0 points must come first and kx >= 0");
+  c->nkx0 = nkx0;
+  // lines: runs of constant (kx,ky), kz consecutive (util.C:1364-1397)
+  std::vector<int> loff, llen, lkx, lky, lkz0;
+  std::map<std::pair<int, int>, int> seen;
+  int kxmax = 0;
+  for (int i = 0; i < n; i++) {
+    if (i == 0 || kx[i] != kx[i - 1] || ky[i] != ky[i - 1]) {
+      if (seen.count({kx[i], ky[i]})) return fail(c, OA_EINVAL, "line (kx,ky) split in two runs");
+      seen[{kx[i], ky[i]}] = (int)loff.size();
+      loff.push_back(i);
+      llen.push_back(0);
+      lkx.push_back(kx[i]);
+      lky.push_back(ky[i]);
+      lkz0.push_back(kz[i]);
+    } else if (kz[i] != kz[i - 1] + 1) {
+      return fail(c, OA_EINVAL, "kz must ascend by one along a line");
+    }
+    llen.back()++;
+    kxmax = std::max(kxmax, kx[i]);
+    if (2 * std::abs(ky[i]) >= c->ny || 2 * std::abs(kz[i]) >= c->nz)
+      return fail(c, OA_EINVAL, "k-vector outside the FFT grid");
+  }
+  c->nlines = (int)loff.size();
+  c->nplane_x = kxmax + 1;
+  if (2 * c->nplane_x > c->nx) return fail(c, OA_EINVAL, "kx range too large for the x grid");
+  if (c->nz >= 4096) return fail(c, OA_EINVAL, "nz >= 4096 unsupported");
+  // populated ky range per kx column must be contiguous
+  std::vector<int> lo(c->nplane_x, 1 << 30), hi(c->nplane_x, -(1 << 30)), cnt(c->nplane_x, 0);
+  for (int l = 0; l < c->nlines; l++) {
+    lo[lkx[l]] = std::min(lo[lkx[l]], lky[l]);
+    hi[lkx[l]] = std::max(hi[lkx[l]], lky[l]);
+    cnt[lkx[l]]++;
+  }
+  for (int x = 0; x < c->nplane_x; x++) {
+    if (cnt[x] == 0) {
+      lo[x] = 1;
+      hi[x] = 0;
+    } else if (hi[x] - lo[x] + 1 != cnt[x]) {
+      return fail(c, OA_EINVAL, "populated ky range of a kx column is not contiguous");
+    }
+  }
+  // redundant kx=0 points and their partners (CP_State_GSpacePlane.C:2862-3061)
+  std::map<std::pair<int, int>, int> kx0idx;
+  for (int i = 0; i < nkx0; i++) kx0idx[{ky[i], kz[i]}] = i;
+  std::vector<int> red, partner;
+  for (int i = 0; i < nkx0; i++) {
+    if (ky[i] < 0 || (ky[i] == 0 && kz[i] < 0)) {
+      auto it = kx0idx.find({-ky[i], -kz[i]});
+      if (it == kx0idx.end()) return fail(c, OA_EINVAL, "redundant kx=0 point without partner");
+      red.push_back(i);
+      partner.push_back(it->second);
+    }
+  }
+  c->nred = (int)red.size();
+  // launch shapes
+  FftCtx& f = c->fc;
+  if (!make_plan(c->nx, f.px) || !make_plan(c->ny, f.py) || !make_plan(c->nz, f.pz))
+    return fail(c, OA_EINVAL, "FFT sizes must be products of 2, 3, 5 (<= 8 passes)");
+  f.lpb = std::max(1, 1024 / c->nz);
+  {
+    const int nch = std::max(1, (c->nplane_x * c->ny + 1023) / 1024);
+    f.kc = (c->nplane_x + nch - 1) / nch;
+  }
+  f.npair = std::max(1, std::min(OA_XSLOTS * OA_NTHR, 512) / c->nx);
+  if (f.npair * c->nx > OA_XSLOTS * OA_NTHR) return fail(c, OA_EINVAL, "nx too large");
+  c->nblk = (c->nlines + f.lpb - 1) / f.lpb;
+  std::vector<int> blk_pt0(c->nblk + 1), pt_code(n), line_dst(c->nlines);
+  for (int b = 0; b < c->nblk; b++) blk_pt0[b] = loff[b * f.lpb];
+  blk_pt0[c->nblk] = n;
+  for (int l = 0; l < c->nlines; l++) {
+    line_dst[l] = lkx[l] * c->ny + imod(lky[l], c->ny);
+    const int li = l % f.lpb;
+    for (int i = 0; i < llen[l]; i++) {
+      const int p = loff[l] + i;
+      const unsigned code = ((lkx[l] == 0) ? 0x80000000u : 0u) | ((unsigned)li << 12) |
+                            (unsigned)imod(lkz0[l] + i, c->nz);
+      pt_code[p] = (int)code;
+    }
+  }
+  std::vector<double> g2(n), dtm(n);
+  for (int i = 0; i < n; i++) g2[i] = gsq(c->hmati, kx[i], ky[i], kz[i]);
+  // dt / cmass with CP_create_mass (cp_control_integrate.C:136-203)
+  {
+    const double tau = c->opt.tau_mass / kTimeConv;
+    const double cmass0 = 2.0 * tau * tau * kCpEmagic;
+    const double ecm = c->opt.ecut_mass;
+    for (int i = 0; i < n; i++) {
+      const double w = (kx[i] == 0) ? 1.0 : 2.0;
+      const double m = (g2[i] > ecm) ? (w * g2[i]) * (cmass0 / ecm) : w * cmass0;
+      dtm[i] = c->opt.dt / m;
+    }
+  }
+  TRY(dalloc(c, &c->blk_pt0, sizeof(int) * blk_pt0.size()));
+  TRY(dalloc(c, &c->pt_code, sizeof(int) * n));
+  TRY(dalloc(c, &c->line_dst, sizeof(int) * c->nlines));
+  TRY(dalloc(c, &c->kylo, sizeof(int) * c->nplane_x));
+  TRY(dalloc(c, &c->kyhi, sizeof(int) * c->nplane_x));
+  TRY(dalloc(c, &c->red, sizeof(int) * std::max(1, c->nred)));
+  TRY(dalloc(c, &c->partner, sizeof(int) * std::max(1, c->nred)));
+  TRY(dalloc(c, &c->g2, sizeof(double) * n));
+  TRY(dalloc(c, &c->dtm, sizeof(double) * n));
+  TRY(upload(c, c->blk_pt0, blk_pt0.data(), sizeof(int) * blk_pt0.size()));
+  TRY(upload(c, c->pt_code, pt_code.data(), sizeof(int) * n));
+  TRY(upload(c, c->line_dst, line_dst.data(), sizeof(int) * c->nlines));
+  TRY(upload(c, c->kylo, lo.data(), sizeof(int) * c->nplane_x));
+  TRY(upload(c, c->kyhi, hi.data(), sizeof(int) * c->nplane_x));
+  if (c->nred) {
+    TRY(upload(c, c->red, red.data(), sizeof(int) * c->nred));
+    TRY(upload(c, c->partner, partner.data(), sizeof(int) * c->nred));
+  }
+  TRY(upload(c, c->g2, g2.data(), sizeof(double) * n));
+  TRY(upload(c, c->dtm, dtm.data(), sizeof(double) * n));
+  auto tx = twiddles(c->nx), ty = twiddles(c->ny), tz = twiddles(c->nz);
+  TRY(dalloc(c, &c->twx, sizeof(double2) * c->nx));
+  TRY(dalloc(c, &c->twy, sizeof(double2) * c->ny));
+  TRY(dalloc(c, &c->twz, sizeof(double2) * c->nz));
+  TRY(upload(c, c->twx, tx.data(), sizeof(double2) * c->nx));
+  TRY(upload(c, c->twy, ty.data(), sizeof(double2) * c->ny));
+  TRY(upload(c, c->twz, tz.data(), sizeof(double2) * c->nz));
+  return OA_OK;
+}
+
+oa_status alloc_state(oa_ctx* c, const double* occ) {
+  const size_t N = (size_t)c->nx * c->ny * c->nz;
+  const size_t gsz = (size_t)c->nst * c->ld * sizeof(double2);
+  const size_t rsz = c->fp32 ? 4 : 8;
+  const size_t csz = 2 * rsz;
+  TRY(dalloc(c, &c->psi, gsz));
+  TRY(dalloc(c, &c->psi2, gsz));
+  TRY(dalloc(c, &c->force, gsz));
+  TRY(dalloc(c, &c->zbuf, (size_t)c->nst * c->nz * c->nplane_x * c->ny * csz));
+  TRY(dalloc(c, &c->xbuf, (size_t)c->nst * c->nz * c->ny * c->nplane_x * csz));
+  TRY(dalloc(c, &c->psir, (size_t)c->nst * N * rsz));
+  TRY(dalloc(c, &c->rho, N * sizeof(double)));
+  TRY(dalloc(c, &c->vks, N * sizeof(double)));
+  TRY(dalloc(c, &c->occ, c->nst * sizeof(double)));
+  TRY(dalloc(c, &c->eke_part, (size_t)c->nst * c->nblk * sizeof(double)));
+  const int n = c->nst;
+  const size_t nn = (size_t)n * n;
+  TRY(dalloc(c, &c->S, nn * sizeof(double)));
+  TRY(dalloc(c, &c->S2, nn * sizeof(double)));
+  TRY(dalloc(c, &c->T, nn * sizeof(double)));
+  TRY(dalloc(c, &c->lam, nn * sizeof(double)));
+  // split-K shape for the gram: aim for >= 512 workgroups
+  const int K = 2 * c->npts;
+  const int ntile = (n + 63) / 64;
+  const int tiles = ntile * ntile;
+  int nsplit = std::max(1, 512 / std::max(1, ntile * (ntile + 1) / 2));
+  int ks = (K + nsplit - 1) / nsplit;
+  ks = std::max(64, ((ks + 15) / 16) * 16);
+  c->kslice = ks;
+  c->gsplit_max = oa::gram_nsplit(K, ks);
+  TRY(dalloc(c, &c->gpart, (size_t)c->gsplit_max * nn * sizeof(double)));
+  (void)tiles;
+  TRY(dalloc(c, &c->nsX0, nn * sizeof(double)));
+  TRY(dalloc(c, &c->nsX1, nn * sizeof(double)));
+  TRY(dalloc(c, &c->nsT0, nn * sizeof(double)));
+  TRY(dalloc(c, &c->nsT1, nn * sizeof(double)));
+  TRY(dalloc(c, &c->nsC, nn * sizeof(double)));
+  const int nt = (n + 31) / 32;
+  TRY(dalloc(c, &c->errpart, (size_t)(c->opt.ns_max_iter + 2) * nt * nt * sizeof(double)));
+  c->nfpart = 1024;
+  c->ndpart = 512;
+  TRY(dalloc(c, &c->fpart, c->nfpart * sizeof(double)));
+  TRY(dalloc(c, &c->dpart, c->ndpart * sizeof(double)));
+  TRY(dalloc(c, &c->res, sizeof(StepDev)));
+  HIPCHK(hipHostMalloc((void**)&c->res_host, sizeof(StepDev), hipHostMallocDefault));
+  std::memset(c->res_host, 0, sizeof(StepDev));
+  HIPCHK(oa::fill_zero(c->psi, gsz, c->stream));
+  HIPCHK(oa::fill_zero(c->psi2, gsz, c->stream));
+  HIPCHK(oa::fill_zero(c->force, gsz, c->stream));
+  HIPCHK(oa::fill_zero(c->vks, N * sizeof(double), c->stream));
+  HIPCHK(oa::fill_zero(c->res, sizeof(StepDev), c->stream));
+  std::vector<double> o(c->nst, 1.0);
+  if (occ)
+    for (int i = 0; i < c->nst; i++) o[i] = occ[i];
+  TRY(upload(c, c->occ, o.data(), sizeof(double) * c->nst));
+  // FFT context
+  FftCtx& f = c->fc;
+  f.g = {c->nx, c->ny, c->nz, c->nplane_x, c->nlines, c->npts, c->ld, c->nst};
+  f.blk_pt0 = c->blk_pt0;
+  f.pt_code = c->pt_code;
+  f.line_dst = c->line_dst;
+  f.kylo = c->kylo;
+  f.kyhi = c->kyhi;
+  f.twx = c->twx;
+  f.twy = c->twy;
+  f.twz = c->twz;
+  f.g2 = c->g2;
+  f.occ = c->occ;
+  f.vks = c->vks;
+  f.zbuf = c->zbuf;
+  f.xbuf = c->xbuf;
+  f.psir = c->psir;
+  f.rho = c->rho;
+  f.eke_part = c->eke_part;
+  f.inv_vol = 1.0 / c->vol;
+  f.ecut_ry = c->ecut_ry;
+  f.scale = -2.0 / ((double)c->nx * c->ny * c->nz);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return OA_OK;
+}
+
+FftCtx& fctx(oa_ctx* c) {
+  c->fc.psi = c->psi;
+  c->fc.force = c->force;
+  return c->fc;
+}
+
+// Timing marks inside oa_step: ev[0] start, ev[1..6] after each FFT stage,
+// ev[7] reductions + Lambda, ev[8] SD + sync, ev[9] S, ev[10] NS,
+// ev[11] Psi T, ev[12] residual (end).
+void mark(oa_ctx* c, int i) {
+  if (c->timing && i >= 0)
+    (void)hipEventRecord(c->ev[(size_t)c->ev_cur * OA_NTIMINGS + i], c->stream);
+}
+
+// ---- stages (enqueue only) ----
+oa_status st_g2r(oa_ctx* c, int ev0 = -1) {
+  for (int s = oa::FFT_ZFWD; s <= oa::FFT_XC2R; s++) {
+    HIPCHK(oa::fft_stage(fctx(c), c->fp32, s, c->stream));
+    mark(c, ev0 < 0 ? -1 : ev0 + s);
+  }
+  return OA_OK;
+}
+
+oa_status st_r2g(oa_ctx* c, int ev0 = -1) {
+  for (int s = oa::FFT_XR2C; s <= oa::FFT_ZBWD; s++) {
+    HIPCHK(oa::fft_stage(fctx(c), c->fp32, s, c->stream));
+    mark(c, ev0 < 0 ? -1 : ev0 + s);
+  }
+  HIPCHK(oa::reduce_to(c->eke_part, c->nst * c->nblk, 1.0, &c->res->eke, c->stream));
+  const size_t N = (size_t)c->nx * c->ny * c->nz;
+  HIPCHK(oa::dot_parts(c->vks, c->rho, N, c->dpart, c->ndpart, c->stream));
+  HIPCHK(oa::reduce_to(c->dpart, c->ndpart, c->vol / (double)N, &c->res->eloc, c->stream));
+  return OA_OK;
+}
+
+oa_status st_gram(oa_ctx* c, const double2* A, const double2* B, bool sym, double wlo, double whi,
+                  double* C) {
+  oa::GramArgs g;
+  g.A = reinterpret_cast<const double*>(A);
+  g.B = reinterpret_cast<const double*>(B);
+  g.n = c->nst;
+  g.lda = 2 * c->ld;
+  g.K = 2 * c->npts;
+  g.k_lo = 2 * c->nkx0;
+  g.w_lo = wlo;
+  g.w_hi = whi;
+  g.sym = sym ? 1 : 0;
+  g.kslice = c->kslice;
+  g.part = c->gpart;
+  HIPCHK(oa::gram_f64(g, c->stream));
+  HIPCHK(oa::gram_reduce(c->gpart, oa::gram_nsplit(g.K, g.kslice), c->nst, g.sym, 1.0, C,
+                         c->stream));
+  return OA_OK;
+}
+
+oa_status st_rotate(oa_ctx* c, const double2* X, const double* M, double2* Y, double beta,
+                    double slo, double shi) {
+  oa::RotArgs r;
+  r.X = reinterpret_cast<const double*>(X);
+  r.M = M;
+  r.Y = reinterpret_cast<double*>(Y);
+  r.n = c->nst;
+  r.ldx = 2 * c->ld;
+  r.K = 2 * c->npts;
+  r.k_lo = 2 * c->nkx0;
+  r.s_lo = slo;
+  r.s_hi = shi;
+  r.beta = beta;
+  HIPCHK(oa::rotate_f64(r, c->stream));
+  return OA_OK;
+}
+
+oa_status st_lambda(oa_ctx* c) {
+  // Lambda_ij = sum_stored Re conj(psi_i) F_j (alpha = 1, cpaimd.C:2527)
+  TRY(st_gram(c, c->psi, c->force, false, 1.0, 1.0, c->lam));
+  // F_j -= w sum_i psi_i Lambda_ij, w = 1/2 on kx=0 (CP_State_GSpacePlane.C:2076-2086)
+  TRY(st_rotate(c, c->psi, c->lam, c->force, 1.0, -0.5, -1.0));
+  return OA_OK;
+}
+
+oa_status st_sd(oa_ctx* c) {
+  HIPCHK(oa::sd_update(c->psi, c->force, c->dtm, c->nst, c->npts, c->ld, c->fpart, c->nfpart,
+                       c->stream));
+  HIPCHK(oa::reduce_to(c->fpart, c->nfpart, 1.0, &c->res->fnorm2, c->stream));
+  HIPCHK(oa::red_sync(c->psi, c->red, c->partner, c->nred, c->nst, c->ld, c->stream));
+  return OA_OK;
+}
+
+oa_status st_overlap(oa_ctx* c) {
+  // S = 2 sum (kx0/sqrt2)^2 ... = sum_p w_p Re conj(psi_i) psi_j, w = 1 on kx=0, 2 elsewhere
+  TRY(st_gram(c, c->psi, c->psi, true, 1.0, 2.0, c->S));
+  c->have_S = true;
+  return OA_OK;
+}
+
+oa_status st_ns(oa_ctx* c, double tol, int max_iter) {
+  if (max_iter > c->opt.ns_max_iter)
+    return fail(c, OA_EINVAL, "max_iter exceeds the context's ns_max_iter");
+  oa::NsArgs a;
+  a.S = c->S;
+  a.X0 = c->nsX0;
+  a.X1 = c->nsX1;
+  a.T0 = c->nsT0;
+  a.T1 = c->nsT1;
+  a.C = c->nsC;
+  a.Tout = c->T;
+  a.errpart = c->errpart;
+  a.res = c->res;
+  a.n = c->nst;
+  a.max_iter = max_iter;
+  a.tol = tol;
+  HIPCHK(oa::ns_run(a, c->stream));
+  c->have_T = true;
+  return OA_OK;
+}
+
+oa_status st_rot_psi(oa_ctx* c) {
+  TRY(st_rotate(c, c->psi, c->T, c->psi2, 0.0, 1.0, 1.0));
+  std::swap(c->psi, c->psi2);
+  return OA_OK;
+}
+
+oa_status fetch_result(oa_ctx* c, oa_step_out* out) {
+  HIPCHK(hipMemcpyAsync(c->res_host, c->res, sizeof(StepDev), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (out) {
+    const StepDev& r = *c->res_host;
+    out->eke = r.eke;
+    out->eloc = r.eloc;
+    out->eelec = r.eke + r.eloc;
+    out->fnorm = std::sqrt(r.fnorm2);
+    out->ns_iters = r.ns_iter < 0 ? -r.ns_iter : r.ns_iter;
+    out->ns_err = r.ns_err;
+    out->resid_max = r.resid_max;
+    out->resid_rms = r.resid_rms;
+    out->ms = 0.0;
+    if (c->timing) {
+      float ms = 0.f;
+      const size_t b = (size_t)c->ev_cur * OA_NTIMINGS;
+      if (hipEventElapsedTime(&ms, c->ev[b], c->ev[b + 12]) == hipSuccess) out->ms = ms;
+    }
+  }
+  return OA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void oa_default_options(oa_options* o) {
+  o->precision = OA_FP64;
+  o->ns_max_iter = 30;
+  o->ns_tol = 1e-15;
+  o->dt = 5.0;
+  o->tau_mass = 25.0;
+  o->ecut_mass = 2.0;
+  o->use_lambda = 1;
+  o->want_resid = 1;
+}
+
+int oa_kvecs(double ecut_ry, const double hmat[9], int* kx, int* ky, int* kz) {
+  std::vector<int> a, b, cc;
+  enumerate_kvecs(ecut_ry, hmat, a, b, cc);
+  if (kx && ky && kz) {
+    std::copy(a.begin(), a.end(), kx);
+    std::copy(b.begin(), b.end(), ky);
+    std::copy(cc.begin(), cc.end(), kz);
+  }
+  return (int)a.size();
+}
+
+oa_status oa_create(int device, const oa_system* sys, const oa_options* opt, oa_ctx** out) {
+  if (!sys || !out) return OA_EINVAL;
+  *out = nullptr;
+  oa_ctx* c = new oa_ctx();
+  auto bail = [&](oa_status s) {
+    std::fprintf(stderr, "oa_create: %s\n", c->err.c_str());
+    oa_destroy(c);
+    return s;
+  };
+  c->device = device;
+  if (opt) c->opt = *opt;
+  else oa_default_options(&c->opt);
+  if (c->opt.ns_max_iter < 1) c->opt.ns_max_iter = 1;
+  c->fp32 = (c->opt.precision == OA_MIXED);
+  c->nx = sys->grid[0];
+  c->ny = sys->grid[1];
+  c->nz = sys->grid[2];
+  c->nst = sys->nstates;
+  c->ecut_ry = sys->ecut_ry;
+  std::memcpy(c->hmat, sys->hmat, sizeof(c->hmat));
+  c->vol = std::fabs(inv3(c->hmat, c->hmati));
+  if (c->nx < 2 || c->ny < 2 || c->nz < 2 || c->nst < 1) {
+    c->err = "bad grid or state count";
+    return bail(OA_EINVAL);
+  }
+  if (sys->npts > 0) {
+    if (!sys->kx || !sys->ky || !sys->kz) {
+      c->err = "npts > 0 needs kx/ky/kz";
+      return bail(OA_EINVAL);
+    }
+    c->kx.assign(sys->kx, sys->kx + sys->npts);
+    c->ky.assign(sys->ky, sys->ky + sys->npts);
+    c->kz.assign(sys->kz, sys->kz + sys->npts);
+  } else {
+    enumerate_kvecs(c->ecut_ry, c->hmat, c->kx, c->ky, c->kz);
+  }
+  c->npts = (int)c->kx.size();
+  c->ld = ((c->npts + 31) / 32) * 32;
+  if (hipSetDevice(device) != hipSuccess) {
+    c->err = "hipSetDevice failed";
+    return bail(OA_EHIP);
+  }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    c->err = "stream creation failed";
+    return bail(OA_EHIP);
+  }
+  oa_status s = build_layout(c);
+  if (s != OA_OK) return bail(s);
+  s = alloc_state(c, sys->occ);
+  if (s != OA_OK) return bail(s);
+  const size_t need = oa::fft_max_smem(c->fc, c->fp32);
+  if (need > 64 * 1024) {
+    c->err = "FFT workgroup needs " + std::to_string(need) + " bytes of LDS";
+    return bail(OA_EINVAL);
+  }
+  *out = c;
+  return OA_OK;
+}
+
+oa_status oa_destroy(oa_ctx* c) {
+  if (!c) return OA_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->res_host) (void)hipHostFree(c->res_host);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  c->ev.clear();
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return OA_OK;
+}
+
+const char* oa_last_error(const oa_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+oa_status oa_layout_info(const oa_ctx* c, int info[4]) {
+  if (!c || !info) return OA_EINVAL;
+  info[0] = c->npts;
+  info[1] = c->nlines;
+  info[2] = c->nplane_x;
+  info[3] = c->nkx0;
+  return OA_OK;
+}
+
+oa_status oa_get_kvecs(const oa_ctx* c, int* kx, int* ky, int* kz) {
+  if (!c || !kx || !ky || !kz) return OA_EINVAL;
+  std::copy(c->kx.begin(), c->kx.end(), kx);
+  std::copy(c->ky.begin(), c->ky.end(), ky);
+  std::copy(c->kz.begin(), c->kz.end(), kz);
+  return OA_OK;
+}
+
+static oa_status copy_g(oa_ctx* c, double2* dev, const double* host, bool in) {
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpy2DAsync(in ? (void*)dev : (void*)host, in ? c->ld * sizeof(double2) : c->npts * sizeof(double2),
+                          in ? (const void*)host : (const void*)dev,
+                          in ? c->npts * sizeof(double2) : c->ld * sizeof(double2),
+                          c->npts * sizeof(double2), c->nst,
+                          in ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return OA_OK;
+}
+
+oa_status oa_psi_set(oa_ctx* c, const double* psi) {
+  if (!c || !psi) return OA_EINVAL;
+  return copy_g(c, c->psi, psi, true);
+}
+oa_status oa_psi_get(oa_ctx* c, double* psi) {
+  if (!c || !psi) return OA_EINVAL;
+  return copy_g(c, c->psi, psi, false);
+}
+oa_status oa_force_get(oa_ctx* c, double* f) {
+  if (!c || !f) return OA_EINVAL;
+  return copy_g(c, c->force, f, false);
+}
+oa_status oa_force_set(oa_ctx* c, const double* f) {
+  if (!c || !f) return OA_EINVAL;
+  return copy_g(c, c->force, f, true);
+}
+
+oa_status oa_vks_set(oa_ctx* c, const double* v) {
+  if (!c || !v) return OA_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  TRY(upload(c, c->vks, v, (size_t)c->nx * c->ny * c->nz * sizeof(double)));
+  c->have_v = true;
+  return OA_OK;
+}
+
+oa_status oa_fft_g2r(oa_ctx* c) {
+  if (!c) return OA_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  TRY(st_g2r(c));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return OA_OK;
+}
+
+oa_status oa_psir_get(oa_ctx* c, int s, double* out) {
+  if (!c || !out || s < 0 || s >= c->nst) return OA_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  const size_t N = (size_t)c->nx * c->ny * c->nz;
+  if (!c->fp32) {
+    HIPCHK(hipMemcpy(out, (double*)c->psir + (size_t)s * N, N * sizeof(double),
+                     hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> tmp(N);
+    HIPCHK(hipMemcpy(tmp.data(), (float*)c->psir + (size_t)s * N, N * sizeof(float),
+                     hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < N; i++) out[i] = tmp[i];
+  }
+  return OA_OK;
+}
+
+oa_status oa_density_get(oa_ctx* c, double* rho) {
+  if (!c || !rho) return OA_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(rho, c->rho, (size_t)c->nx * c->ny * c->nz * sizeof(double),
+                   hipMemcpyDeviceToHost));
+  return OA_OK;
+}
+
+oa_status oa_vpsi_r2g(oa_ctx* c, double* eke) {
+  if (!c) return OA_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  TRY(st_r2g(c));
+  TRY(fetch_result(c, nullptr));
+  if (eke) *eke = c->res_host->eke;
+  return OA_OK;
+}
+
+oa_status oa_lambda(oa_ctx* c, double* lambda) {
+  if (!c) return OA_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  TRY(st_lambda(c));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (lambda)
+    HIPCHK(hipMemcpy(lambda, c->lam, (size_t)c->nst * c->nst * sizeof(double),
+                     hipMemcpyDeviceToHost));
+  return OA_OK;
+}
+
+oa_status oa_sd_update(oa_ctx* c, double* fnorm) {
+  if (!c) return OA_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  TRY(st_sd(c));
+  TRY(fetch_result(c, nullptr));
+  if (fnorm) *fnorm = std::sqrt(c->res_host->fnorm2);
+  return OA_OK;
+}
+
+oa_status oa_overlap(oa_ctx* c, double* S) {
+  if (!c) return OA_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  TRY(st_overlap(c));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (S)
+    HIPCHK(hipMemcpy(S, c->S, (size_t)c->nst * c->nst * sizeof(double), hipMemcpyDeviceToHost));
+  return OA_OK;
+}
+
+oa_status oa_ortho_ns(oa_ctx* c, double tol, int max_iter, int* iters, double* err, double* T) {
+  if (!c || max_iter < 1) return OA_EINVAL;
+  if (!c->have_S) return fail(c, OA_ESTATE, "oa_ortho_ns before oa_overlap");
+  HIPCHK(hipSetDevice(c->device));
+  TRY(st_ns(c, tol, max_iter));
+  TRY(fetch_result(c, nullptr));
+  const int it = c->res_host->ns_iter;
+  if (iters) *iters = it < 0 ? -it : it;
+  if (err) *err = c->res_host->ns_err;
+  if (T)
+    HIPCHK(hipMemcpy(T, c->T, (size_t)c->nst * c->nst * sizeof(double), hipMemcpyDeviceToHost));
+  if (it < 0) return fail(c, OA_ENOCONV, "invsqr_tolerance not met within invsqr_max_iter");
+  return OA_OK;
+}
+
+oa_status oa_tmat_set(oa_ctx* c, const double* T) {
+  if (!c || !T) return OA_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  TRY(upload(c, c->T, T, (size_t)c->nst * c->nst * sizeof(double)));
+  c->have_T = true;
+  return OA_OK;
+}
+
+oa_status oa_rotate(oa_ctx* c) {
+  if (!c) return OA_EINVAL;
+  if (!c->have_T) return fail(c, OA_ESTATE, "oa_rotate before a T is available");
+  HIPCHK(hipSetDevice(c->device));
+  TRY(st_rot_psi(c));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return OA_OK;
+}
+
+oa_status oa_step(oa_ctx* c, oa_step_out* out) {
+  if (!c) return OA_EINVAL;
+  if (!c->have_v) return fail(c, OA_ESTATE, "oa_step before oa_vks_set");
+  HIPCHK(hipSetDevice(c->device));
+  if (c->timing) c->ev_cur = c->ev_count % oa_ctx::kRing;
+  mark(c, 0);
+  TRY(st_g2r(c, 1));   // marks 1..3
+  TRY(st_r2g(c, 1));   // marks 4..6, then the EKE / E_loc reductions
+  if (c->opt.use_lambda) TRY(st_lambda(c));
+  mark(c, 7);
+  TRY(st_sd(c));
+  mark(c, 8);
+  TRY(st_overlap(c));
+  mark(c, 9);
+  TRY(st_ns(c, c->opt.ns_tol, c->opt.ns_max_iter));
+  mark(c, 10);
+  TRY(st_rot_psi(c));
+  mark(c, 11);
+  if (c->opt.want_resid) {
+    TRY(st_gram(c, c->psi, c->psi, true, 1.0, 2.0, c->S2));
+    HIPCHK(oa::resid_stats(c->S2, c->nst, c->res, c->stream));
+  }
+  mark(c, 12);
+  if (c->timing) c->ev_count++;
+  if (out) {
+    TRY(fetch_result(c, out));
+    if (c->res_host->ns_iter < 0)
+      return fail(c, OA_ENOCONV, "invsqr_tolerance not met within invsqr_max_iter");
+  }
+  return OA_OK;
+}
+
+oa_status oa_sync(oa_ctx* c) {
+  if (!c) return OA_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return OA_OK;
+}
+
+oa_status oa_step_result(oa_ctx* c, oa_step_out* out) {
+  if (!c || !out) return OA_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  TRY(fetch_result(c, out));
+  if (c->res_host->ns_iter < 0)
+    return fail(c, OA_ENOCONV, "invsqr_tolerance not met within invsqr_max_iter");
+  return OA_OK;
+}
+
+oa_status oa_set_timing(oa_ctx* c, int on) {
+  if (!c) return OA_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  if (on && c->ev.empty()) {
+    c->ev.assign((size_t)oa_ctx::kRing * OA_NTIMINGS, nullptr);
+    for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
+  }
+  c->timing = on != 0;
+  c->ev_count = 0;
+  c->ev_cur = 0;
+  return OA_OK;
+}
+
+// Average over the last min(steps timed, ring size) steps.
+oa_status oa_timings(oa_ctx* c, double* ms, int n) {
+  if (!c || !ms || n < 1) return OA_EINVAL;
+  if (!c->timing || c->ev_count == 0) return fail(c, OA_ESTATE, "no timed step");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const int nsteps = std::min(c->ev_count, (int)oa_ctx::kRing);
+  for (int i = 0; i < n && i < OA_NTIMINGS; i++) {
+    const int a = (i == OA_NTIMINGS - 1) ? 0 : i;
+    const int b = (i == OA_NTIMINGS - 1) ? 12 : i + 1;
+    double acc = 0.0;
+    for (int k = 0; k < nsteps; k++) {
+      const size_t base = (size_t)((c->ev_count - 1 - k) % oa_ctx::kRing) * OA_NTIMINGS;
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, c->ev[base + a], c->ev[base + b]));
+      acc += t;
+    }
+    ms[i] = acc / nsteps;
+  }
+  return OA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,202 @@
+"""Host-side view of one device context of the CP electronic-step library.
+
+Method names follow the reference's chare-facing calls where one exists
+(paths relative to the OpenAtom tree):
+
+  fft_g2r      FFTcache::doStpFFTGtoR_Gchare/_Rchare + density reduction
+               (fftCache.C:1133-1243, CP_State_RealSpacePlane.C:349-517)
+  vpsi_r2g     unpackVks + doStpFFTRtoG_* + doIFFT + addForces + CP_eke_calc
+               (CP_State_RealSpacePlane.C:530-597, fftCache.C:1099-1319,
+                CP_State_GSpacePlane.C:1582-1778)
+  lambda_      asymmetric PairCalculator forward/backward (ckPairCalculator.C)
+  sd_update    CP_integrate_min_STD + redundant-psi sync
+  overlap      symmetric PairCalculator::multiplyForward (S ~ 2I)
+  ortho_ns     Ortho Newton-Schulz (ortho.C:862-985)
+  rotate       PairCalculator::bwMultiplyHelper (Psi T)
+  step         GSpaceDriver minimisation iteration (gspace.ci:106-190)
+
+All calls go through liboa_step.so; nothing here computes physics.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import OaError, ptr
+
+
+class CPContext:
+    def __init__(self, grid, hmat, ecut_ry, nstates, kx=None, ky=None, kz=None,
+                 occ=None, precision="fp64", device=0, **opts):
+        L = _lib.load()
+        self._L = L
+        self.grid = tuple(int(g) for g in grid)
+        self.nx, self.ny, self.nz = self.grid
+        self.nst = int(nstates)
+        sysd = _lib.OaSystem()
+        for i, v in enumerate(np.asarray(hmat, np.float64).reshape(9)):
+            sysd.hmat[i] = float(v)
+        for i, v in enumerate(self.grid):
+            sysd.grid[i] = v
+        sysd.ecut_ry = float(ecut_ry)
+        sysd.nstates = self.nst
+        self._occ = None if occ is None else np.ascontiguousarray(occ, np.float64)
+        sysd.occ = ptr(self._occ)
+        if kx is not None:
+            self._k = [np.ascontiguousarray(a, np.int32) for a in (kx, ky, kz)]
+            sysd.npts = int(self._k[0].size)
+            sysd.kx, sysd.ky, sysd.kz = (ptr(a) for a in self._k)
+        else:
+            sysd.npts = 0
+        o = _lib.default_options()
+        o.precision = _lib.OA_MIXED if precision == "mixed" else _lib.OA_FP64
+        for k, v in opts.items():
+            if not hasattr(o, k):
+                raise TypeError(f"unknown option {k}")
+            setattr(o, k, v)
+        self.opts = o
+        self.precision = precision
+        h = ctypes.c_void_p()
+        st = L.oa_create(int(device), ctypes.byref(sysd), ctypes.byref(o), ctypes.byref(h))
+        if st != _lib.OA_OK:
+            raise OaError(st, "oa_create failed (see stderr)")
+        self.h = h
+        info = np.zeros(4, np.int32)
+        self._chk(L.oa_layout_info(self.h, ptr(info)))
+        self.npts, self.nlines, self.nplane_x, self.nkx0 = (int(v) for v in info)
+
+    # ------------------------------------------------------------------ plumbing
+    def _chk(self, st):
+        if st != _lib.OA_OK:
+            msg = self._L.oa_last_error(self.h).decode() if getattr(self, "h", None) else ""
+            raise OaError(st, msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.oa_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def ngrid(self):
+        return self.nx * self.ny * self.nz
+
+    def kvecs(self):
+        kx = np.zeros(self.npts, np.int32); ky = np.zeros_like(kx); kz = np.zeros_like(kx)
+        self._chk(self._L.oa_get_kvecs(self.h, ptr(kx), ptr(ky), ptr(kz)))
+        return kx, ky, kz
+
+    # ------------------------------------------------------------------ data
+    def set_psi(self, psi):
+        a = np.ascontiguousarray(psi, np.complex128)
+        assert a.shape == (self.nst, self.npts), a.shape
+        self._chk(self._L.oa_psi_set(self.h, ptr(a)))
+
+    def get_psi(self):
+        a = np.zeros((self.nst, self.npts), np.complex128)
+        self._chk(self._L.oa_psi_get(self.h, ptr(a)))
+        return a
+
+    def set_vks(self, v):
+        a = np.ascontiguousarray(v, np.float64)
+        assert a.size == self.ngrid
+        self._chk(self._L.oa_vks_set(self.h, ptr(a)))
+
+    def get_force(self):
+        a = np.zeros((self.nst, self.npts), np.complex128)
+        self._chk(self._L.oa_force_get(self.h, ptr(a)))
+        return a
+
+    def set_force(self, f):
+        a = np.ascontiguousarray(f, np.complex128)
+        self._chk(self._L.oa_force_set(self.h, ptr(a)))
+
+    def get_psir(self, state):
+        a = np.zeros((self.nz, self.ny, self.nx))
+        self._chk(self._L.oa_psir_get(self.h, int(state), ptr(a)))
+        return a
+
+    def get_density(self):
+        a = np.zeros((self.nz, self.ny, self.nx))
+        self._chk(self._L.oa_density_get(self.h, ptr(a)))
+        return a
+
+    # ------------------------------------------------------------------ stages
+    def fft_g2r(self):
+        self._chk(self._L.oa_fft_g2r(self.h))
+
+    def vpsi_r2g(self):
+        e = ctypes.c_double()
+        self._chk(self._L.oa_vpsi_r2g(self.h, ctypes.byref(e)))
+        return e.value
+
+    def lambda_(self):
+        lam = np.zeros((self.nst, self.nst))
+        self._chk(self._L.oa_lambda(self.h, ptr(lam)))
+        return lam
+
+    def sd_update(self):
+        f = ctypes.c_double()
+        self._chk(self._L.oa_sd_update(self.h, ctypes.byref(f)))
+        return f.value
+
+    def overlap(self):
+        S = np.zeros((self.nst, self.nst))
+        self._chk(self._L.oa_overlap(self.h, ptr(S)))
+        return S
+
+    def ortho_ns(self, tol=1e-15, max_iter=30):
+        T = np.zeros((self.nst, self.nst))
+        it = ctypes.c_int()
+        err = ctypes.c_double()
+        st = self._L.oa_ortho_ns(self.h, float(tol), int(max_iter), ctypes.byref(it),
+                                 ctypes.byref(err), ptr(T))
+        if st not in (_lib.OA_OK, _lib.OA_ENOCONV):
+            self._chk(st)
+        return T, it.value, err.value, st == _lib.OA_OK
+
+    def set_tmat(self, T):
+        a = np.ascontiguousarray(T, np.float64)
+        self._chk(self._L.oa_tmat_set(self.h, ptr(a)))
+
+    def rotate(self):
+        self._chk(self._L.oa_rotate(self.h))
+
+    def step(self, fetch=True):
+        if fetch:
+            out = _lib.OaStepOut()
+            self._chk(self._L.oa_step(self.h, ctypes.byref(out)))
+            return out.as_dict()
+        self._chk(self._L.oa_step(self.h, None))
+        return None
+
+    def result(self):
+        out = _lib.OaStepOut()
+        self._chk(self._L.oa_step_result(self.h, ctypes.byref(out)))
+        return out.as_dict()
+
+    def sync(self):
+        self._chk(self._L.oa_sync(self.h))
+
+    def set_timing(self, on=True):
+        self._chk(self._L.oa_set_timing(self.h, int(bool(on))))
+
+    TIMING_NAMES = ("k_zfwd", "k_y_fwd", "k_xc2r", "k_xr2c", "k_y_bwd", "k_zbwd",
+                    "reduce_lambda", "sd_sync", "overlap", "ns", "rotate", "resid", "step")
+
+    def timings(self):
+        """Per-stage device ms of the last step (HIP events on the context's stream)."""
+        ms = np.zeros(len(self.TIMING_NAMES))
+        self._chk(self._L.oa_timings(self.h, ptr(ms), len(ms)))
+        return dict(zip(self.TIMING_NAMES, ms.tolist()))

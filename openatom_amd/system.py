@@ -1,0 +1,170 @@
+"""Seeded synthetic inputs for the CP electronic step: liquid-water boxes,
+Gaussian-seeded orthonormal coefficients and a Gaussian local potential.
+
+Workloads (sizes follow the measurement plan of the build):
+  water8   :   8 H2O, L = 6.21 A,   32 states, 30 Ry, 48^3
+  water32  :  32 H2O, L = 9.86 A,  128 states, 70 Ry, 100^3
+  water128 : 128 H2O, L = 15.65 A, 512 states, 70 Ry, 160^3
+
+Coefficients follow c_s(G) = (xi1 + i xi2) exp(-|G|^2/4) on the unique half
+sphere, Hermitian on the kx=0 plane, c(0) real, orthonormalised under the
+Gamma inner product <a,b> = sum_stored w_p Re(a* b) (w = 1 on kx=0, 2
+elsewhere) and scaled to the reference's norm 2
+(src_piny_physics_v1.0/interface/coords_cp/gen_wave.C:699-710).
+"""
+import numpy as np
+
+from . import _lib
+
+BOHR_PER_ANG = 1.8897261246257702
+OH_ANG = 0.9572
+HOH_DEG = 104.52
+
+WORKLOADS = {
+    "water8": dict(nmol=8, box_ang=6.21, nstates=32, ecut_ry=30.0, grid=48),
+    "water32": dict(nmol=32, box_ang=9.86, nstates=128, ecut_ry=70.0, grid=100),
+    "water128": dict(nmol=128, box_ang=15.65, nstates=512, ecut_ry=70.0, grid=160),
+}
+
+
+def _rand_rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    a, b, c, d = q
+    return np.array([
+        [a * a + b * b - c * c - d * d, 2 * (b * c - a * d), 2 * (b * d + a * c)],
+        [2 * (b * c + a * d), a * a - b * b + c * c - d * d, 2 * (c * d - a * b)],
+        [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
+
+
+def place_water(nmol, box_ang, seed, min_oo_ang=2.5, max_tries=200000):
+    """O uniform in a cubic box, min-image O-O >= min_oo_ang; H at 0.9572 A and
+    104.52 deg with a uniform random orientation.  Returns (pos_bohr, species)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    L = box_ang
+    oxy = []
+    tries = 0
+    while len(oxy) < nmol:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError("could not place water molecules")
+        p = rng.uniform(0.0, L, size=3)
+        ok = True
+        for q in oxy:
+            d = p - q
+            d -= L * np.round(d / L)
+            if np.dot(d, d) < min_oo_ang ** 2:
+                ok = False
+                break
+        if ok:
+            oxy.append(p)
+    half = np.deg2rad(HOH_DEG) / 2.0
+    h1 = OH_ANG * np.array([np.sin(half), np.cos(half), 0.0])
+    h2 = OH_ANG * np.array([-np.sin(half), np.cos(half), 0.0])
+    pos, spec = [], []
+    for o in oxy:
+        R = _rand_rotation(rng)
+        pos += [o, o + R @ h1, o + R @ h2]
+        spec += ["O", "H", "H"]
+    pos = np.mod(np.array(pos), L) * BOHR_PER_ANG
+    return pos, spec
+
+
+def cubic_hmat(box_bohr):
+    return np.diag([box_bohr] * 3).reshape(9).astype(np.float64)
+
+
+def gvec_sq(kx, ky, kz, hmat):
+    hi = np.linalg.inv(np.asarray(hmat, np.float64).reshape(3, 3))
+    k = np.stack([kx, ky, kz]).astype(np.float64)
+    G = 2.0 * np.pi * (hi @ k)
+    return (G * G).sum(axis=0)
+
+
+def redundant_pairs(kx, ky, kz):
+    """(redundant index, unique partner index) on the kx=0 plane."""
+    idx = {(int(b), int(c)): i for i, (a, b, c) in enumerate(zip(kx, ky, kz)) if a == 0}
+    red, par = [], []
+    for i, (a, b, c) in enumerate(zip(kx, ky, kz)):
+        if a == 0 and (b < 0 or (b == 0 and c < 0)):
+            red.append(i)
+            par.append(idx[(-int(b), -int(c))])
+    return np.array(red, np.int64), np.array(par, np.int64)
+
+
+def gamma_weights(kx):
+    return np.where(np.asarray(kx) == 0, 1.0, 2.0)
+
+
+def gamma_overlap(psi, kx):
+    """S_ij = sum_p w_p Re(conj psi_i psi_j): the full-sphere overlap."""
+    w = gamma_weights(kx)
+    A = psi * np.sqrt(w)[None, :]
+    Ar = np.concatenate([A.real, A.imag], axis=1)
+    return Ar @ Ar.T
+
+
+def initial_coeffs(kx, ky, kz, hmat, nstates, seed, norm=2.0):
+    """Seeded Gaussian coefficients, Gram-Schmidt orthonormalised (Cholesky-QR
+    twice, which reproduces Gram-Schmidt's triangular factor), norm `norm`."""
+    rng = np.random.Generator(np.random.PCG64(seed + 7919))
+    npts = len(kx)
+    g2 = gvec_sq(kx, ky, kz, hmat)
+    amp = np.exp(-g2 / 4.0)
+    xi = rng.normal(size=(nstates, npts)) + 1j * rng.normal(size=(nstates, npts))
+    psi = xi * amp[None, :]
+    red, par = redundant_pairs(kx, ky, kz)
+    psi[:, red] = np.conj(psi[:, par])
+    g0 = np.where((np.asarray(kx) == 0) & (np.asarray(ky) == 0) & (np.asarray(kz) == 0))[0]
+    psi[:, g0] = psi[:, g0].real
+    for _ in range(2):
+        S = gamma_overlap(psi, kx)
+        Lc = np.linalg.cholesky(S)
+        psi = np.linalg.solve(Lc, psi)
+    return psi * np.sqrt(norm)
+
+
+def gaussian_vloc(pos, spec, hmat, grid, seed):
+    """V(r) = sum_a A_a exp(-|r - R_a|^2 / (2 sigma_a^2)) with per-species
+    (A, sigma) drawn from the seed; orthorhombic minimum image.
+    Returns (V[Nz][Ny][Nx], params)."""
+    rng = np.random.Generator(np.random.PCG64(seed + 104729))
+    params = {"O": (float(rng.uniform(-2.0, -1.0)), float(rng.uniform(0.6, 0.9))),
+              "H": (float(rng.uniform(-0.8, -0.4)), float(rng.uniform(0.4, 0.6)))}
+    h = np.asarray(hmat, np.float64).reshape(3, 3)
+    Lx, Ly, Lz = h[0, 0], h[1, 1], h[2, 2]
+    nx, ny, nz = grid
+    xs = np.arange(nx) * (Lx / nx)
+    ys = np.arange(ny) * (Ly / ny)
+    zs = np.arange(nz) * (Lz / nz)
+    V = np.zeros((nz, ny, nx))
+    for p, s in zip(pos, spec):
+        A, sig = params[s]
+        dx = xs - p[0]; dx -= Lx * np.round(dx / Lx)
+        dy = ys - p[1]; dy -= Ly * np.round(dy / Ly)
+        dz = zs - p[2]; dz -= Lz * np.round(dz / Lz)
+        c = -1.0 / (2.0 * sig * sig)
+        ex = np.exp(c * dx * dx)
+        ey = np.exp(c * dy * dy)
+        ez = np.exp(c * dz * dz)
+        V += A * ez[:, None, None] * ey[None, :, None] * ex[None, None, :]
+    return V, params
+
+
+def build(workload="water32", seed=1234, nstates=None, kvec_source=None):
+    """Build one seeded system.  kvec_source(ecut_ry, hmat) -> (kx, ky, kz)
+    defaults to the library's own enumeration (oa_kvecs)."""
+    w = dict(WORKLOADS[workload]) if isinstance(workload, str) else dict(workload)
+    nst = int(nstates or w["nstates"])
+    L = w["box_ang"] * BOHR_PER_ANG
+    hmat = cubic_hmat(L)
+    grid = (w["grid"],) * 3 if np.isscalar(w["grid"]) else tuple(w["grid"])
+    pos, spec = place_water(w["nmol"], w["box_ang"], seed)
+    src = kvec_source or _lib.kvecs
+    kx, ky, kz = src(w["ecut_ry"], hmat)
+    psi = initial_coeffs(kx, ky, kz, hmat, nst, seed)
+    V, params = gaussian_vloc(pos, spec, hmat, grid, seed)
+    return dict(name=workload if isinstance(workload, str) else "custom", seed=seed,
+                hmat=hmat, grid=grid, ecut_ry=float(w["ecut_ry"]), nstates=nst,
+                natoms=len(spec), nmol=w["nmol"], pos=pos, spec=spec, kx=kx, ky=ky, kz=kz,
+                psi=psi, vks=V, vparams=params)

@@ -1,0 +1,273 @@
+"""ctypes view of the CPU oracle (oracle/liboa_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the host baseline.
+The product library never loads it.  Build with ``make -C oracle``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboa_oracle.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = ctypes.CDLL(_LIB)
+        c_int, c_double, vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+        L.oo_kmax.argtypes = [c_double, _dp, _ip]
+        L.oo_kvec_half.argtypes = [c_double, _dp, _ip, vp, vp, vp]
+        L.oo_kvec_half.restype = c_int
+        L.oo_flip.argtypes = [c_int, _ip, _ip, _ip, vp, vp, vp]
+        L.oo_flip.restype = c_int
+        L.oo_stored_kvecs.argtypes = [c_double, _dp, vp, vp, vp]
+        L.oo_stored_kvecs.restype = c_int
+        L.oo_inv3.argtypes = [_dp, _dp]
+        L.oo_inv3.restype = c_double
+        L.oo_fft1d.argtypes = [c_int, c_int, _dp]
+        L.oo_dft1d.argtypes = [c_int, c_int, _dp, _dp]
+        L.oo_create.argtypes = [c_int, c_int, c_int, _dp, c_double, c_int, vp,
+                                c_int, _ip, _ip, _ip]
+        L.oo_create.restype = vp
+        L.oo_destroy.argtypes = [vp]
+        for name in ("oo_nlines", "oo_nplane_x", "oo_nkx0", "oo_nkx0_red"):
+            getattr(L, name).argtypes = [vp]
+            getattr(L, name).restype = c_int
+        L.oo_volume.argtypes = [vp]
+        L.oo_volume.restype = c_double
+        L.oo_g2.argtypes = [vp, _dp]
+        L.oo_g2r.argtypes = [vp, _dp, _dp]
+        L.oo_g2r_all.argtypes = [vp, _dp, _dp]
+        L.oo_density.argtypes = [vp, _dp, _dp]
+        L.oo_vpsi_r2g.argtypes = [vp, c_int, _dp, _dp, _dp]
+        L.oo_vpsi_r2g_all.argtypes = [vp, _dp, _dp, _dp]
+        L.oo_eke_force.argtypes = [vp, _dp, _dp]
+        L.oo_eke_force.restype = c_double
+        L.oo_lambda.argtypes = [vp, _dp, _dp, _dp]
+        L.oo_lambda_apply.argtypes = [vp, _dp, _dp, _dp]
+        L.oo_cmass.argtypes = [vp, c_double, c_double, _dp]
+        L.oo_sd_update.argtypes = [vp, _dp, c_double, _dp, _dp]
+        L.oo_red_sync.argtypes = [vp, _dp]
+        L.oo_overlap.argtypes = [vp, _dp, _dp]
+        L.oo_ns.argtypes = [c_int, _dp, c_double, c_int, _dp,
+                            ctypes.POINTER(c_double)]
+        L.oo_ns.restype = c_int
+        L.oo_rotate.argtypes = [vp, _dp, _dp, _dp]
+        L.oo_eloc.argtypes = [vp, _dp, _dp]
+        L.oo_eloc.restype = c_double
+        L.oo_step.argtypes = [vp, ctypes.POINTER(StepOpts), _dp, _dp, vp, vp,
+                              vp, vp, ctypes.POINTER(StepOut)]
+        L.oo_step.restype = c_int
+        L.oo_set_threads.argtypes = [c_int]
+        L.oo_get_threads.restype = c_int
+        _lib = L
+    return _lib
+
+
+class StepOpts(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("tau_fs", ctypes.c_double),
+                ("ecut_mass", ctypes.c_double), ("ns_tol", ctypes.c_double),
+                ("ns_max_iter", ctypes.c_int), ("use_lambda", ctypes.c_int)]
+
+
+class StepOut(ctypes.Structure):
+    _fields_ = [("eke", ctypes.c_double), ("eloc", ctypes.c_double),
+                ("eelec", ctypes.c_double), ("fnorm", ctypes.c_double),
+                ("ns_iters", ctypes.c_int), ("ns_err", ctypes.c_double)]
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def stored_kvecs(ecut_ry, hmat):
+    """Stored (kx, ky, kz) of the reference layout for a cell/cutoff."""
+    h = np.ascontiguousarray(hmat, dtype=np.float64).reshape(9)
+    n = lib().oo_stored_kvecs(ecut_ry, h, None, None, None)
+    kx = np.zeros(n, np.int32); ky = np.zeros(n, np.int32); kz = np.zeros(n, np.int32)
+    lib().oo_stored_kvecs(ecut_ry, h, _ptr(kx), _ptr(ky), _ptr(kz))
+    return kx, ky, kz
+
+
+def fft1d(x, sign):
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    lib().oo_fft1d(a.size, sign, a.view(np.float64))
+    return a
+
+
+def dft1d(x, sign):
+    a = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.zeros_like(a)
+    lib().oo_dft1d(a.size, sign, a.view(np.float64), out.view(np.float64))
+    return out
+
+
+class System:
+    """One Gamma-point system on the oracle (reference order of points)."""
+
+    def __init__(self, grid, hmat, ecut_ry, nstates, kx, ky, kz, occ=None):
+        self.nx, self.ny, self.nz = (int(g) for g in grid)
+        self.nst = int(nstates)
+        self.kx = np.ascontiguousarray(kx, np.int32)
+        self.ky = np.ascontiguousarray(ky, np.int32)
+        self.kz = np.ascontiguousarray(kz, np.int32)
+        self.npts = self.kx.size
+        self._occ = None if occ is None else np.ascontiguousarray(occ, np.float64)
+        h = np.ascontiguousarray(hmat, np.float64).reshape(9)
+        self.h = lib().oo_create(self.nx, self.ny, self.nz, h, float(ecut_ry),
+                                 self.nst, _ptr(self._occ), self.npts,
+                                 self.kx, self.ky, self.kz)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oo_destroy(self.h)
+            self.h = None
+
+    @property
+    def ngrid(self):
+        return self.nx * self.ny * self.nz
+
+    @property
+    def nlines(self):
+        return lib().oo_nlines(self.h)
+
+    @property
+    def nplane_x(self):
+        return lib().oo_nplane_x(self.h)
+
+    @property
+    def nkx0(self):
+        return lib().oo_nkx0(self.h)
+
+    @property
+    def volume(self):
+        return lib().oo_volume(self.h)
+
+    def g2(self):
+        out = np.zeros(self.npts)
+        lib().oo_g2(self.h, out)
+        return out
+
+    @staticmethod
+    def _c(psi):
+        return np.ascontiguousarray(psi, np.complex128).view(np.float64)
+
+    def g2r(self, c):
+        out = np.zeros((self.nz, self.ny, self.nx))
+        lib().oo_g2r(self.h, self._c(c).reshape(-1), out.reshape(-1))
+        return out
+
+    def g2r_all(self, psi):
+        out = np.zeros((self.nst, self.nz, self.ny, self.nx))
+        lib().oo_g2r_all(self.h, self._c(psi).reshape(-1), out.reshape(-1))
+        return out
+
+    def density(self, psir):
+        out = np.zeros((self.nz, self.ny, self.nx))
+        lib().oo_density(self.h, np.ascontiguousarray(psir).reshape(-1), out.reshape(-1))
+        return out
+
+    def vpsi_r2g(self, istate, psir_s, v):
+        f = np.zeros(self.npts, np.complex128)
+        lib().oo_vpsi_r2g(self.h, istate, np.ascontiguousarray(psir_s).reshape(-1),
+                          np.ascontiguousarray(v).reshape(-1), f.view(np.float64))
+        return f
+
+    def vpsi_r2g_all(self, psir, v):
+        f = np.zeros((self.nst, self.npts), np.complex128)
+        lib().oo_vpsi_r2g_all(self.h, np.ascontiguousarray(psir).reshape(-1),
+                              np.ascontiguousarray(v).reshape(-1),
+                              f.view(np.float64).reshape(-1))
+        return f
+
+    def eke_force(self, psi, f):
+        """Applies addForces + CP_eke_calc to f in place; returns EKE."""
+        return lib().oo_eke_force(self.h, self._c(psi).reshape(-1),
+                                  f.view(np.float64).reshape(-1))
+
+    def lambda_(self, psi, f):
+        lam = np.zeros((self.nst, self.nst))
+        lib().oo_lambda(self.h, self._c(psi).reshape(-1), self._c(f).reshape(-1),
+                        lam.reshape(-1))
+        return lam
+
+    def lambda_apply(self, psi, lam, f):
+        lib().oo_lambda_apply(self.h, self._c(psi).reshape(-1),
+                              np.ascontiguousarray(lam).reshape(-1),
+                              f.view(np.float64).reshape(-1))
+
+    def cmass(self, tau_fs=25.0, ecut_mass=2.0):
+        out = np.zeros(self.npts)
+        lib().oo_cmass(self.h, tau_fs, ecut_mass, out)
+        return out
+
+    def sd_update(self, cmass, dt, f, psi):
+        lib().oo_sd_update(self.h, cmass, dt, self._c(f).reshape(-1),
+                           psi.view(np.float64).reshape(-1))
+
+    def red_sync(self, psi):
+        lib().oo_red_sync(self.h, psi.view(np.float64).reshape(-1))
+
+    def overlap(self, psi):
+        S = np.zeros((self.nst, self.nst))
+        lib().oo_overlap(self.h, self._c(psi).reshape(-1), S.reshape(-1))
+        return S
+
+    def rotate(self, psi, T):
+        out = np.zeros((self.nst, self.npts), np.complex128)
+        lib().oo_rotate(self.h, self._c(psi).reshape(-1),
+                        np.ascontiguousarray(T, np.float64).reshape(-1),
+                        out.view(np.float64).reshape(-1))
+        return out
+
+    def eloc(self, v, rho):
+        return lib().oo_eloc(self.h, np.ascontiguousarray(v).reshape(-1),
+                             np.ascontiguousarray(rho).reshape(-1))
+
+    def step(self, v, psi, dt=5.0, tau_fs=25.0, ecut_mass=2.0, ns_tol=1e-15,
+             ns_max_iter=100, use_lambda=True, want=()):
+        """One minimisation step; psi (complex [nst][npts]) updated in place."""
+        o = StepOpts(dt, tau_fs, ecut_mass, ns_tol, ns_max_iter, int(use_lambda))
+        out = StepOut()
+        extra = {}
+        rho = np.zeros((self.nz, self.ny, self.nx)) if "rho" in want else None
+        S = np.zeros((self.nst, self.nst)) if "S" in want else None
+        T = np.zeros((self.nst, self.nst)) if "T" in want else None
+        f = np.zeros((self.nst, self.npts), np.complex128) if "f" in want else None
+        rc = lib().oo_step(self.h, ctypes.byref(o), np.ascontiguousarray(v).reshape(-1),
+                           psi.view(np.float64).reshape(-1), _ptr(rho), _ptr(S),
+                           _ptr(T), _ptr(None if f is None else f.view(np.float64)),
+                           ctypes.byref(out))
+        for k, a in (("rho", rho), ("S", S), ("T", T), ("f", f)):
+            if a is not None:
+                extra[k] = a
+        res = dict(eke=out.eke, eloc=out.eloc, eelec=out.eelec, fnorm=out.fnorm,
+                   ns_iters=out.ns_iters, ns_err=out.ns_err, rc=rc)
+        res.update(extra)
+        return res
+
+
+def ns(S, tol=1e-15, max_iter=100):
+    S = np.ascontiguousarray(S, np.float64)
+    n = S.shape[0]
+    T = np.zeros_like(S)
+    err = ctypes.c_double()
+    it = lib().oo_ns(n, S.reshape(-1), tol, max_iter, T.reshape(-1), ctypes.byref(err))
+    return T, it, err.value
+
+
+def set_threads(n):
+    lib().oo_set_threads(int(n))
+
+
+def get_threads():
+    return lib().oo_get_threads()

@@ -1,0 +1,161 @@
+/*
+ * oa_oracle.h -- CPU restatement of OpenAtom's Gamma-point electronic-state
+ * step, used ONLY as the parity checker (tests/, __graft_entry__.smoke())
+ * and as the host-CPU baseline (bench.py cpu_baseline leg).  Nothing in the
+ * product links or calls this library.
+ *
+ * Conventions (restated from the reference, paths relative to the
+ * reference tree):
+ *   - states carry norm 2 and occupation 1
+ *     (src_piny_physics_v1.0/interface/coords_cp/gen_wave.C:699-710);
+ *   - stored points: kx=0 plane first, redundant half (ky<0 or ky=0,kz<0)
+ *     first, then G=0, then the PINY half sphere
+ *     (src_charm_driver/utility/para_grp_parse.C:178-285);
+ *   - G->R: psi(r) = sum_G c(G) exp(-i G.r), unnormalised, via z, y complex
+ *     FFTs (FFTW_FORWARD), conjugation and an x c2r
+ *     (src_charm_driver/fft_slab_ctrl/fftCache.C:1133-1243);
+ *   - R->G: x r2c, conjugation, y and z FFTW_BACKWARD, pack, x(-2/N)
+ *     (fftCache.C:1099-1123, 1252-1319;
+ *      src_charm_driver/cp_state_ctrl/CP_State_GSpacePlane.C:1582-1627).
+ * Real-space arrays are [Nz][Ny][Nx] with x fastest.
+ */
+#ifndef OA_ORACLE_H
+#define OA_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* calc_cutoff kmax (set_cp_ewald.C:176-197); hmati row-major 3x3 */
+void oo_kmax(double ecut_ry, const double hmati[9], int kmax[3]);
+
+/* setkvec3d_sm (set_cp_ewald.C:523-680) followed by the PINY G=0 append;
+ * returns the count (half sphere + G=0).  ka/kb/kc may be NULL to count. */
+int oo_kvec_half(double ecut_ry, const double hmati[9], const int kmax[3],
+                 int *ka, int *kb, int *kc);
+
+/* flip_data_set (para_grp_parse.C:178-285): half sphere (with G=0 last)
+ * -> stored order with the redundant kx=0 half first.  Returns npts. */
+int oo_flip(int n, const int *ka, const int *kb, const int *kc,
+            int *kx, int *ky, int *kz);
+
+/* one-shot helper: enumerate the stored points of a cell/cutoff.
+ * Pass NULL arrays to count only. */
+int oo_stored_kvecs(double ecut_ry, const double hmat[9],
+                    int *kx, int *ky, int *kz);
+
+/* 3x3 inverse of a row-major matrix; returns the determinant */
+double oo_inv3(const double h[9], double hi[9]);
+
+/* textbook mixed-radix complex DFT, in place, unnormalised;
+ * sign = -1 (FFTW_FORWARD) or +1 (FFTW_BACKWARD) */
+void oo_fft1d(int n, int sign, double *data /* interleaved re,im [n] */);
+
+/* naive O(n^2) DFT with the same conventions (pins oo_fft1d) */
+void oo_dft1d(int n, int sign, const double *in, double *out);
+
+typedef struct oo_sys oo_sys;
+
+/* Build the oracle's view of one system.  kx/ky/kz: stored points in the
+ * reference order (see header comment).  occ may be NULL (all ones). */
+oo_sys *oo_create(int nx, int ny, int nz, const double hmat[9],
+                  double ecut_ry, int nstates, const double *occ,
+                  int npts, const int *kx, const int *ky, const int *kz);
+void oo_destroy(oo_sys *s);
+int oo_nlines(const oo_sys *s);
+int oo_nplane_x(const oo_sys *s);
+int oo_nkx0(const oo_sys *s);
+int oo_nkx0_red(const oo_sys *s);
+double oo_volume(const oo_sys *s);
+void oo_g2(const oo_sys *s, double *g2 /* [npts] */);
+
+/* per-state G->R (doStpFFTGtoR_Gchare + doStpFFTGtoR_Rchare) */
+void oo_g2r(const oo_sys *s, const double *c /* [npts] cplx */,
+            double *psir /* [Nz][Ny][Nx] */);
+/* all states: psir[nst][N], threads over states */
+void oo_g2r_all(const oo_sys *s, const double *psi, double *psir);
+
+/* rho(r) = (1/Omega) sum_s occ_s psi_s(r)^2
+ * (CP_State_RealSpacePlane.C:349-355, CP_Rho_RealSpacePlane.C:501,630-650) */
+void oo_density(const oo_sys *s, const double *psir, double *rho);
+
+/* per-state V psi product + R->G + (-2/N) scale
+ * (CP_State_RealSpacePlane.C:580-596 then doStpFFTRtoG_*, doIFFT).
+ * Output is the doIFFT force before GStateSlab::addForces. */
+void oo_vpsi_r2g(const oo_sys *s, int istate, const double *psir_s,
+                 const double *v, double *f /* [npts] cplx */);
+void oo_vpsi_r2g_all(const oo_sys *s, const double *psir, const double *v,
+                     double *f);
+
+/* GStateSlab::addForces weighting (stateSlab.C:240-272, no non-local part)
+ * + CP_eke_calc (cp_eke.C:30-140).  Returns EKE summed over states. */
+double oo_eke_force(const oo_sys *s, const double *psi, double *f);
+
+/* Lambda = Re sum_stored conj(psi_i) F_j  (asym PairCalculator forward,
+ * alpha = 1, ckPairCalculator.C:682-802 with cpaimd.C:2527) and
+ * F_j -= w_p sum_i psi_i Lambda_ij, w = 1/2 on kx=0, 1 elsewhere
+ * (CP_State_GSpacePlane.C:2045-2225, cp_min_opt = 1, LDA) */
+void oo_lambda(const oo_sys *s, const double *psi, const double *f,
+               double *lambda /* [n][n] */);
+void oo_lambda_apply(const oo_sys *s, const double *psi,
+                     const double *lambda, double *f);
+
+/* fictitious masses (cp_control_integrate.C:136-203) */
+void oo_cmass(const oo_sys *s, double tau_fs, double ecut_mass,
+              double *cmass /* [npts] */);
+/* steepest descent psi += dt/m F (cp_min_std.C:56-86) */
+void oo_sd_update(const oo_sys *s, const double *cmass, double dt,
+                  const double *f, double *psi);
+/* redundant kx=0 points <- conj(unique partner)
+ * (CP_State_GSpacePlane.C:2862-3061) */
+void oo_red_sync(const oo_sys *s, double *psi);
+
+/* S = 2 sum_k A_ik A_jk over stored points with kx=0 scaled by 1/sqrt2
+ * (sendPsi CP_State_GSpacePlane.C:3118-3125 + symm PC forward alpha=2,
+ * ckPairCalculator.C:682-802) */
+void oo_overlap(const oo_sys *s, const double *psi, double *S /* [n][n] */);
+
+/* Newton-Schulz T = (S/2)^-1/2 (cp_lowdin.C:180-265; ortho.C:109-146,
+ * 862-985).  Returns iterations; *err = final rms step; -1 if max_iter hit
+ * with err > tol. */
+int oo_ns(int n, const double *S, double tol, int max_iter, double *T,
+          double *err);
+
+/* psi'_j = sum_i psi_i T_ij (bwMultiplyHelper, ckPairCalculator.C:1189-1425) */
+void oo_rotate(const oo_sys *s, const double *psi, const double *T,
+               double *psi_out);
+
+/* E_loc = (Omega/N) sum_r V rho */
+double oo_eloc(const oo_sys *s, const double *v, const double *rho);
+
+typedef struct {
+  double dt;          /* SD step (a.u.)                        */
+  double tau_fs;      /* cp_mass_tau_def (fs), default 25      */
+  double ecut_mass;   /* cp_mass_cut_def, default 2            */
+  double ns_tol;      /* invsqr_tolerance                      */
+  int ns_max_iter;    /* invsqr_max_iter                       */
+  int use_lambda;     /* project forces with Lambda            */
+} oo_step_opts;
+
+typedef struct {
+  double eke, eloc, eelec;
+  double fnorm;       /* sqrt(sum |F|^2) after projection        */
+  int ns_iters;
+  double ns_err;
+} oo_step_out;
+
+/* One minimisation step on the host (gspace.ci:106-190 order):
+ * G->R, rho, V psi, R->G, EKE + kinetic force, [Lambda], SD, red sync,
+ * S, NS, Psi.T.  psi is updated in place; rho/S/T/f may be NULL. */
+int oo_step(const oo_sys *s, const oo_step_opts *o, const double *v,
+            double *psi, double *rho, double *S, double *T, double *f,
+            oo_step_out *out);
+
+/* thread control for the CPU baseline */
+void oo_set_threads(int n);
+int oo_get_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,184 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle, stage by
+stage and for whole steps.  FP64 bar: 1e-10 relative (max-abs error over
+max-abs value) on psi(r), rho, F, EKE, Lambda, S, T, rotated psi and the
+electronic energy.  Mixed precision: FP32 FFTs, errors stated below."""
+import numpy as np
+import pytest
+
+import oracle
+from openatom_amd import system
+from openatom_amd.step import CPContext
+from tests.helpers import rel, small_system
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _systems():
+    return {
+        "small": lambda: small_system(),
+        "ragged37": lambda: small_system(nstates=37, ecut=18.0, seed=8),
+        "water8": lambda: system.build("water8", seed=11, kvec_source=oracle.stored_kvecs),
+    }
+
+
+_cache = {}
+
+
+def get_system(name):
+    if name not in _cache:
+        _cache[name] = _systems()[name]()
+    return _cache[name]
+
+
+def make_ctx(s, precision="fp64", **kw):
+    return CPContext(s["grid"], s["hmat"], s["ecut_ry"], s["nstates"], s["kx"], s["ky"],
+                     s["kz"], precision=precision, **kw)
+
+
+def make_oracle(s):
+    return oracle.System(s["grid"], s["hmat"], s["ecut_ry"], s["nstates"], s["kx"], s["ky"],
+                         s["kz"])
+
+
+def report(tag, **errs):
+    print(tag, " ".join(f"{k}={v:.3e}" for k, v in errs.items()), flush=True)
+
+
+@pytest.mark.parametrize("name", ["small", "ragged37", "water8"])
+def test_g2r_density(gpu, name):
+    s = get_system(name)
+    o = make_oracle(s)
+    with make_ctx(s) as c:
+        c.set_psi(s["psi"])
+        c.fft_g2r()
+        pr = o.g2r_all(s["psi"])
+        e_psir = max(rel(c.get_psir(st), pr[st]) for st in range(s["nstates"]))
+        e_rho = rel(c.get_density(), o.density(pr))
+    report(f"[{name}] g2r", psir=e_psir, rho=e_rho)
+    assert e_psir < TOL and e_rho < TOL
+
+
+@pytest.mark.parametrize("name", ["small", "ragged37", "water8"])
+def test_r2g_force_eke(gpu, name):
+    s = get_system(name)
+    o = make_oracle(s)
+    with make_ctx(s) as c:
+        c.set_psi(s["psi"])
+        c.set_vks(s["vks"])
+        c.fft_g2r()
+        eke = c.vpsi_r2g()
+        F = c.get_force()
+    pr = o.g2r_all(s["psi"])
+    Fo = o.vpsi_r2g_all(pr, s["vks"])
+    eko = o.eke_force(s["psi"], Fo)
+    report(f"[{name}] r2g", force=rel(F, Fo), eke=abs(eke - eko) / abs(eko))
+    assert rel(F, Fo) < TOL
+    assert abs(eke - eko) < TOL * abs(eko)
+
+
+@pytest.mark.parametrize("name", ["small", "ragged37", "water8"])
+def test_lambda_sd_overlap_ns_rotate(gpu, name):
+    s = get_system(name)
+    o = make_oracle(s)
+    dt = 5.0
+    with make_ctx(s, dt=dt) as c:
+        c.set_psi(s["psi"])
+        c.set_vks(s["vks"])
+        c.fft_g2r()
+        c.vpsi_r2g()
+        lam = c.lambda_()
+        F = c.get_force()
+        fn = c.sd_update()
+        psi_sd = c.get_psi()
+        S = c.overlap()
+        T, it, err, ok = c.ortho_ns(1e-15, 30)
+        c.rotate()
+        psi_out = c.get_psi()
+    # oracle chain on the same inputs
+    pr = o.g2r_all(s["psi"])
+    Fo = o.vpsi_r2g_all(pr, s["vks"])
+    o.eke_force(s["psi"], Fo)
+    lamo = o.lambda_(s["psi"], Fo)
+    o.lambda_apply(s["psi"], lamo, Fo)
+    psio = s["psi"].copy()
+    o.sd_update(o.cmass(25.0, 2.0), dt, Fo, psio)
+    o.red_sync(psio)
+    So = o.overlap(psio)
+    To, ito, erro = oracle.ns(So, tol=1e-15, max_iter=30)
+    outo = o.rotate(psio, To)
+    errs = dict(lam=rel(lam, lamo), force=rel(F, Fo), fnorm=abs(fn - np.linalg.norm(Fo)) / np.linalg.norm(Fo),
+                psi_sd=rel(psi_sd, psio), S=rel(S, So), T=rel(T, To), psi_out=rel(psi_out, outo))
+    report(f"[{name}] chain it={it}/{abs(ito)} err={err:.2e}", **errs)
+    assert ok and it >= 1
+    for k, v in errs.items():
+        assert v < TOL, (k, v)
+    # orthonormal output (norm-2 convention)
+    Sout = system.gamma_overlap(psi_out, s["kx"])
+    assert np.abs(Sout / 2 - np.eye(s["nstates"])).max() < 1e-12
+
+
+@pytest.mark.parametrize("name", ["small", "water8"])
+def test_full_steps_match_oracle(gpu, name):
+    s = get_system(name)
+    o = make_oracle(s)
+    psio = s["psi"].copy()
+    with make_ctx(s, dt=5.0, ns_tol=1e-15, ns_max_iter=30) as c:
+        c.set_psi(s["psi"])
+        c.set_vks(s["vks"])
+        for k in range(3):
+            r = c.step()
+            ro = o.step(s["vks"], psio, dt=5.0, ns_tol=1e-15, ns_max_iter=30, use_lambda=True)
+            psi = c.get_psi()
+            de = abs(r["eelec"] - ro["eelec"]) / abs(ro["eelec"])
+            report(f"[{name}] step{k}", dE=de, eke=abs(r["eke"] - ro["eke"]) / abs(ro["eke"]),
+                   psi=rel(psi, psio), resid=r["resid_max"])
+            assert de < TOL
+            assert rel(psi, psio) < TOL
+            assert r["resid_max"] < 1e-12
+            assert r["ns_iters"] >= 1
+
+
+def test_mixed_precision_errors_stated(gpu):
+    """Mixed mode (FP32 FFT chain, FP64 master coefficients and FP64 GEMMs):
+    errors against the FP64 oracle for one step."""
+    s = get_system("water8")
+    o = make_oracle(s)
+    psio = s["psi"].copy()
+    ro = o.step(s["vks"], psio, dt=5.0, ns_tol=1e-15, ns_max_iter=30, use_lambda=True, want=("rho",))
+    with make_ctx(s, precision="mixed", dt=5.0) as c:
+        c.set_psi(s["psi"])
+        c.set_vks(s["vks"])
+        r = c.step()
+        rho = c.get_density()
+        psi = c.get_psi()
+    natoms = s["natoms"]
+    dE_atom = abs(r["eelec"] - ro["eelec"]) / natoms
+    report("[water8 mixed]", dE_per_atom=dE_atom, rho=rel(rho, ro["rho"]), psi=rel(psi, psio),
+           resid=r["resid_max"])
+    assert dE_atom < 1e-6
+    assert rel(rho, ro["rho"]) < 1e-5
+    assert r["resid_max"] < 1e-10
+
+
+def test_enumerated_layout_equals_explicit(gpu):
+    """npts = 0 (library enumerates) gives the same k-vectors as passing them."""
+    s = get_system("water8")
+    c = CPContext(s["grid"], s["hmat"], s["ecut_ry"], s["nstates"])
+    kx, ky, kz = c.kvecs()
+    c.close()
+    assert np.array_equal(kx, s["kx"]) and np.array_equal(ky, s["ky"]) and np.array_equal(kz, s["kz"])
+
+
+def test_bad_layout_rejected(gpu):
+    s = get_system("small")
+    kx = s["kx"].copy()
+    ky = s["ky"].copy()
+    kz = s["kz"].copy()
+    # break kz contiguity inside a line
+    i = int(np.where((kx > 0))[0][3])
+    kz[i] += 5
+    from openatom_amd import OaError
+    with pytest.raises(OaError):
+        CPContext(s["grid"], s["hmat"], s["ecut_ry"], s["nstates"], kx, ky, kz)

@@ -1,0 +1,250 @@
+"""Pins of the CPU oracle (oracle/oa_oracle.c) against numpy, brute force,
+closed forms and invariants.  CPU only.  Each test names the oracle function
+it pins and a mistake it would catch."""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+from openatom_amd import system
+from tests.helpers import dense_grid, psi_r_numpy, rel, small_system
+
+
+@pytest.fixture(scope="module")
+def small():
+    s = small_system()
+    o = oracle.System(s["grid"], s["hmat"], s["ecut_ry"], s["nstates"], s["kx"], s["ky"], s["kz"])
+    return s, o
+
+
+# ---------------------------------------------------------------- FFT
+
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 7, 12, 45, 48, 54, 100, 128, 160])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_fft1d_vs_numpy_and_bruteforce(n, sign):
+    """oo_fft1d: sign convention (FFTW_FORWARD = e^{-i}) and no normalisation."""
+    rng = np.random.default_rng(n)
+    x = rng.normal(size=n) + 1j * rng.normal(size=n)
+    ref = np.fft.fft(x) if sign < 0 else np.fft.ifft(x) * n
+    assert rel(oracle.fft1d(x, sign), ref) < 1e-13
+    assert rel(oracle.dft1d(x, sign), ref) < 1e-13
+
+
+# ---------------------------------------------------------------- k-vectors
+
+@pytest.mark.parametrize("name,count", [("water8", 2395), ("water32", 32961), ("water128", 130496)])
+def test_stored_counts(name, count):
+    """oo_stored_kvecs: stored points incl. the redundant kx=0 half."""
+    w = system.WORKLOADS[name]
+    h = system.cubic_hmat(w["box_ang"] * system.BOHR_PER_ANG)
+    kx, ky, kz = oracle.stored_kvecs(w["ecut_ry"], h)
+    assert kx.size == count
+
+
+def test_kvec_set_bruteforce():
+    """oo_kvec_half + oo_flip: the stored set is exactly {|G|^2 <= Ecut, kx >= 0}
+    (a cubic and an orthorhombic cell), G=0 once, the kx=0 plane closed under
+    G -> -G, and the layout rules of stateSlab.C:285-380."""
+    for box, ecut in (((6.21, 6.21, 6.21), 30.0), ((5.0, 5.4, 5.8), 14.0)):
+        hmat = np.diag([b * system.BOHR_PER_ANG for b in box]).reshape(9)
+        kx, ky, kz = oracle.stored_kvecs(ecut, hmat)
+        got = set(zip(kx.tolist(), ky.tolist(), kz.tolist()))
+        assert len(got) == kx.size
+        hi = np.linalg.inv(hmat.reshape(3, 3))
+        kmax = [int(np.sqrt(ecut) / (2 * np.pi * np.linalg.norm(hi[:, d]))) + 2 for d in range(3)]
+        want = set()
+        for a in range(0, kmax[0] + 1):
+            for b in range(-kmax[1], kmax[1] + 1):
+                for c in range(-kmax[2], kmax[2] + 1):
+                    G = 2 * np.pi * hi @ np.array([a, b, c], float)
+                    if G @ G <= ecut:
+                        want.add((a, b, c))
+        assert got == want
+        n0 = int((kx == 0).sum())
+        assert (kx[:n0] == 0).all() and (kx[n0:] > 0).all()
+        red = (kx == 0) & ((ky < 0) | ((ky == 0) & (kz < 0)))
+        nred = int(red.sum())
+        assert red[:nred].all() and not red[nred:].any()
+        assert (kx[nred], ky[nred], kz[nred]) == (0, 0, 0)
+        # lines contiguous, kz ascending by one
+        for i in range(1, kx.size):
+            if kx[i] == kx[i - 1] and ky[i] == ky[i - 1]:
+                assert kz[i] == kz[i - 1] + 1
+
+
+# ---------------------------------------------------------------- G <-> R
+
+def test_g2r_vs_numpy(small):
+    """oo_g2r: psi(r) = sum c e^{-iG.r} on an unequal grid and orthorhombic cell
+    (catches a swapped axis, a wrong sign, a missing conjugate-mirror term)."""
+    s, o = small
+    for st in range(3):
+        c = s["psi"][st]
+        ref = psi_r_numpy(c, s["kx"], s["ky"], s["kz"], s["grid"])
+        assert rel(o.g2r(c), ref) < 1e-13
+
+
+def test_g2r_parseval_and_real(small):
+    s, o = small
+    c = s["psi"][0]
+    pr = o.g2r(c)
+    full = np.fft.fftn(dense_grid(c, s["kx"], s["ky"], s["kz"], s["grid"]))
+    assert np.abs(full.imag).max() < 1e-12 * np.abs(full.real).max()
+    N = np.prod(s["grid"])
+    assert abs((pr ** 2).sum() / N - 2.0) < 1e-12  # norm-2 states
+
+
+def test_g2r_nonhermitian_kx0_takes_real_part(small):
+    """FFTW c2r ignores Im of the kx=0 column: a non-Hermitian kx=0 plane gives
+    psi = Re(sum_kx0 c e^{-iGr}) + 2 Re(sum_kx>0 c e^{-iGr})."""
+    s, o = small
+    rng = np.random.default_rng(9)
+    c = rng.normal(size=len(s["kx"])) + 1j * rng.normal(size=len(s["kx"]))
+    nx, ny, nz = s["grid"]
+    kx, ky, kz = s["kx"], s["ky"], s["kz"]
+    x = np.arange(nx); y = np.arange(ny); z = np.arange(nz)
+    ph = lambda a, b, cc: np.exp(-2j * np.pi * (cc * z[:, None, None] / nz + b * y[None, :, None] / ny
+                                                + a * x[None, None, :] / nx))
+    ref = np.zeros((nz, ny, nx))
+    for v, a, b, cc in zip(c, kx, ky, kz):
+        term = v * ph(a, b, cc)
+        ref += term.real if a == 0 else 2.0 * term.real
+    assert rel(o.g2r(c), ref) < 1e-12
+
+
+def test_density_sum_rule(small):
+    """oo_density: rho = (1/Omega) sum occ psi^2; (Omega/N) sum rho = 2 nstates."""
+    s, o = small
+    pr = o.g2r_all(s["psi"])
+    rho = o.density(pr)
+    assert rel(rho, (pr ** 2).sum(axis=0) / o.volume) < 1e-14
+    N = np.prod(s["grid"])
+    assert abs(rho.sum() * o.volume / N - 2 * s["nstates"]) < 1e-10
+    assert rho.min() >= 0.0
+
+
+def test_vpsi_r2g_vs_numpy(small):
+    """oo_vpsi_r2g: F(G) = -(2/N) sum_r V psi e^{+iG.r} (doIFFT, before addForces)."""
+    s, o = small
+    pr = o.g2r_all(s["psi"])
+    N = np.prod(s["grid"])
+    for st in range(2):
+        f = o.vpsi_r2g(st, pr[st], s["vks"])
+        full = np.fft.ifftn(s["vks"] * pr[st]) * N
+        nx, ny, nz = s["grid"]
+        ref = full[s["kz"] % nz, s["ky"] % ny, s["kx"] % nx] * (-2.0 / N)
+        assert rel(f, ref) < 1e-13
+
+
+def test_eke_force_closed_form(small):
+    """oo_eke_force: addForces doubles kx>0, F -= w g^2 psi, EKE = 1/2 sum_full g^2|c|^2."""
+    s, o = small
+    rng = np.random.default_rng(2)
+    f0 = rng.normal(size=s["psi"].shape) + 1j * rng.normal(size=s["psi"].shape)
+    f = f0.copy()
+    eke = o.eke_force(s["psi"], f)
+    g2 = system.gvec_sq(s["kx"], s["ky"], s["kz"], s["hmat"])
+    w = system.gamma_weights(s["kx"])
+    ref_f = f0 * w[None, :] - s["psi"] * (w * g2)[None, :]
+    assert rel(f, ref_f) < 1e-14
+    ref_e = 0.5 * (w[None, :] * g2[None, :] * np.abs(s["psi"]) ** 2).sum()
+    assert abs(eke - ref_e) < 1e-12 * abs(ref_e)
+
+
+def test_overlap_and_lambda(small):
+    """oo_overlap = full-sphere overlap (2I for norm-2 orthonormal input);
+    oo_lambda = Re sum_stored conj(psi_i) F_j; oo_lambda_apply weights 1/2, 1."""
+    s, o = small
+    S = o.overlap(s["psi"])
+    assert np.abs(S - 2 * np.eye(s["nstates"])).max() < 1e-12
+    rng = np.random.default_rng(4)
+    psi = s["psi"] + 0.01 * (rng.normal(size=s["psi"].shape) + 1j * rng.normal(size=s["psi"].shape))
+    assert rel(o.overlap(psi), system.gamma_overlap(psi, s["kx"])) < 1e-13
+    F = rng.normal(size=psi.shape) + 1j * rng.normal(size=psi.shape)
+    lam = o.lambda_(psi, F)
+    ref = (np.conj(psi) @ F.T).real
+    assert rel(lam, ref) < 1e-13
+    F2 = F.copy()
+    o.lambda_apply(psi, lam, F2)
+    w = np.where(s["kx"] == 0, 0.5, 1.0)
+    assert rel(F2, F - w[None, :] * (lam.T @ psi)) < 1e-13
+
+
+def test_lambda_projection_is_tangent(small):
+    """After F -= w Psi Lambda the true gradient is orthogonal to every state
+    (full-sphere inner product with the kx>0 force halved)."""
+    s, o = small
+    rng = np.random.default_rng(5)
+    F = rng.normal(size=s["psi"].shape) + 1j * rng.normal(size=s["psi"].shape)
+    red, par = system.redundant_pairs(s["kx"], s["ky"], s["kz"])
+    F[:, red] = np.conj(F[:, par])
+    lam = o.lambda_(s["psi"], F)
+    o.lambda_apply(s["psi"], lam, F)
+    w = np.where(s["kx"] == 0, 1.0, 0.5)  # per-coefficient gradient
+    Ft = F * w[None, :]
+    ov = (system.gamma_weights(s["kx"])[None, :] * np.conj(s["psi"]) @ Ft.T).real
+    assert np.abs(ov).max() < 1e-12 * np.abs(F).max()
+
+
+def test_ns_vs_eigh():
+    """oo_ns converges to (S/2)^{-1/2} (get_diag_Tmat's closed form,
+    cp_lowdin.C:136-153) and reports rms changes; T (S/2) T = I."""
+    rng = np.random.default_rng(1)
+    n = 24
+    A = rng.normal(size=(n, n)) * 0.05
+    S = 2.0 * (np.eye(n) + 0.5 * (A + A.T))
+    T, it, err = oracle.ns(S, tol=1e-14, max_iter=50)
+    lam, U = np.linalg.eigh(S / 2)
+    Tref = (U / np.sqrt(lam)) @ U.T
+    assert it > 0 and err <= 1e-14
+    assert rel(T, Tref) < 1e-13
+    assert np.abs(T @ (S / 2) @ T - np.eye(n)).max() < 1e-13
+
+
+def test_ns_nonconvergence_reported():
+    """max_iter reached with err > tol -> negative iteration count (the
+    reference aborts, ortho.C:129-133)."""
+    rng = np.random.default_rng(2)
+    n = 16
+    Q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    S = 2.0 * (Q * np.linspace(0.3, 2.5, n)) @ Q.T
+    T, it, err = oracle.ns(S, tol=1e-15, max_iter=3)
+    assert it == -3 and err > 1e-15
+
+
+def test_rotate_cmass_sd_redsync(small):
+    s, o = small
+    rng = np.random.default_rng(6)
+    n = s["nstates"]
+    T = np.eye(n) + 0.01 * rng.normal(size=(n, n))
+    assert rel(o.rotate(s["psi"], T), T.T @ s["psi"]) < 1e-14
+    cm = o.cmass(25.0, 2.0)
+    g2 = system.gvec_sq(s["kx"], s["ky"], s["kz"], s["hmat"])
+    w = system.gamma_weights(s["kx"])
+    tau = 25.0 / 0.0241888
+    m0 = 2 * tau * tau * 0.0005
+    ref = np.where(g2 > 2.0, w * g2 * m0 / 2.0, w * m0)
+    assert rel(cm, ref) < 1e-14
+    psi = s["psi"].copy()
+    F = rng.normal(size=psi.shape) + 1j * rng.normal(size=psi.shape)
+    o.sd_update(cm, 5.0, F, psi)
+    assert rel(psi, s["psi"] + F * (5.0 / cm)[None, :]) < 1e-14
+    o.red_sync(psi)
+    red, par = system.redundant_pairs(s["kx"], s["ky"], s["kz"])
+    assert np.array_equal(psi[:, red], np.conj(psi[:, par]))
+
+
+def test_step_orthonormal_and_descends(small):
+    """oo_step: output is orthonormal (S = 2I) and, at a small dt, E_elec
+    decreases along successive steps with fixed V."""
+    s, o = small
+    psi = s["psi"].copy()
+    es = []
+    for _ in range(4):
+        r = o.step(s["vks"], psi, dt=2.0, ns_tol=1e-14, ns_max_iter=50, use_lambda=True)
+        assert r["rc"] == 0
+        es.append(r["eelec"])
+    assert all(b < a for a, b in zip(es, es[1:]))
+    S = o.overlap(psi)
+    assert np.abs(S - 2 * np.eye(s["nstates"])).max() < 1e-12
