@@ -21,7 +21,8 @@ namespace oa {
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 // D(16x16) += A(16x4) B(4x16).  Lane l supplies A[l%16][l/16] and
-// B[l/16][l%16]; it receives D[4*(l/16)+v][l%16] in element v.
+// B[l/16][l%16]; it receives D[l/16 + 4*v][l%16] in element v (layout
+// measured on gfx950 with a single-wave probe).
 __device__ __forceinline__ v4d mfma16(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(256) void k_gram(GramArgs a, int ntile) {
     for (int y = 0; y < 2; y++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int row = ti * GT + wi * 32 + x * 16 + 4 * lq + v;
+        const int row = ti * GT + wi * 32 + x * 16 + lq + 4 * v;
         const int col = tj * GT + wj * 32 + y * 16 + l16;
         if (row < a.n && col < a.n) a.part[((size_t)sp * a.n + row) * a.n + col] = acc[x][y][v];
       }
@@ -169,7 +170,7 @@ __global__ __launch_bounds__(256) void k_rotate(RotArgs a) {
     for (int y = 0; y < 2; y++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int j = j0 + wj * 32 + x * 16 + 4 * lq + v;
+        const int j = j0 + wj * 32 + x * 16 + lq + 4 * v;
         const int k = k0 + wk * 32 + y * 16 + l16;
         if (j < a.n && k < a.K) {
           const double s = (k < a.k_lo) ? a.s_lo : a.s_hi;
@@ -220,7 +221,7 @@ __device__ double small_gemm_tile(const double* __restrict__ A, const double* __
   double err = 0.0;
 #pragma unroll
   for (int v = 0; v < 4; v++) {
-    const int row = ti * NT + wi * 16 + 4 * lq + v;
+    const int row = ti * NT + wi * 16 + lq + 4 * v;
     const int col = tj * NT + wj * 16 + l16;
     if (row < n && col < n) {
       double val = alpha * acc[v] + (row == col ? gamma : 0.0);
