@@ -29,7 +29,8 @@ METRIC = ("CP steps/s (128 states, 70 Ry) mixed vs FP64 at ΔE≤1e-6 Ha/atom; "
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_TFLOPS = 78.6
 
-STAGES = ("k_zfwd", "k_y_fwd", "k_xc2r", "k_xr2c", "k_y_bwd", "k_zbwd")
+STAGES = ("k_zfwd", "k_y_fwd", "k_xc2r", "k_xr2c", "k_y_bwd", "k_zbwd",
+          "k_plane_fwd", "k_plane_bwd")
 
 
 def log(*a):
@@ -39,13 +40,19 @@ def log(*a):
 def stage_bytes(stage, geo, csz, rsz):
     """Algorithmic bytes per state of one FFT pass in the reference's 3-pass
     pruned model: B_z = 2 n_lines Nz c, B_y = 2 nplane_x Nz Ny c,
-    B_x = Ny Nz (Nx/2+1) c + Nx Ny Nz r (c, r: complex and real sizes)."""
+    B_x = Ny Nz (Nx/2+1) c + Nx Ny Nz r (c, r: complex and real sizes).
+    A fused plane kernel does the y and the x pass: B_y + B_x."""
     nx, ny, nz, nl, npx = geo["nx"], geo["ny"], geo["nz"], geo["nlines"], geo["nplane_x"]
+    bz = 2 * nl * nz * csz
+    by = 2 * npx * nz * ny * csz
+    bx = ny * nz * (nx // 2 + 1) * csz + nx * ny * nz * rsz
     if stage in ("k_zfwd", "k_zbwd"):
-        return 2 * nl * nz * csz
+        return bz
     if stage in ("k_y_fwd", "k_y_bwd"):
-        return 2 * npx * nz * ny * csz
-    return ny * nz * (nx // 2 + 1) * csz + nx * ny * nz * rsz
+        return by
+    if stage in ("k_plane_fwd", "k_plane_bwd"):
+        return by + bx
+    return bx
 
 
 def run_precision(sysd, precision, steps, warmup, device=0):
@@ -69,9 +76,10 @@ def run_precision(sysd, precision, steps, warmup, device=0):
     tim = c.timings()
     geo = dict(nx=c.nx, ny=c.ny, nz=c.nz, nlines=c.nlines, nplane_x=c.nplane_x, npts=c.npts)
     psi = c.get_psi()
+    cfg = dict(c.config)
     c.close()
     return dict(precision=precision, ms=(t1 - t0) * 1e3 / steps, steps_per_s=steps / (t1 - t0),
-                first=first, last=last, timings=tim, geo=geo, psi=psi)
+                first=first, last=last, timings=tim, geo=geo, psi=psi, launch=cfg)
 
 
 def roofline(res, nst, traffic=None):
@@ -81,7 +89,7 @@ def roofline(res, nst, traffic=None):
     csz, rsz = (8, 4) if fp32 else (16, 8)
     geo = res["geo"]
     K = 2 * geo["npts"]
-    cands = {k: tim[k] for k in STAGES + ("overlap", "rotate")}
+    cands = {k: tim[k] for k in STAGES + ("overlap", "rotate") if k in tim}
     dom = max(cands, key=cands.get)
     ms = cands[dom]
     if dom in STAGES:
@@ -191,7 +199,7 @@ def main():
                    "global_batch": nst, "seq_len": r64["geo"]["npts"], "parallelism": "none (1 GPU)"},
         "fp64": {"steps_per_s": round(r64["steps_per_s"], 3), "ms_per_step": round(r64["ms"], 4),
                  "ns_iters": r64["last"]["ns_iters"], "resid_max": r64["last"]["resid_max"],
-                 "roofline": rl64, "timings_ms": r64["timings"]},
+                 "roofline": rl64, "timings_ms": r64["timings"], "launch": r64["launch"]},
         "speedup_mixed_vs_fp64": round(rmx["steps_per_s"] / r64["steps_per_s"], 3),
         "quality": {"dE_per_atom_mixed_vs_fp64_after_steps": de_atom,
                     "dE_per_atom_first_step": de_first,
@@ -199,6 +207,7 @@ def main():
                     "ns_iters_mixed": rmx["last"]["ns_iters"],
                     "steps_compared": args.warmup + args.steps},
         "timings_ms": rmx["timings"],
+        "launch": rmx["launch"],
         "roofline": rl,
         "cpu_baseline": cpu,
     }

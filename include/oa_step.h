@@ -66,6 +66,8 @@ typedef struct {
   double ecut_mass;    /* cp_mass_cut_def (default 2) */
   int use_lambda;      /* project F with Lambda (cp_min_opt = 1) */
   int want_resid;      /* compute ||Psi^H Psi/2 - I|| after each step */
+  int fft_mode;        /* 0: two-factor register engine where the lengths allow (default),
+                          1: Stockham LDS kernels, one per pass, 2: Stockham fused planes */
 } oa_options;
 
 typedef struct {
@@ -99,6 +101,12 @@ const char *oa_last_error(const oa_ctx *ctx);
 
 /* sizes of the context's layout: npts, nlines, nplane_x, nkx0 */
 oa_status oa_layout_info(const oa_ctx *ctx, int info[4]);
+/* launch shapes in use: [0] FFT engine (0 two-factor register engine, 1
+ * Stockham one kernel per pass, 2 Stockham fused y+x planes), [1] z-lines,
+ * [2] y columns, [3] x pair-rows per workgroup, [4] y columns per batch
+ * (fused), [5] x pair-rows per batch (fused), [6] states per workgroup in
+ * the x passes, [7] state chunks (rho partials) */
+oa_status oa_get_config(const oa_ctx *ctx, int cfg[8]);
 /* the stored k-vectors in use */
 oa_status oa_get_kvecs(const oa_ctx *ctx, int *kx, int *ky, int *kz);
 

@@ -19,7 +19,7 @@ STATUS_NAMES = {0: "OA_OK", 1: "OA_EINVAL", 2: "OA_EHIP", 3: "OA_ENOMEM",
 # every symbol include/oa_step.h declares
 EXPORTS = (
     "oa_default_options", "oa_kvecs", "oa_create", "oa_destroy", "oa_last_error",
-    "oa_layout_info", "oa_get_kvecs", "oa_psi_set", "oa_psi_get", "oa_vks_set",
+    "oa_layout_info", "oa_get_config", "oa_get_kvecs", "oa_psi_set", "oa_psi_get", "oa_vks_set",
     "oa_fft_g2r", "oa_psir_get", "oa_density_get", "oa_vpsi_r2g", "oa_force_get",
     "oa_force_set", "oa_lambda", "oa_sd_update", "oa_overlap", "oa_ortho_ns",
     "oa_tmat_set", "oa_rotate", "oa_step", "oa_sync", "oa_step_result", "oa_timings",
@@ -38,7 +38,8 @@ class OaOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("ns_max_iter", ctypes.c_int),
                 ("ns_tol", ctypes.c_double), ("dt", ctypes.c_double),
                 ("tau_mass", ctypes.c_double), ("ecut_mass", ctypes.c_double),
-                ("use_lambda", ctypes.c_int), ("want_resid", ctypes.c_int)]
+                ("use_lambda", ctypes.c_int), ("want_resid", ctypes.c_int),
+                ("fft_mode", ctypes.c_int)]
 
 
 class OaStepOut(ctypes.Structure):
@@ -80,7 +81,7 @@ def load():
                             ctypes.POINTER(vp)]
     L.oa_last_error.argtypes = [vp]
     L.oa_last_error.restype = ctypes.c_char_p
-    for name, args in (("oa_destroy", [vp]), ("oa_layout_info", [vp, vp]),
+    for name, args in (("oa_destroy", [vp]), ("oa_layout_info", [vp, vp]), ("oa_get_config", [vp, vp]),
                        ("oa_get_kvecs", [vp, vp, vp, vp]), ("oa_psi_set", [vp, vp]),
                        ("oa_psi_get", [vp, vp]), ("oa_vks_set", [vp, vp]),
                        ("oa_fft_g2r", [vp]), ("oa_psir_get", [vp, ci, vp]),

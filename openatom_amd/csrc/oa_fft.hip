@@ -19,6 +19,8 @@
 // one complex transform.  rho is accumulated in registers by workgroups that
 // own a block of rows and loop over all states, so it is written once and
 // deterministically.
+#include <algorithm>
+
 #include "oa_common.h"
 #include "oa_kernels.h"
 
@@ -456,9 +458,308 @@ __global__ __launch_bounds__(OA_NTHR) void k_xr2c(Geom g, FftPlan px, int npair,
   }
 }
 
+// ---------------------------------------------------------------- fused planes
+//
+// One workgroup owns one z-plane for a chunk of `cs` states and keeps the
+// whole half plane Y[kx][y] (nplane_x x Ny complex) in LDS, so the y-FFT and
+// the x-FFT of a state meet in LDS instead of in HBM.  Every thread owns the
+// fixed plane positions r = tid + i*NTH (i < MAXS) for the whole launch:
+// that is where rho is accumulated (forward) and V is held (backward).  A
+// batch of pair-rows [ya, yb) covers the positions [ya*nx, yb*nx), so a slot
+// is live in a batch iff its position falls in that range.
+
+// y-FFT FFTW_FORWARD on the populated columns, conjugate + c2r per row as one
+// forward FFT of two Hermitian-extended rows, psi(r) out, sum occ psi^2 in
+// registers (fftCache.C:1167-1243; CP_State_RealSpacePlane.C:349-355).
+template <typename T, int NTH, int MAXS>
+__global__ __launch_bounds__(NTH) void k_plane_fwd(Geom g, FftPlan px, FftPlan py, int kc, int pc,
+                                                   int cs, const int* __restrict__ kylo,
+                                                   const int* __restrict__ kyhi,
+                                                   const double2* __restrict__ twx,
+                                                   const double2* __restrict__ twy,
+                                                   const cpx<T>* __restrict__ zbuf,
+                                                   const double* __restrict__ occ,
+                                                   T* __restrict__ psir,
+                                                   double* __restrict__ rho_part) {
+  extern __shared__ __align__(16) char smem_raw[];
+  const int nx = g.nx, ny = g.ny, npx = g.nplane_x;
+  const int scr = max(kc * ny, pc * nx);
+  cpx<T>* twX = reinterpret_cast<cpx<T>*>(smem_raw);
+  cpx<T>* twY = twX + nx;
+  cpx<T>* Y = twY + ny;
+  cpx<T>* s1 = Y + npx * ny;
+  cpx<T>* s2 = s1 + scr;
+  const int z = blockIdx.x, chunk = blockIdx.y;
+  const int st0 = chunk * cs, st1 = min(g.nst, st0 + cs);
+  const int NP = nx * ny;
+  const size_t N = (size_t)NP * g.nz;
+  const size_t plane = (size_t)npx * ny;
+  load_tw(twX, twx, nx);
+  load_tw(twY, twy, ny);
+  double acc[MAXS];
+#pragma unroll
+  for (int i = 0; i < MAXS; i++) acc[i] = 0.0;
+  const int npairs = (ny + 1) / 2;
+  for (int s = st0; s < st1; s++) {
+    // ---- y pass: load the populated columns, FFT in column chunks
+    const cpx<T>* zs = zbuf + ((size_t)s * g.nz + z) * plane;
+    for (int idx = threadIdx.x; idx < npx * ny; idx += NTH) {
+      const int kx = idx / ny, y = idx - kx * ny;
+      const int kyv = (y <= (ny - 1) / 2) ? y : y - ny;
+      Y[idx] = (kyv >= kylo[kx] && kyv <= kyhi[kx]) ? zs[idx] : cpx<T>{(T)0, (T)0};
+    }
+    __syncthreads();
+    for (int kx0 = 0; kx0 < npx; kx0 += kc) {
+      const int nk = min(kc, npx - kx0);
+      cpx<T>* a = Y + kx0 * ny;
+      cpx<T>* r = fft_lds<T, -1>(a, s1, py, nk, ny, twY);
+      if (r != a) {
+        for (int i = threadIdx.x; i < nk * ny; i += NTH) a[i] = r[i];
+        __syncthreads();
+      }
+    }
+    // ---- x pass in chunks of pair-rows
+    const double oc = occ[s];
+    T* pr = psir + (size_t)s * N + (size_t)z * NP;
+    for (int q0 = 0; q0 < npairs; q0 += pc) {
+      const int np = min(pc, npairs - q0);
+      for (int idx = threadIdx.x; idx < np * nx; idx += NTH) {
+        const int ql = idx / nx, k = idx - ql * nx;
+        const int y1 = 2 * (q0 + ql), y2 = y1 + 1;
+        cpx<T> Z = {(T)0, (T)0};
+        int kk = -1;
+        bool mirror = false;
+        if (k < npx) kk = k;
+        else if (k > nx - npx) { kk = nx - k; mirror = true; }
+        if (kk >= 0) {
+          const cpx<T> X1 = Y[kk * ny + y1];
+          const cpx<T> X2 = (y2 < ny) ? Y[kk * ny + y2] : cpx<T>{(T)0, (T)0};
+          if (kk == 0) Z = {X1.x, X2.x};
+          else if (!mirror) Z = {X1.x - X2.y, X1.y + X2.x};
+          else Z = {X1.x + X2.y, X2.x - X1.y};
+        }
+        s1[idx] = Z;
+      }
+      __syncthreads();
+      cpx<T>* r = fft_lds<T, -1>(s1, s2, px, np, nx, twX);
+      const int ya = 2 * q0, yb = min(ny, 2 * (q0 + np));
+      const int ra = ya * nx, rb = yb * nx;
+#pragma unroll
+      for (int i = 0; i < MAXS; i++) {
+        const int rr = threadIdx.x + i * NTH;
+        if (rr >= ra && rr < rb) {
+          const int off = rr - ra;
+          const int yl = off / nx;  // ya is even: yl carries the row parity
+          const int x = off - yl * nx;
+          const cpx<T> v = r[(yl >> 1) * nx + x];
+          const T val = (yl & 1) ? v.y : v.x;
+          pr[rr] = val;
+          acc[i] += oc * (double)val * (double)val;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  double* rp = rho_part + (size_t)chunk * N + (size_t)z * NP;
+#pragma unroll
+  for (int i = 0; i < MAXS; i++) {
+    const int rr = threadIdx.x + i * NTH;
+    if (rr < NP) rp[rr] = acc[i];
+  }
+}
+
+// V psi occ product, FFTW r2c + conjugate per row (two real rows per complex
+// FFT), y-FFT FFTW_BACKWARD on the populated columns
+// (CP_State_RealSpacePlane.C:580-596; fftCache.C:1252-1319).
+template <typename T, int NTH, int MAXS>
+__global__ __launch_bounds__(NTH) void k_plane_bwd(Geom g, FftPlan px, FftPlan py, int kc, int pc,
+                                                   int cs, const int* __restrict__ kylo,
+                                                   const int* __restrict__ kyhi,
+                                                   const double2* __restrict__ twx,
+                                                   const double2* __restrict__ twy,
+                                                   const T* __restrict__ psir,
+                                                   const double* __restrict__ vks,
+                                                   const double* __restrict__ occ,
+                                                   cpx<T>* __restrict__ zbuf) {
+  extern __shared__ __align__(16) char smem_raw[];
+  const int nx = g.nx, ny = g.ny, npx = g.nplane_x;
+  const int scr = max(kc * ny, pc * nx);
+  cpx<T>* twX = reinterpret_cast<cpx<T>*>(smem_raw);
+  cpx<T>* twY = twX + nx;
+  cpx<T>* Y = twY + ny;
+  cpx<T>* s1 = Y + npx * ny;
+  cpx<T>* s2 = s1 + scr;
+  const int z = blockIdx.x, chunk = blockIdx.y;
+  const int st0 = chunk * cs, st1 = min(g.nst, st0 + cs);
+  const int NP = nx * ny;
+  const size_t N = (size_t)NP * g.nz;
+  const size_t plane = (size_t)npx * ny;
+  load_tw(twX, twx, nx);
+  load_tw(twY, twy, ny);
+  T vv[MAXS];
+  const double* vz = vks + (size_t)z * NP;
+#pragma unroll
+  for (int i = 0; i < MAXS; i++) {
+    const int rr = threadIdx.x + i * NTH;
+    vv[i] = (rr < NP) ? (T)vz[rr] : (T)0;
+  }
+  const int npairs = (ny + 1) / 2;
+  for (int s = st0; s < st1; s++) {
+    const T oc = (T)occ[s];
+    const T* pr = psir + (size_t)s * N + (size_t)z * NP;
+    // ---- x pass: rows -> populated kx of the half plane Y[kx][y] (conjugated)
+    for (int q0 = 0; q0 < npairs; q0 += pc) {
+      const int np = min(pc, npairs - q0);
+      const int ya = 2 * q0, yb = min(ny, 2 * (q0 + np));
+      const int ra = ya * nx, rb = yb * nx;
+#pragma unroll
+      for (int i = 0; i < MAXS; i++) {
+        const int rr = threadIdx.x + i * NTH;
+        if (rr >= ra && rr < rb) {
+          const int off = rr - ra;
+          const int yl = off / nx;
+          const int x = off - yl * nx;
+          const T f = pr[rr] * (vv[i] * oc);
+          T* dst = reinterpret_cast<T*>(&s1[(yl >> 1) * nx + x]);
+          dst[yl & 1] = f;
+        }
+      }
+      if ((ny & 1) && q0 + np == npairs) {
+        // the last pair of an odd Ny has no second row
+        for (int x = threadIdx.x; x < nx; x += NTH) s1[(np - 1) * nx + x].y = (T)0;
+      }
+      __syncthreads();
+      cpx<T>* r = fft_lds<T, -1>(s1, s2, px, np, nx, twX);
+      for (int idx = threadIdx.x; idx < np * npx; idx += NTH) {
+        const int ql = idx / npx, k = idx - ql * npx;
+        const int y1 = ya + 2 * ql, y2 = y1 + 1;
+        const cpx<T> Zk = r[ql * nx + k];
+        const cpx<T> Zm = r[ql * nx + (k == 0 ? 0 : nx - k)];
+        const cpx<T> c1 = {(T)0.5 * (Zk.x + Zm.x), -(T)0.5 * (Zk.y - Zm.y)};
+        const cpx<T> c2 = {(T)0.5 * (Zk.y + Zm.y), (T)0.5 * (Zk.x - Zm.x)};
+        Y[k * ny + y1] = c1;
+        if (y2 < ny) Y[k * ny + y2] = c2;
+      }
+      __syncthreads();
+    }
+    // ---- y pass backward, column chunks, populated ky written out
+    cpx<T>* zs = zbuf + ((size_t)s * g.nz + z) * plane;
+    for (int kx0 = 0; kx0 < npx; kx0 += kc) {
+      const int nk = min(kc, npx - kx0);
+      cpx<T>* a = Y + kx0 * ny;
+      cpx<T>* r = fft_lds<T, +1>(a, s1, py, nk, ny, twY);
+      for (int idx = threadIdx.x; idx < nk * ny; idx += NTH) {
+        const int kxl = idx / ny, y = idx - kxl * ny;
+        const int kx = kx0 + kxl;
+        const int kyv = (y <= (ny - 1) / 2) ? y : y - ny;
+        if (kyv >= kylo[kx] && kyv <= kyhi[kx]) zs[(size_t)kx * ny + y] = r[idx];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// rho = (1/Omega) sum_chunks rho_part; per-workgroup partials of sum V rho.
+__global__ void k_rho_reduce(const double* __restrict__ part, int nchunk, size_t n,
+                             double inv_vol, const double* __restrict__ vks,
+                             double* __restrict__ rho, double* __restrict__ dpart) {
+  __shared__ double red[16];
+  double e = 0.0;
+  for (size_t r = blockIdx.x * (size_t)blockDim.x + threadIdx.x; r < n;
+       r += (size_t)gridDim.x * blockDim.x) {
+    double a = 0.0;
+    for (int c = 0; c < nchunk; c++) a += part[(size_t)c * n + r];
+    a *= inv_vol;
+    rho[r] = a;
+    e += a * vks[r];
+  }
+  for (int o = 32; o > 0; o >>= 1) e += __shfl_down(e, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += red[w];
+    dpart[blockIdx.x] = t;
+  }
+}
+
 // ---------------------------------------------------------------- launchers
 
 static size_t fft_smem(int n, int batch, size_t csz) { return csz * ((size_t)n + 2 * (size_t)batch * n); }
+
+size_t plane_smem(const FftCtx& c, bool fp32) {
+  const size_t cs = fp32 ? 8 : 16;
+  const Geom& g = c.g;
+  const size_t scr = (size_t)std::max(c.kcp * g.ny, c.pcp * g.nx);
+  return cs * ((size_t)g.nx + g.ny + (size_t)g.nplane_x * g.ny + 2 * scr);
+}
+
+template <typename T, int NTH, int MAXS>
+hipError_t launch_plane(const FftCtx& c, bool fwd, hipStream_t st) {
+  const Geom& g = c.g;
+  const dim3 grid(g.nz, c.nchunk);
+  const size_t smem = plane_smem(c, sizeof(T) == 4);
+  // raise the dynamic LDS limit once per kernel instantiation
+  static size_t granted_fwd = 65536, granted_bwd = 65536;
+  if (fwd) {
+    auto k = k_plane_fwd<T, NTH, MAXS>;
+    if (smem > granted_fwd) {
+      hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)smem);
+      if (e != hipSuccess) return e;
+      granted_fwd = smem;
+    }
+    hipLaunchKernelGGL(k, grid, dim3(NTH), smem, st, g, c.px, c.py, c.kcp, c.pcp, c.cs, c.kylo,
+                       c.kyhi, c.twx, c.twy, (const cpx<T>*)c.zbuf, c.occ, (T*)c.psir,
+                       c.rho_part);
+  } else {
+    auto k = k_plane_bwd<T, NTH, MAXS>;
+    if (smem > granted_bwd) {
+      hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)smem);
+      if (e != hipSuccess) return e;
+      granted_bwd = smem;
+    }
+    hipLaunchKernelGGL(k, grid, dim3(NTH), smem, st, g, c.px, c.py, c.kcp, c.pcp, c.cs, c.kylo,
+                       c.kyhi, c.twx, c.twy, (const T*)c.psir, c.vks, c.occ, (cpx<T>*)c.zbuf);
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_plane_sel(const FftCtx& c, bool fwd, hipStream_t st) {
+  switch (c.plane_cfg) {
+    case 0: return launch_plane<T, 1024, 4>(c, fwd, st);
+    case 1: return launch_plane<T, 1024, 10>(c, fwd, st);
+    case 2: return launch_plane<T, 1024, 16>(c, fwd, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// register-slot variant for a plane of np points (1024 threads each)
+int plane_config_for(int np) {
+  if (np <= 1024 * 4) return 0;
+  if (np <= 1024 * 10) return 1;
+  if (np <= 1024 * 16) return 2;
+  return -1;
+}
+
+int plane_threads(int) { return 1024; }
+
+hipError_t plane_fwd(const FftCtx& c, bool fp32, hipStream_t st) {
+  return fp32 ? launch_plane_sel<float>(c, true, st) : launch_plane_sel<double>(c, true, st);
+}
+
+hipError_t plane_bwd(const FftCtx& c, bool fp32, hipStream_t st) {
+  return fp32 ? launch_plane_sel<float>(c, false, st) : launch_plane_sel<double>(c, false, st);
+}
+
+hipError_t rho_reduce(const FftCtx& c, double* dpart, int ndpart, hipStream_t st) {
+  const size_t n = (size_t)c.g.nx * c.g.ny * c.g.nz;
+  hipLaunchKernelGGL(k_rho_reduce, dim3(ndpart), dim3(256), 0, st, c.rho_part, c.nchunk, n,
+                     c.inv_vol, c.vks, c.rho, dpart);
+  return hipGetLastError();
+}
 
 // Stage s of the chain: 0 k_zfwd, 1 k_y<-1>, 2 k_xc2r, 3 k_xr2c, 4 k_y<+1>, 5 k_zbwd.
 template <typename T>

@@ -1,0 +1,38 @@
+// Host dispatch from an axis length to its compiled two-factor kernels.
+#include "oa_fft2_sizes.h"
+#include "oa_kernels.h"
+
+namespace oa2 {
+#define OA_DECL(n, r1, r2) \
+  hipError_t launch_n##n(int kind, bool fp32, const oa::FftCtx& c, hipStream_t st);
+OA_FFT2_SIZES(OA_DECL)
+#undef OA_DECL
+}  // namespace oa2
+
+namespace oa {
+
+hipError_t fft2_launch(int kind, int n, bool fp32, const FftCtx& c, hipStream_t st) {
+  switch (n) {
+#define OA_CASE(nn, r1, r2) \
+  case nn:                  \
+    return oa2::launch_n##nn(kind, fp32, c, st);
+    OA_FFT2_SIZES(OA_CASE)
+#undef OA_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+int fft2_ntr(int n) {
+  switch (n) {
+#define OA_CASE(nn, r1, r2) \
+  case nn:                  \
+    return 256 / ((r1) > (r2) ? (r1) : (r2));
+    OA_FFT2_SIZES(OA_CASE)
+#undef OA_CASE
+    default:
+      return 0;
+  }
+}
+
+}  // namespace oa

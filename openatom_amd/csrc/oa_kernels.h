@@ -29,7 +29,31 @@ struct FftCtx {
   double* rho;            // [N]
   double* eke_part;       // [nst][nblk]
   double inv_vol, ecut_ry, scale;
+  // fused plane kernels
+  int kcp;                // kx columns per y-FFT batch
+  int pcp;                // pairs of rows per x-FFT batch
+  int cs;                 // states per workgroup
+  int nchunk;             // state chunks (= rho partials)
+  int plane_cfg;          // threads / register-slot variant
+  double* rho_part;       // [nchunk][N]
+  // two-factor engine
+  const int4* lines;      // [nlines] (first point, length, first kz, kx*ny + ky mod ny)
 };
+
+// two-factor register FFT engine (oa_fft2_*.hip): kind 0 z fwd, 1 y fwd,
+// 2 x c2r + rho partials, 3 V psi + x r2c, 4 y bwd, 5 z bwd + kinetic/EKE;
+// n is the length of the kind's axis
+hipError_t fft2_launch(int kind, int n, bool fp32, const FftCtx& c, hipStream_t st);
+int fft2_ntr(int n);  // transforms per workgroup for length n, 0 if not compiled
+
+// fused y+x plane passes (one z-plane, a chunk of states per workgroup)
+hipError_t plane_fwd(const FftCtx& c, bool fp32, hipStream_t st);
+hipError_t plane_bwd(const FftCtx& c, bool fp32, hipStream_t st);
+// rho = sum of partials / Omega, and partial sums of V rho into dpart[ndpart]
+hipError_t rho_reduce(const FftCtx& c, double* dpart, int ndpart, hipStream_t st);
+size_t plane_smem(const FftCtx& c, bool fp32);
+int plane_config_for(int plane_points);  // -1 if the plane is too large
+int plane_threads(int cfg);
 
 // stage: 0 z-gather+FFT, 1 y-FFT fwd, 2 x c2r + rho, 3 V psi + x r2c,
 //        4 y-FFT bwd, 5 z-FFT + pack + kinetic force / EKE partials

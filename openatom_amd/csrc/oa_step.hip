@@ -163,7 +163,16 @@ struct oa_ctx {
   std::vector<hipEvent_t> ev;  // [kRing][OA_NTIMINGS] once timing is enabled
   int ev_cur = 0, ev_count = 0;
   bool have_S = false, have_T = false, have_v = false;
+  int engine = 0;     // kEngFft2, kEngSep or kEngFused
+  int nzblk = 0;      // EKE partials per state of the active z kernel
+  int4* lines = nullptr;
 };
+
+namespace {
+constexpr int kEngFft2 = 0;   // two-factor register engine, one kernel per pass
+constexpr int kEngSep = 1;    // Stockham LDS kernels, one kernel per pass
+constexpr int kEngFused = 2;  // Stockham LDS, y and x passes fused per plane
+}  // namespace
 
 namespace {
 
@@ -287,12 +296,54 @@ oa_status build_layout(oa_ctx* c) {
   }
   f.npair = std::max(1, std::min(OA_XSLOTS * OA_NTHR, 512) / c->nx);
   if (f.npair * c->nx > OA_XSLOTS * OA_NTHR) return fail(c, OA_EINVAL, "nx too large");
+  // FFT engine: the two-factor register engine when all three lengths are
+  // compiled for it (mode 0), else one Stockham kernel per pass (mode 1);
+  // mode 2 selects the fused Stockham plane kernels.
+  const int npairs_all = (c->ny + 1) / 2;
+  if (c->opt.fft_mode == 0 && oa::fft2_ntr(c->nx) > 0 && oa::fft2_ntr(c->ny) > 0 &&
+      oa::fft2_ntr(c->nz) > 0)
+    c->engine = kEngFft2;
+  else if (c->opt.fft_mode == 2)
+    c->engine = kEngFused;
+  else
+    c->engine = kEngSep;
+  f.plane_cfg = oa::plane_config_for(c->nx * c->ny);
+  if (c->engine == kEngFused && f.plane_cfg < 0) c->engine = kEngSep;
+  if (c->engine == kEngFft2) {
+    const int qblk = (npairs_all + oa::fft2_ntr(c->nx) - 1) / oa::fft2_ntr(c->nx);
+    const int nchunk = std::max(1, std::min(c->nst, (1024 + qblk * c->nz - 1) / (qblk * c->nz)));
+    f.cs = (c->nst + nchunk - 1) / nchunk;
+    f.nchunk = (c->nst + f.cs - 1) / f.cs;
+    c->nzblk = (c->nlines + oa::fft2_ntr(c->nz) - 1) / oa::fft2_ntr(c->nz);
+  }
+  if (c->engine == kEngFused) {
+    int lds_max = 0;
+    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) !=
+            hipSuccess || lds_max <= 0)
+      lds_max = 65536;
+    lds_max = std::min(lds_max, 160 * 1024);
+    const int npairs = (c->ny + 1) / 2;
+    const int pc0 = std::min(npairs, std::max(1, 2048 / c->nx));
+    bool ok = false;
+    for (int pc = pc0; pc >= 1 && !ok; pc = (pc > 1) ? pc / 2 : 0) {
+      f.pcp = pc;
+      f.kcp = std::min(c->nplane_x, std::max(1, (pc * c->nx) / c->ny));
+      if ((int)oa::plane_smem(f, c->fp32) <= lds_max) ok = true;
+    }
+    if (!ok) c->engine = kEngSep;
+    const int nchunk = std::max(1, std::min(c->nst, (1024 + c->nz - 1) / c->nz));
+    f.cs = (c->nst + nchunk - 1) / nchunk;
+    f.nchunk = (c->nst + f.cs - 1) / f.cs;
+  }
   c->nblk = (c->nlines + f.lpb - 1) / f.lpb;
+  if (c->engine != kEngFft2) c->nzblk = c->nblk;
   std::vector<int> blk_pt0(c->nblk + 1), pt_code(n), line_dst(c->nlines);
+  std::vector<int4> lines(c->nlines);
   for (int b = 0; b < c->nblk; b++) blk_pt0[b] = loff[b * f.lpb];
   blk_pt0[c->nblk] = n;
   for (int l = 0; l < c->nlines; l++) {
     line_dst[l] = lkx[l] * c->ny + imod(lky[l], c->ny);
+    lines[l] = make_int4(loff[l], llen[l], lkz0[l], line_dst[l]);
     const int li = l % f.lpb;
     for (int i = 0; i < llen[l]; i++) {
       const int p = loff[l] + i;
@@ -326,6 +377,8 @@ oa_status build_layout(oa_ctx* c) {
   TRY(upload(c, c->blk_pt0, blk_pt0.data(), sizeof(int) * blk_pt0.size()));
   TRY(upload(c, c->pt_code, pt_code.data(), sizeof(int) * n));
   TRY(upload(c, c->line_dst, line_dst.data(), sizeof(int) * c->nlines));
+  TRY(dalloc(c, &c->lines, sizeof(int4) * c->nlines));
+  TRY(upload(c, c->lines, lines.data(), sizeof(int4) * c->nlines));
   TRY(upload(c, c->kylo, lo.data(), sizeof(int) * c->nplane_x));
   TRY(upload(c, c->kyhi, hi.data(), sizeof(int) * c->nplane_x));
   if (c->nred) {
@@ -353,12 +406,15 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
   TRY(dalloc(c, &c->psi2, gsz));
   TRY(dalloc(c, &c->force, gsz));
   TRY(dalloc(c, &c->zbuf, (size_t)c->nst * c->nz * c->nplane_x * c->ny * csz));
-  TRY(dalloc(c, &c->xbuf, (size_t)c->nst * c->nz * c->ny * c->nplane_x * csz));
+  if (c->engine != kEngSep)
+    TRY(dalloc(c, &c->fc.rho_part, (size_t)c->fc.nchunk * N * sizeof(double)));
+  if (c->engine != kEngFused)
+    TRY(dalloc(c, &c->xbuf, (size_t)c->nst * c->nz * c->ny * c->nplane_x * csz));
   TRY(dalloc(c, &c->psir, (size_t)c->nst * N * rsz));
   TRY(dalloc(c, &c->rho, N * sizeof(double)));
   TRY(dalloc(c, &c->vks, N * sizeof(double)));
   TRY(dalloc(c, &c->occ, c->nst * sizeof(double)));
-  TRY(dalloc(c, &c->eke_part, (size_t)c->nst * c->nblk * sizeof(double)));
+  TRY(dalloc(c, &c->eke_part, (size_t)c->nst * c->nzblk * sizeof(double)));
   const int n = c->nst;
   const size_t nn = (size_t)n * n;
   TRY(dalloc(c, &c->S, nn * sizeof(double)));
@@ -418,6 +474,7 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
   f.psir = c->psir;
   f.rho = c->rho;
   f.eke_part = c->eke_part;
+  f.lines = c->lines;
   f.inv_vol = 1.0 / c->vol;
   f.ecut_ry = c->ecut_ry;
   f.scale = -2.0 / ((double)c->nx * c->ny * c->nz);
@@ -440,22 +497,61 @@ void mark(oa_ctx* c, int i) {
 }
 
 // ---- stages (enqueue only) ----
+// Two-factor engine: z, y, x c2r (rho partials), then the rho reduction with
+// partial sums of V rho.  Fused Stockham: z, plane forward, rho reduction.
+// Separate Stockham: z, y, x c2r + rho.  Marks ev0+0..2.
 oa_status st_g2r(oa_ctx* c, int ev0 = -1) {
-  for (int s = oa::FFT_ZFWD; s <= oa::FFT_XC2R; s++) {
-    HIPCHK(oa::fft_stage(fctx(c), c->fp32, s, c->stream));
-    mark(c, ev0 < 0 ? -1 : ev0 + s);
+  auto mk = [&](int k) { mark(c, ev0 < 0 ? -1 : ev0 + k); };
+  if (c->engine == kEngFft2) {
+    HIPCHK(oa::fft2_launch(oa::FFT_ZFWD, c->nz, c->fp32, fctx(c), c->stream));
+    mk(0);
+    HIPCHK(oa::fft2_launch(oa::FFT_YFWD, c->ny, c->fp32, fctx(c), c->stream));
+    mk(1);
+    HIPCHK(oa::fft2_launch(oa::FFT_XC2R, c->nx, c->fp32, fctx(c), c->stream));
+    HIPCHK(oa::rho_reduce(fctx(c), c->dpart, c->ndpart, c->stream));
+    mk(2);
+  } else if (c->engine == kEngFused) {
+    HIPCHK(oa::fft_stage(fctx(c), c->fp32, oa::FFT_ZFWD, c->stream));
+    mk(0);
+    HIPCHK(oa::plane_fwd(fctx(c), c->fp32, c->stream));
+    mk(1);
+    HIPCHK(oa::rho_reduce(fctx(c), c->dpart, c->ndpart, c->stream));
+    mk(2);
+  } else {
+    for (int s = oa::FFT_ZFWD; s <= oa::FFT_XC2R; s++) {
+      HIPCHK(oa::fft_stage(fctx(c), c->fp32, s, c->stream));
+      mk(s);
+    }
   }
   return OA_OK;
 }
 
+// Two-factor / separate: x r2c, y, z.  Fused: plane backward, z.
+// Marks ev0+3..5, then the EKE and E_loc reductions.
 oa_status st_r2g(oa_ctx* c, int ev0 = -1) {
-  for (int s = oa::FFT_XR2C; s <= oa::FFT_ZBWD; s++) {
-    HIPCHK(oa::fft_stage(fctx(c), c->fp32, s, c->stream));
-    mark(c, ev0 < 0 ? -1 : ev0 + s);
-  }
-  HIPCHK(oa::reduce_to(c->eke_part, c->nst * c->nblk, 1.0, &c->res->eke, c->stream));
+  auto mk = [&](int k) { mark(c, ev0 < 0 ? -1 : ev0 + k); };
   const size_t N = (size_t)c->nx * c->ny * c->nz;
-  HIPCHK(oa::dot_parts(c->vks, c->rho, N, c->dpart, c->ndpart, c->stream));
+  if (c->engine == kEngFft2) {
+    HIPCHK(oa::fft2_launch(oa::FFT_XR2C, c->nx, c->fp32, fctx(c), c->stream));
+    mk(3);
+    HIPCHK(oa::fft2_launch(oa::FFT_YBWD, c->ny, c->fp32, fctx(c), c->stream));
+    mk(4);
+    HIPCHK(oa::fft2_launch(oa::FFT_ZBWD, c->nz, c->fp32, fctx(c), c->stream));
+    mk(5);
+  } else if (c->engine == kEngFused) {
+    HIPCHK(oa::plane_bwd(fctx(c), c->fp32, c->stream));
+    mk(3);
+    mk(4);
+    HIPCHK(oa::fft_stage(fctx(c), c->fp32, oa::FFT_ZBWD, c->stream));
+    mk(5);
+  } else {
+    for (int s = oa::FFT_XR2C; s <= oa::FFT_ZBWD; s++) {
+      HIPCHK(oa::fft_stage(fctx(c), c->fp32, s, c->stream));
+      mk(s);
+    }
+    HIPCHK(oa::dot_parts(c->vks, c->rho, N, c->dpart, c->ndpart, c->stream));
+  }
+  HIPCHK(oa::reduce_to(c->eke_part, c->nst * c->nzblk, 1.0, &c->res->eke, c->stream));
   HIPCHK(oa::reduce_to(c->dpart, c->ndpart, c->vol / (double)N, &c->res->eloc, c->stream));
   return OA_OK;
 }
@@ -583,6 +679,7 @@ void oa_default_options(oa_options* o) {
   o->ecut_mass = 2.0;
   o->use_lambda = 1;
   o->want_resid = 1;
+  o->fft_mode = 0;
 }
 
 int oa_kvecs(double ecut_ry, const double hmat[9], int* kx, int* ky, int* kz) {
@@ -677,6 +774,21 @@ oa_status oa_layout_info(const oa_ctx* c, int info[4]) {
   info[1] = c->nlines;
   info[2] = c->nplane_x;
   info[3] = c->nkx0;
+  return OA_OK;
+}
+
+oa_status oa_get_config(const oa_ctx* c, int cfg[8]) {
+  if (!c || !cfg) return OA_EINVAL;
+  const FftCtx& f = c->fc;
+  const bool fused = c->engine == kEngFused, fft2 = c->engine == kEngFft2;
+  cfg[0] = c->engine;
+  cfg[1] = fft2 ? oa::fft2_ntr(c->nz) : f.lpb;
+  cfg[2] = fft2 ? oa::fft2_ntr(c->ny) : f.kc;
+  cfg[3] = fft2 ? oa::fft2_ntr(c->nx) : f.npair;
+  cfg[4] = fused ? f.kcp : 0;
+  cfg[5] = fused ? f.pcp : 0;
+  cfg[6] = (fused || fft2) ? f.cs : 0;
+  cfg[7] = (fused || fft2) ? f.nchunk : 0;
   return OA_OK;
 }
 

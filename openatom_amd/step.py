@@ -26,6 +26,8 @@ from ._lib import OaError, ptr
 
 
 class CPContext:
+    ENGINES = {0: "two-factor register FFT", 1: "Stockham per pass", 2: "Stockham fused planes"}
+
     def __init__(self, grid, hmat, ecut_ry, nstates, kx=None, ky=None, kz=None,
                  occ=None, precision="fp64", device=0, **opts):
         L = _lib.load()
@@ -64,6 +66,11 @@ class CPContext:
         info = np.zeros(4, np.int32)
         self._chk(L.oa_layout_info(self.h, ptr(info)))
         self.npts, self.nlines, self.nplane_x, self.nkx0 = (int(v) for v in info)
+        cfg = np.zeros(8, np.int32)
+        self._chk(L.oa_get_config(self.h, ptr(cfg)))
+        self.config = dict(zip(("engine", "zlines", "ycols", "xpairs", "kcp", "pcp", "cs",
+                                "nchunk"), (int(v) for v in cfg)))
+        self.config["engine_name"] = self.ENGINES.get(self.config["engine"], "?")
 
     # ------------------------------------------------------------------ plumbing
     def _chk(self, st):
@@ -194,9 +201,16 @@ class CPContext:
 
     TIMING_NAMES = ("k_zfwd", "k_y_fwd", "k_xc2r", "k_xr2c", "k_y_bwd", "k_zbwd",
                     "reduce_lambda", "sd_sync", "overlap", "ns", "rotate", "resid", "step")
+    TIMING_NAMES_FUSED = ("k_zfwd", "k_plane_fwd", "k_rho_reduce", "k_plane_bwd", "none",
+                          "k_zbwd", "reduce_lambda", "sd_sync", "overlap", "ns", "rotate",
+                          "resid", "step")
 
     def timings(self):
-        """Per-stage device ms of the last step (HIP events on the context's stream)."""
-        ms = np.zeros(len(self.TIMING_NAMES))
+        """Per-stage device ms averaged over the timed steps (HIP events on the
+        context's stream); names follow the FFT mode in use."""
+        names = self.TIMING_NAMES_FUSED if self.config["engine"] == 2 else self.TIMING_NAMES
+        ms = np.zeros(len(names))
         self._chk(self._L.oa_timings(self.h, ptr(ms), len(ms)))
-        return dict(zip(self.TIMING_NAMES, ms.tolist()))
+        out = dict(zip(names, ms.tolist()))
+        out.pop("none", None)
+        return out

@@ -46,25 +46,31 @@ def report(tag, **errs):
     print(tag, " ".join(f"{k}={v:.3e}" for k, v in errs.items()), flush=True)
 
 
+# FFT engines: 0 two-factor register engine, 1 Stockham per pass, 2 Stockham fused planes
+MODES = [0, 1, 2]
+
+
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("name", ["small", "ragged37", "water8"])
-def test_g2r_density(gpu, name):
+def test_g2r_density(gpu, name, mode):
     s = get_system(name)
     o = make_oracle(s)
-    with make_ctx(s) as c:
+    with make_ctx(s, fft_mode=mode) as c:
         c.set_psi(s["psi"])
         c.fft_g2r()
         pr = o.g2r_all(s["psi"])
         e_psir = max(rel(c.get_psir(st), pr[st]) for st in range(s["nstates"]))
         e_rho = rel(c.get_density(), o.density(pr))
-    report(f"[{name}] g2r", psir=e_psir, rho=e_rho)
+    report(f"[{name} mode{mode}] g2r", psir=e_psir, rho=e_rho)
     assert e_psir < TOL and e_rho < TOL
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("name", ["small", "ragged37", "water8"])
-def test_r2g_force_eke(gpu, name):
+def test_r2g_force_eke(gpu, name, mode):
     s = get_system(name)
     o = make_oracle(s)
-    with make_ctx(s) as c:
+    with make_ctx(s, fft_mode=mode) as c:
         c.set_psi(s["psi"])
         c.set_vks(s["vks"])
         c.fft_g2r()
@@ -73,7 +79,7 @@ def test_r2g_force_eke(gpu, name):
     pr = o.g2r_all(s["psi"])
     Fo = o.vpsi_r2g_all(pr, s["vks"])
     eko = o.eke_force(s["psi"], Fo)
-    report(f"[{name}] r2g", force=rel(F, Fo), eke=abs(eke - eko) / abs(eko))
+    report(f"[{name} mode{mode}] r2g", force=rel(F, Fo), eke=abs(eke - eko) / abs(eko))
     assert rel(F, Fo) < TOL
     assert abs(eke - eko) < TOL * abs(eko)
 
@@ -119,12 +125,13 @@ def test_lambda_sd_overlap_ns_rotate(gpu, name):
     assert np.abs(Sout / 2 - np.eye(s["nstates"])).max() < 1e-12
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("name", ["small", "water8"])
-def test_full_steps_match_oracle(gpu, name):
+def test_full_steps_match_oracle(gpu, name, mode):
     s = get_system(name)
     o = make_oracle(s)
     psio = s["psi"].copy()
-    with make_ctx(s, dt=5.0, ns_tol=1e-15, ns_max_iter=30) as c:
+    with make_ctx(s, dt=5.0, ns_tol=1e-15, ns_max_iter=30, fft_mode=mode) as c:
         c.set_psi(s["psi"])
         c.set_vks(s["vks"])
         for k in range(3):
@@ -132,7 +139,7 @@ def test_full_steps_match_oracle(gpu, name):
             ro = o.step(s["vks"], psio, dt=5.0, ns_tol=1e-15, ns_max_iter=30, use_lambda=True)
             psi = c.get_psi()
             de = abs(r["eelec"] - ro["eelec"]) / abs(ro["eelec"])
-            report(f"[{name}] step{k}", dE=de, eke=abs(r["eke"] - ro["eke"]) / abs(ro["eke"]),
+            report(f"[{name} mode{mode}] step{k}", dE=de, eke=abs(r["eke"] - ro["eke"]) / abs(ro["eke"]),
                    psi=rel(psi, psio), resid=r["resid_max"])
             assert de < TOL
             assert rel(psi, psio) < TOL
