@@ -68,6 +68,8 @@ typedef struct {
   int want_resid;      /* compute ||Psi^H Psi/2 - I|| after each step */
   int fft_mode;        /* 0: two-factor register engine where the lengths allow (default),
                           1: Stockham LDS kernels, one per pass, 2: Stockham fused planes */
+  int ns_mode;         /* 0: one cooperative launch when all tiles fit (default),
+                          1: two launches per iteration with a device-side early exit */
 } oa_options;
 
 typedef struct {

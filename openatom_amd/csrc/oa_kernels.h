@@ -112,10 +112,12 @@ struct NsArgs {
   StepDev* res;
   int n, max_iter;
   double tol;
+  int coop;  // one cooperative launch (all tiles resident) or two launches per iteration
 };
 hipError_t ns_run(const NsArgs& a, hipStream_t st);
+bool ns_coop_possible(int n, int device);
 
-// ---- pointwise G-space kernels (oa_gemm.hip) ----
+// ---- pointwise G-space kernels (oa_pointwise.hip) ----
 // psi += dtm[p] * F (cp_min_std.C:56-86); accumulates |F|^2 partials
 hipError_t sd_update(double2* psi, const double2* f, const double* dtm, int nst, int npts,
                      int ld, double* fpart, int nfpart, hipStream_t st);

@@ -1,0 +1,143 @@
+// oa_pointwise.hip -- G-space pointwise kernels and small reductions of the
+// CP electronic step.
+//   k_sd       : psi += dt/m F  (CP_integrate_min_STD, cp_min_std.C:56-86)
+//   k_red_sync : redundant kx=0 points <- conj(partner)
+//                (CP_State_GSpacePlane.C:2862-3061)
+//   k_reduce_to, k_dot, k_resid, k_fill_zero: deterministic reductions and
+//   graph-safe zero fill.
+#include "oa_common.h"
+#include "oa_kernels.h"
+
+namespace oa {
+
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double red[16];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += red[w];
+  return t;
+}
+
+__global__ void k_sd(double2* __restrict__ psi, const double2* __restrict__ f,
+                     const double* __restrict__ dtm, int nst, int npts, int ld,
+                     double* __restrict__ fpart) {
+  const size_t tot = (size_t)nst * npts;
+  double fn = 0.0;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int s = (int)(e / npts), p = (int)(e - (size_t)s * npts);
+    const size_t o = (size_t)s * ld + p;
+    const double2 fv = f[o];
+    const double m = dtm[p];
+    double2 c = psi[o];
+    c.x += fv.x * m;
+    c.y += fv.y * m;
+    psi[o] = c;
+    fn += fv.x * fv.x + fv.y * fv.y;
+  }
+  const double t = block_sum(fn);
+  if (threadIdx.x == 0) fpart[blockIdx.x] = t;
+}
+
+hipError_t sd_update(double2* psi, const double2* f, const double* dtm, int nst, int npts, int ld,
+                     double* fpart, int nfpart, hipStream_t st) {
+  hipLaunchKernelGGL(k_sd, dim3(nfpart), dim3(256), 0, st, psi, f, dtm, nst, npts, ld, fpart);
+  return hipGetLastError();
+}
+
+__global__ void k_red_sync(double2* __restrict__ psi, const int* __restrict__ red,
+                           const int* __restrict__ partner, int nred, int nst, int ld) {
+  const size_t tot = (size_t)nst * nred;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int s = (int)(e / nred), r = (int)(e - (size_t)s * nred);
+    const double2 v = psi[(size_t)s * ld + partner[r]];
+    psi[(size_t)s * ld + red[r]] = make_double2(v.x, -v.y);
+  }
+}
+
+hipError_t red_sync(double2* psi, const int* red, const int* partner, int nred, int nst, int ld,
+                    hipStream_t st) {
+  if (nred == 0) return hipSuccess;
+  const size_t tot = (size_t)nst * nred;
+  int blocks = (int)((tot + 255) / 256);
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(k_red_sync, dim3(blocks), dim3(256), 0, st, psi, red, partner, nred, nst,
+                     ld);
+  return hipGetLastError();
+}
+
+__global__ void k_reduce_to(const double* __restrict__ part, int n, double scale,
+                            double* __restrict__ out) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) v += part[i];
+  const double t = block_sum(v);
+  if (threadIdx.x == 0) *out = t * scale;
+}
+
+hipError_t reduce_to(const double* part, int n, double scale, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce_to, dim3(1), dim3(1024), 0, st, part, n, scale, out);
+  return hipGetLastError();
+}
+
+__global__ void k_dot(const double* __restrict__ a, const double* __restrict__ b, size_t n,
+                      double* __restrict__ part) {
+  double v = 0.0;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x)
+    v += a[e] * b[e];
+  const double t = block_sum(v);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+hipError_t dot_parts(const double* a, const double* b, size_t n, double* part, int nparts,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(k_dot, dim3(nparts), dim3(256), 0, st, a, b, n, part);
+  return hipGetLastError();
+}
+
+__global__ void k_resid(const double* __restrict__ S, int n, StepDev* res) {
+  __shared__ double rmax[16];
+  double mx = 0.0, ss = 0.0;
+  const size_t nn = (size_t)n * n;
+  for (size_t e = threadIdx.x; e < nn; e += blockDim.x) {
+    const size_t i = e / n, j = e - i * n;
+    const double d = 0.5 * S[e] - (i == j ? 1.0 : 0.0);
+    mx = fmax(mx, fabs(d));
+    ss += d * d;
+  }
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_down(mx, o, 64));
+  if ((threadIdx.x & 63) == 0) rmax[threadIdx.x >> 6] = mx;
+  const double tot = block_sum(ss);
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) m = fmax(m, rmax[w]);
+    res->resid_max = m;
+    res->resid_rms = sqrt(tot / (double)nn);
+  }
+}
+
+hipError_t resid_stats(const double* S, int n, StepDev* res, hipStream_t st) {
+  hipLaunchKernelGGL(k_resid, dim3(1), dim3(1024), 0, st, S, n, res);
+  return hipGetLastError();
+}
+
+__global__ void k_fill_zero(uint32_t* __restrict__ p, size_t n) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x)
+    p[e] = 0u;
+}
+
+hipError_t fill_zero(void* p, size_t bytes, hipStream_t st) {
+  const size_t n = bytes / 4;
+  if (n == 0) return hipSuccess;
+  int blocks = (int)((n + 255) / 256);
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_fill_zero, dim3(blocks), dim3(256), 0, st, (uint32_t*)p, n);
+  return hipGetLastError();
+}
+
+}  // namespace oa

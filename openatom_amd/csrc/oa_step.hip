@@ -163,6 +163,7 @@ struct oa_ctx {
   std::vector<hipEvent_t> ev;  // [kRing][OA_NTIMINGS] once timing is enabled
   int ev_cur = 0, ev_count = 0;
   bool have_S = false, have_T = false, have_v = false;
+  bool ns_coop = false;  // Newton-Schulz as one cooperative launch
   int engine = 0;     // kEngFft2, kEngSep or kEngFused
   int nzblk = 0;      // EKE partials per state of the active z kernel
   int4* lines = nullptr;
@@ -421,17 +422,16 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
   TRY(dalloc(c, &c->S2, nn * sizeof(double)));
   TRY(dalloc(c, &c->T, nn * sizeof(double)));
   TRY(dalloc(c, &c->lam, nn * sizeof(double)));
-  // split-K shape for the gram: aim for >= 512 workgroups
+  // split-K shape for the gram (32x32 tiles): about 1024 workgroups for the
+  // symmetric product, K slices a multiple of 64 (16 per wave)
   const int K = 2 * c->npts;
-  const int ntile = (n + 63) / 64;
-  const int tiles = ntile * ntile;
-  int nsplit = std::max(1, 512 / std::max(1, ntile * (ntile + 1) / 2));
+  const int ntile = (n + 31) / 32;
+  const int nsplit = std::max(1, 1024 / (ntile * (ntile + 1) / 2));
   int ks = (K + nsplit - 1) / nsplit;
-  ks = std::max(64, ((ks + 15) / 16) * 16);
+  ks = std::max(64, ((ks + 63) / 64) * 64);
   c->kslice = ks;
   c->gsplit_max = oa::gram_nsplit(K, ks);
   TRY(dalloc(c, &c->gpart, (size_t)c->gsplit_max * nn * sizeof(double)));
-  (void)tiles;
   TRY(dalloc(c, &c->nsX0, nn * sizeof(double)));
   TRY(dalloc(c, &c->nsX1, nn * sizeof(double)));
   TRY(dalloc(c, &c->nsT0, nn * sizeof(double)));
@@ -439,6 +439,7 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
   TRY(dalloc(c, &c->nsC, nn * sizeof(double)));
   const int nt = (n + 31) / 32;
   TRY(dalloc(c, &c->errpart, (size_t)(c->opt.ns_max_iter + 2) * nt * nt * sizeof(double)));
+  c->ns_coop = (c->opt.ns_mode == 0) && oa::ns_coop_possible(n, c->device);
   c->nfpart = 1024;
   c->ndpart = 512;
   TRY(dalloc(c, &c->fpart, c->nfpart * sizeof(double)));
@@ -632,6 +633,7 @@ oa_status st_ns(oa_ctx* c, double tol, int max_iter) {
   a.n = c->nst;
   a.max_iter = max_iter;
   a.tol = tol;
+  a.coop = c->ns_coop ? 1 : 0;
   HIPCHK(oa::ns_run(a, c->stream));
   c->have_T = true;
   return OA_OK;
@@ -680,6 +682,7 @@ void oa_default_options(oa_options* o) {
   o->use_lambda = 1;
   o->want_resid = 1;
   o->fft_mode = 0;
+  o->ns_mode = 0;
 }
 
 int oa_kvecs(double ecut_ry, const double hmat[9], int* kx, int* ky, int* kz) {

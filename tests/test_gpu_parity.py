@@ -84,12 +84,13 @@ def test_r2g_force_eke(gpu, name, mode):
     assert abs(eke - eko) < TOL * abs(eko)
 
 
+@pytest.mark.parametrize("ns_mode", [0, 1])  # cooperative single launch, launches per iteration
 @pytest.mark.parametrize("name", ["small", "ragged37", "water8"])
-def test_lambda_sd_overlap_ns_rotate(gpu, name):
+def test_lambda_sd_overlap_ns_rotate(gpu, name, ns_mode):
     s = get_system(name)
     o = make_oracle(s)
     dt = 5.0
-    with make_ctx(s, dt=dt) as c:
+    with make_ctx(s, dt=dt, ns_mode=ns_mode) as c:
         c.set_psi(s["psi"])
         c.set_vks(s["vks"])
         c.fft_g2r()
@@ -116,7 +117,7 @@ def test_lambda_sd_overlap_ns_rotate(gpu, name):
     outo = o.rotate(psio, To)
     errs = dict(lam=rel(lam, lamo), force=rel(F, Fo), fnorm=abs(fn - np.linalg.norm(Fo)) / np.linalg.norm(Fo),
                 psi_sd=rel(psi_sd, psio), S=rel(S, So), T=rel(T, To), psi_out=rel(psi_out, outo))
-    report(f"[{name}] chain it={it}/{abs(ito)} err={err:.2e}", **errs)
+    report(f"[{name} ns{ns_mode}] chain it={it}/{abs(ito)} err={err:.2e}", **errs)
     assert ok and it >= 1
     for k, v in errs.items():
         assert v < TOL, (k, v)
