@@ -57,9 +57,12 @@ def stage_bytes(stage, geo, csz, rsz):
 
 def run_precision(sysd, precision, steps, warmup, device=0):
     from openatom_amd.step import CPContext
+    # The reference step has no ||Psi^H Psi - I|| evaluation (Ortho checks the
+    # T iterates); it is measured once after the timed steps instead.
     c = CPContext(sysd["grid"], sysd["hmat"], sysd["ecut_ry"], sysd["nstates"], sysd["kx"],
                   sysd["ky"], sysd["kz"], precision=precision, device=device, dt=5.0,
-                  ns_tol=1e-15 if precision == "fp64" else 1e-14, ns_max_iter=30)
+                  ns_tol=1e-15 if precision == "fp64" else 1e-14, ns_max_iter=30,
+                  want_resid=0)
     c.set_psi(sysd["psi"])
     c.set_vks(sysd["vks"])
     first = c.step()
@@ -74,6 +77,7 @@ def run_precision(sysd, precision, steps, warmup, device=0):
     t1 = time.perf_counter()
     last = c.result()
     tim = c.timings()
+    last["resid_max"], last["resid_rms"] = c.residual()
     geo = dict(nx=c.nx, ny=c.ny, nz=c.nz, nlines=c.nlines, nplane_x=c.nplane_x, npts=c.npts)
     psi = c.get_psi()
     cfg = dict(c.config)
@@ -132,8 +136,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="water32")
     ap.add_argument("--seed", type=int, default=1234)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib-mb", type=int, default=0,
+                    help="before the runs, copy a buffer of this many MiB three times "
+                         "(a plain copy kernel that scales the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the step on a single "
@@ -144,6 +151,8 @@ def main():
     import oracle
     from openatom_amd import system
 
+    if args.copy_calib_mb > 0:
+        copy_calibration(args.copy_calib_mb)
     log(f"[bench] building workload {args.workload} (seed {args.seed})")
     sysd = system.build(args.workload, seed=args.seed, kvec_source=oracle.stored_kvecs)
     nst = sysd["nstates"]
@@ -212,6 +221,28 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def copy_calibration(mib):
+    """Three device-to-device copies of `mib` MiB through the HIP runtime (its
+    __amd_rocclr_copyBuffer kernel): the known-size dispatches that scale the
+    FETCH_SIZE / WRITE_SIZE counters of a --pmc run."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    nb = ctypes.c_size_t(mib << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), nb) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(a, 1, nb) != 0:
+        raise RuntimeError("hipMemset failed")
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, nb, 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"[bench] copy calibration: 3 x {mib} MiB device-to-device")
 
 
 if __name__ == "__main__":

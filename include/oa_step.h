@@ -66,8 +66,12 @@ typedef struct {
   double ecut_mass;    /* cp_mass_cut_def (default 2) */
   int use_lambda;      /* project F with Lambda (cp_min_opt = 1) */
   int want_resid;      /* compute ||Psi^H Psi/2 - I|| after each step */
-  int fft_mode;        /* 0: two-factor register engine where the lengths allow (default),
-                          1: Stockham LDS kernels, one per pass, 2: Stockham fused planes */
+  int fft_mode;        /* 0 (default): the fastest engine compiled for the grid (two-factor
+                          fused planes, else two-factor per pass, else Stockham per pass);
+                          1: Stockham LDS kernels, one per pass; 2: Stockham fused planes;
+                          3: two-factor register engine, one kernel per pass;
+                          4: two-factor fused y+x planes, psi(r) recomputed on chip
+                          (square x-y planes only; OA_EINVAL where not compiled) */
   int ns_mode;         /* 0: one cooperative launch when all tiles fit (default),
                           1: two launches per iteration with a device-side early exit */
 } oa_options;
@@ -103,11 +107,13 @@ const char *oa_last_error(const oa_ctx *ctx);
 
 /* sizes of the context's layout: npts, nlines, nplane_x, nkx0 */
 oa_status oa_layout_info(const oa_ctx *ctx, int info[4]);
-/* launch shapes in use: [0] FFT engine (0 two-factor register engine, 1
- * Stockham one kernel per pass, 2 Stockham fused y+x planes), [1] z-lines,
- * [2] y columns, [3] x pair-rows per workgroup, [4] y columns per batch
- * (fused), [5] x pair-rows per batch (fused), [6] states per workgroup in
- * the x passes, [7] state chunks (rho partials) */
+/* launch shapes in use: [0] FFT engine (0 two-factor per pass, 1 Stockham
+ * per pass, 2 Stockham fused y+x planes, 3 two-factor fused y+x planes),
+ * [1] z-lines, [2] y columns, [3] x pair-rows per workgroup (or per round),
+ * [4] y columns per batch (Stockham fused) or states per R->G chunk
+ * (two-factor fused), [5] x pair-rows per batch (Stockham fused) or R->G
+ * chunks (two-factor fused), [6] states per workgroup in the G->R x pass,
+ * [7] state chunks (rho partials) */
 oa_status oa_get_config(const oa_ctx *ctx, int cfg[8]);
 /* the stored k-vectors in use */
 oa_status oa_get_kvecs(const oa_ctx *ctx, int *kx, int *ky, int *kz);
@@ -172,6 +178,11 @@ oa_status oa_rotate(oa_ctx *ctx);
  * redundancy sync, S, Newton-Schulz, Psi T.  out may be NULL (no host
  * synchronisation); oa_step_result() fetches it later. */
 oa_status oa_step(oa_ctx *ctx, oa_step_out *out);
+/* ||Psi^H Psi / 2 - I|| of the current coefficients (max-abs and rms over
+ * the n x n entries): the orthonormality that Ortho's tolerance_check
+ * (ortho.C:109-146) controls through T; computed on demand, not per step
+ * unless opt.want_resid is set.  Either pointer may be NULL. */
+oa_status oa_residual(oa_ctx *ctx, double *resid_max, double *resid_rms);
 oa_status oa_sync(oa_ctx *ctx);
 oa_status oa_step_result(oa_ctx *ctx, oa_step_out *out);
 

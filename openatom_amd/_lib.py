@@ -22,7 +22,7 @@ EXPORTS = (
     "oa_layout_info", "oa_get_config", "oa_get_kvecs", "oa_psi_set", "oa_psi_get", "oa_vks_set",
     "oa_fft_g2r", "oa_psir_get", "oa_density_get", "oa_vpsi_r2g", "oa_force_get",
     "oa_force_set", "oa_lambda", "oa_sd_update", "oa_overlap", "oa_ortho_ns",
-    "oa_tmat_set", "oa_rotate", "oa_step", "oa_sync", "oa_step_result", "oa_timings",
+    "oa_tmat_set", "oa_rotate", "oa_step", "oa_residual", "oa_sync", "oa_step_result", "oa_timings",
     "oa_set_timing",
 )
 
@@ -91,7 +91,7 @@ def load():
                        ("oa_overlap", [vp, vp]),
                        ("oa_ortho_ns", [vp, cd, ci, vp, vp, vp]),
                        ("oa_tmat_set", [vp, vp]), ("oa_rotate", [vp]),
-                       ("oa_step", [vp, vp]), ("oa_sync", [vp]),
+                       ("oa_step", [vp, vp]), ("oa_residual", [vp, vp, vp]), ("oa_sync", [vp]),
                        ("oa_step_result", [vp, vp]), ("oa_timings", [vp, vp, ci]),
                        ("oa_set_timing", [vp, ci])):
         f = getattr(L, name)

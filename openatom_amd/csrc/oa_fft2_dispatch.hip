@@ -9,6 +9,14 @@ OA_FFT2_SIZES(OA_DECL)
 #undef OA_DECL
 }  // namespace oa2
 
+namespace oa3 {
+#define OA_DECL(n, r1, r2)                                                                  \
+  hipError_t plane3_n##n(int kind, bool fp32, const oa::FftCtx& c, hipStream_t st);        \
+  size_t plane3_smem_n##n(int npx, bool fp32);
+OA_FFT2_SIZES(OA_DECL)
+#undef OA_DECL
+}  // namespace oa3
+
 namespace oa {
 
 hipError_t fft2_launch(int kind, int n, bool fp32, const FftCtx& c, hipStream_t st) {
@@ -20,6 +28,30 @@ hipError_t fft2_launch(int kind, int n, bool fp32, const FftCtx& c, hipStream_t 
 #undef OA_CASE
     default:
       return hipErrorInvalidValue;
+  }
+}
+
+hipError_t plane3_launch(int kind, int n, bool fp32, const FftCtx& c, hipStream_t st) {
+  switch (n) {
+#define OA_CASE(nn, r1, r2) \
+  case nn:                  \
+    return oa3::plane3_n##nn(kind, fp32, c, st);
+    OA_FFT2_SIZES(OA_CASE)
+#undef OA_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+size_t plane3_smem(int n, int npx, bool fp32) {
+  switch (n) {
+#define OA_CASE(nn, r1, r2) \
+  case nn:                  \
+    return oa3::plane3_smem_n##nn(npx, fp32);
+    OA_FFT2_SIZES(OA_CASE)
+#undef OA_CASE
+    default:
+      return 0;
   }
 }
 

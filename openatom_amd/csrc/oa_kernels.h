@@ -38,6 +38,11 @@ struct FftCtx {
   double* rho_part;       // [nchunk][N]
   // two-factor engine
   const int4* lines;      // [nlines] (first point, length, first kz, kx*ny + ky mod ny)
+  // fused two-factor plane passes
+  void* zout;             // [nst][nz][nplane_x][ny] complex T: R->G plane output
+  int nchunk_r;           // state chunks of the R->G plane pass ...
+  int cs_r;               // ... and states per chunk
+  int psir_state;         // state whose psi(r) the G->R plane pass writes to psir
 };
 
 // two-factor register FFT engine (oa_fft2_*.hip): kind 0 z fwd, 1 y fwd,
@@ -45,6 +50,12 @@ struct FftCtx {
 // n is the length of the kind's axis
 hipError_t fft2_launch(int kind, int n, bool fp32, const FftCtx& c, hipStream_t st);
 int fft2_ntr(int n);  // transforms per workgroup for length n, 0 if not compiled
+
+// fused y+x plane passes on the two-factor engine (oa_plane3.h), square
+// planes nx == ny == n: kind 0 G->R + rho partials, 1 R->G (V psi, x and
+// y back to zout), 2 G->R writing psi(r) of c.psir_state
+hipError_t plane3_launch(int kind, int n, bool fp32, const FftCtx& c, hipStream_t st);
+size_t plane3_smem(int n, int nplane_x, bool fp32);  // 0 if n is not compiled
 
 // fused y+x plane passes (one z-plane, a chunk of states per workgroup)
 hipError_t plane_fwd(const FftCtx& c, bool fp32, hipStream_t st);

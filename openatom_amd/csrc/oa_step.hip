@@ -144,7 +144,7 @@ struct oa_ctx {
       *kyhi = nullptr, *red = nullptr, *partner = nullptr;
   double2 *twx = nullptr, *twy = nullptr, *twz = nullptr;
   double *g2 = nullptr, *occ = nullptr, *dtm = nullptr, *vks = nullptr, *rho = nullptr;
-  void *zbuf = nullptr, *xbuf = nullptr, *psir = nullptr;
+  void *zbuf = nullptr, *xbuf = nullptr, *psir = nullptr, *zout = nullptr;
   double* eke_part = nullptr;
   int nblk = 0;
   double *S = nullptr, *T = nullptr, *lam = nullptr, *gpart = nullptr, *S2 = nullptr;
@@ -164,7 +164,7 @@ struct oa_ctx {
   int ev_cur = 0, ev_count = 0;
   bool have_S = false, have_T = false, have_v = false;
   bool ns_coop = false;  // Newton-Schulz as one cooperative launch
-  int engine = 0;     // kEngFft2, kEngSep or kEngFused
+  int engine = 0;     // kEngFft2, kEngSep, kEngFused or kEngPlane
   int nzblk = 0;      // EKE partials per state of the active z kernel
   int4* lines = nullptr;
 };
@@ -173,6 +173,7 @@ namespace {
 constexpr int kEngFft2 = 0;   // two-factor register engine, one kernel per pass
 constexpr int kEngSep = 1;    // Stockham LDS kernels, one kernel per pass
 constexpr int kEngFused = 2;  // Stockham LDS, y and x passes fused per plane
+constexpr int kEngPlane = 3;  // two-factor engine, y and x fused per plane, psi(r) recomputed
 }  // namespace
 
 namespace {
@@ -297,14 +298,30 @@ oa_status build_layout(oa_ctx* c) {
   }
   f.npair = std::max(1, std::min(OA_XSLOTS * OA_NTHR, 512) / c->nx);
   if (f.npair * c->nx > OA_XSLOTS * OA_NTHR) return fail(c, OA_EINVAL, "nx too large");
-  // FFT engine: the two-factor register engine when all three lengths are
-  // compiled for it (mode 0), else one Stockham kernel per pass (mode 1);
-  // mode 2 selects the fused Stockham plane kernels.
+  // FFT engine.  Mode 0 (auto): the fused two-factor plane passes when the
+  // plane is square and its size compiled, else the two-factor engine with
+  // one kernel per pass when all three lengths are compiled, else the
+  // Stockham kernels.  Modes 1-4 force: 1 Stockham per pass, 2 Stockham fused
+  // planes, 3 two-factor per pass, 4 two-factor fused planes.
   const int npairs_all = (c->ny + 1) / 2;
-  if (c->opt.fft_mode == 0 && oa::fft2_ntr(c->nx) > 0 && oa::fft2_ntr(c->ny) > 0 &&
-      oa::fft2_ntr(c->nz) > 0)
+  int lds_max = 0;
+  if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) !=
+          hipSuccess || lds_max <= 0)
+    lds_max = 65536;
+  lds_max = std::min(lds_max, 160 * 1024);
+  const bool fft2_ok =
+      oa::fft2_ntr(c->nx) > 0 && oa::fft2_ntr(c->ny) > 0 && oa::fft2_ntr(c->nz) > 0;
+  const size_t p3 = (c->nx == c->ny) ? oa::plane3_smem(c->nx, c->nplane_x, c->fp32) : 0;
+  const bool plane_ok = fft2_ok && p3 > 0 && p3 <= (size_t)lds_max;
+  const int mode = c->opt.fft_mode;
+  if (mode < 0 || mode > 4) return fail(c, OA_EINVAL, "fft_mode must be 0..4");
+  if ((mode == 3 && !fft2_ok) || (mode == 4 && !plane_ok))
+    return fail(c, OA_EINVAL, "the requested FFT engine is not compiled for this grid");
+  if (mode == 4 || (mode == 0 && plane_ok))
+    c->engine = kEngPlane;
+  else if (mode == 3 || (mode == 0 && fft2_ok))
     c->engine = kEngFft2;
-  else if (c->opt.fft_mode == 2)
+  else if (mode == 2)
     c->engine = kEngFused;
   else
     c->engine = kEngSep;
@@ -317,12 +334,18 @@ oa_status build_layout(oa_ctx* c) {
     f.nchunk = (c->nst + f.cs - 1) / f.cs;
     c->nzblk = (c->nlines + oa::fft2_ntr(c->nz) - 1) / oa::fft2_ntr(c->nz);
   }
+  if (c->engine == kEngPlane) {
+    // G->R: about 2 workgroups per CU over (z, row parts, chunks); every
+    // chunk writes one rho partial.  R->G: no partials, finer chunks.
+    const int nchunk = std::max(1, std::min(c->nst, (512 + c->nz - 1) / c->nz));
+    f.cs = (c->nst + nchunk - 1) / nchunk;
+    f.nchunk = (c->nst + f.cs - 1) / f.cs;
+    const int nchunk_r = std::max(1, std::min(c->nst, (1024 + c->nz - 1) / c->nz));
+    f.cs_r = (c->nst + nchunk_r - 1) / nchunk_r;
+    f.nchunk_r = (c->nst + f.cs_r - 1) / f.cs_r;
+    c->nzblk = (c->nlines + oa::fft2_ntr(c->nz) - 1) / oa::fft2_ntr(c->nz);
+  }
   if (c->engine == kEngFused) {
-    int lds_max = 0;
-    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) !=
-            hipSuccess || lds_max <= 0)
-      lds_max = 65536;
-    lds_max = std::min(lds_max, 160 * 1024);
     const int npairs = (c->ny + 1) / 2;
     const int pc0 = std::min(npairs, std::max(1, 2048 / c->nx));
     bool ok = false;
@@ -337,7 +360,7 @@ oa_status build_layout(oa_ctx* c) {
     f.nchunk = (c->nst + f.cs - 1) / f.cs;
   }
   c->nblk = (c->nlines + f.lpb - 1) / f.lpb;
-  if (c->engine != kEngFft2) c->nzblk = c->nblk;
+  if (c->engine != kEngFft2 && c->engine != kEngPlane) c->nzblk = c->nblk;
   std::vector<int> blk_pt0(c->nblk + 1), pt_code(n), line_dst(c->nlines);
   std::vector<int4> lines(c->nlines);
   for (int b = 0; b < c->nblk; b++) blk_pt0[b] = loff[b * f.lpb];
@@ -406,12 +429,19 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
   TRY(dalloc(c, &c->psi, gsz));
   TRY(dalloc(c, &c->psi2, gsz));
   TRY(dalloc(c, &c->force, gsz));
-  TRY(dalloc(c, &c->zbuf, (size_t)c->nst * c->nz * c->nplane_x * c->ny * csz));
+  const size_t zsz = (size_t)c->nst * c->nz * c->nplane_x * c->ny * csz;
+  TRY(dalloc(c, &c->zbuf, zsz));
   if (c->engine != kEngSep)
     TRY(dalloc(c, &c->fc.rho_part, (size_t)c->fc.nchunk * N * sizeof(double)));
-  if (c->engine != kEngFused)
-    TRY(dalloc(c, &c->xbuf, (size_t)c->nst * c->nz * c->ny * c->nplane_x * csz));
-  TRY(dalloc(c, &c->psir, (size_t)c->nst * N * rsz));
+  if (c->engine == kEngPlane) {
+    // psi(r) stays on chip; one state's worth for oa_psir_get
+    TRY(dalloc(c, &c->zout, zsz));
+    TRY(dalloc(c, &c->psir, N * rsz));
+  } else {
+    if (c->engine != kEngFused)
+      TRY(dalloc(c, &c->xbuf, (size_t)c->nst * c->nz * c->ny * c->nplane_x * csz));
+    TRY(dalloc(c, &c->psir, (size_t)c->nst * N * rsz));
+  }
   TRY(dalloc(c, &c->rho, N * sizeof(double)));
   TRY(dalloc(c, &c->vks, N * sizeof(double)));
   TRY(dalloc(c, &c->occ, c->nst * sizeof(double)));
@@ -471,6 +501,7 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
   f.occ = c->occ;
   f.vks = c->vks;
   f.zbuf = c->zbuf;
+  f.zout = c->zout;
   f.xbuf = c->xbuf;
   f.psir = c->psir;
   f.rho = c->rho;
@@ -503,7 +534,14 @@ void mark(oa_ctx* c, int i) {
 // Separate Stockham: z, y, x c2r + rho.  Marks ev0+0..2.
 oa_status st_g2r(oa_ctx* c, int ev0 = -1) {
   auto mk = [&](int k) { mark(c, ev0 < 0 ? -1 : ev0 + k); };
-  if (c->engine == kEngFft2) {
+  if (c->engine == kEngPlane) {
+    HIPCHK(oa::fft2_launch(oa::FFT_ZFWD, c->nz, c->fp32, fctx(c), c->stream));
+    mk(0);
+    HIPCHK(oa::plane3_launch(0, c->nx, c->fp32, fctx(c), c->stream));
+    mk(1);
+    HIPCHK(oa::rho_reduce(fctx(c), c->dpart, c->ndpart, c->stream));
+    mk(2);
+  } else if (c->engine == kEngFft2) {
     HIPCHK(oa::fft2_launch(oa::FFT_ZFWD, c->nz, c->fp32, fctx(c), c->stream));
     mk(0);
     HIPCHK(oa::fft2_launch(oa::FFT_YFWD, c->ny, c->fp32, fctx(c), c->stream));
@@ -532,7 +570,15 @@ oa_status st_g2r(oa_ctx* c, int ev0 = -1) {
 oa_status st_r2g(oa_ctx* c, int ev0 = -1) {
   auto mk = [&](int k) { mark(c, ev0 < 0 ? -1 : ev0 + k); };
   const size_t N = (size_t)c->nx * c->ny * c->nz;
-  if (c->engine == kEngFft2) {
+  if (c->engine == kEngPlane) {
+    HIPCHK(oa::plane3_launch(1, c->nx, c->fp32, fctx(c), c->stream));
+    mk(3);
+    mk(4);
+    FftCtx f = fctx(c);
+    f.zbuf = c->zout;  // the z pass back reads the R->G plane output
+    HIPCHK(oa::fft2_launch(oa::FFT_ZBWD, c->nz, c->fp32, f, c->stream));
+    mk(5);
+  } else if (c->engine == kEngFft2) {
     HIPCHK(oa::fft2_launch(oa::FFT_XR2C, c->nx, c->fp32, fctx(c), c->stream));
     mk(3);
     HIPCHK(oa::fft2_launch(oa::FFT_YBWD, c->ny, c->fp32, fctx(c), c->stream));
@@ -783,13 +829,14 @@ oa_status oa_layout_info(const oa_ctx* c, int info[4]) {
 oa_status oa_get_config(const oa_ctx* c, int cfg[8]) {
   if (!c || !cfg) return OA_EINVAL;
   const FftCtx& f = c->fc;
-  const bool fused = c->engine == kEngFused, fft2 = c->engine == kEngFft2;
+  const bool fused = c->engine == kEngFused, plane = c->engine == kEngPlane;
+  const bool fft2 = c->engine == kEngFft2 || plane;
   cfg[0] = c->engine;
   cfg[1] = fft2 ? oa::fft2_ntr(c->nz) : f.lpb;
   cfg[2] = fft2 ? oa::fft2_ntr(c->ny) : f.kc;
   cfg[3] = fft2 ? oa::fft2_ntr(c->nx) : f.npair;
-  cfg[4] = fused ? f.kcp : 0;
-  cfg[5] = fused ? f.pcp : 0;
+  cfg[4] = fused ? f.kcp : plane ? f.cs_r : 0;
+  cfg[5] = fused ? f.pcp : plane ? f.nchunk_r : 0;
   cfg[6] = (fused || fft2) ? f.cs : 0;
   cfg[7] = (fused || fft2) ? f.nchunk : 0;
   return OA_OK;
@@ -851,12 +898,20 @@ oa_status oa_psir_get(oa_ctx* c, int s, double* out) {
   if (!c || !out || s < 0 || s >= c->nst) return OA_EINVAL;
   HIPCHK(hipSetDevice(c->device));
   const size_t N = (size_t)c->nx * c->ny * c->nz;
+  size_t off = (size_t)s * N;
+  if (c->engine == kEngPlane) {
+    // psi(r) is not kept: recompute state s from the z-pass output
+    FftCtx f = fctx(c);
+    f.psir_state = s;
+    HIPCHK(oa::plane3_launch(2, c->nx, c->fp32, f, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    off = 0;
+  }
   if (!c->fp32) {
-    HIPCHK(hipMemcpy(out, (double*)c->psir + (size_t)s * N, N * sizeof(double),
-                     hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(out, (double*)c->psir + off, N * sizeof(double), hipMemcpyDeviceToHost));
   } else {
     std::vector<float> tmp(N);
-    HIPCHK(hipMemcpy(tmp.data(), (float*)c->psir + (size_t)s * N, N * sizeof(float),
+    HIPCHK(hipMemcpy(tmp.data(), (float*)c->psir + off, N * sizeof(float),
                      hipMemcpyDeviceToHost));
     for (size_t i = 0; i < N; i++) out[i] = tmp[i];
   }
@@ -972,6 +1027,17 @@ oa_status oa_step(oa_ctx* c, oa_step_out* out) {
     if (c->res_host->ns_iter < 0)
       return fail(c, OA_ENOCONV, "invsqr_tolerance not met within invsqr_max_iter");
   }
+  return OA_OK;
+}
+
+oa_status oa_residual(oa_ctx* c, double* rmax, double* rrms) {
+  if (!c) return OA_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  TRY(st_gram(c, c->psi, c->psi, true, 1.0, 2.0, c->S2));
+  HIPCHK(oa::resid_stats(c->S2, c->nst, c->res, c->stream));
+  TRY(fetch_result(c, nullptr));
+  if (rmax) *rmax = c->res_host->resid_max;
+  if (rrms) *rrms = c->res_host->resid_rms;
   return OA_OK;
 }
 

@@ -26,7 +26,10 @@ from ._lib import OaError, ptr
 
 
 class CPContext:
-    ENGINES = {0: "two-factor register FFT", 1: "Stockham per pass", 2: "Stockham fused planes"}
+    # engine codes reported by oa_get_config (fft_mode option: 0 auto, 1-4 force
+    # Stockham per pass, Stockham fused, two-factor per pass, two-factor fused)
+    ENGINES = {0: "two-factor register FFT, one kernel per pass", 1: "Stockham per pass",
+               2: "Stockham fused planes", 3: "two-factor fused y+x planes"}
 
     def __init__(self, grid, hmat, ecut_ry, nstates, kx=None, ky=None, kz=None,
                  occ=None, precision="fp64", device=0, **opts):
@@ -193,6 +196,12 @@ class CPContext:
         self._chk(self._L.oa_step_result(self.h, ctypes.byref(out)))
         return out.as_dict()
 
+    def residual(self):
+        """(max, rms) of |Psi^H Psi / 2 - I| for the current coefficients."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._chk(self._L.oa_residual(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def sync(self):
         self._chk(self._L.oa_sync(self.h))
 
@@ -201,6 +210,9 @@ class CPContext:
 
     TIMING_NAMES = ("k_zfwd", "k_y_fwd", "k_xc2r", "k_xr2c", "k_y_bwd", "k_zbwd",
                     "reduce_lambda", "sd_sync", "overlap", "ns", "rotate", "resid", "step")
+    TIMING_NAMES_PLANE = ("k_zfwd", "k_plane_g2r", "k_rho_reduce", "k_plane_r2g", "none",
+                          "k_zbwd", "reduce_lambda", "sd_sync", "overlap", "ns", "rotate",
+                          "resid", "step")
     TIMING_NAMES_FUSED = ("k_zfwd", "k_plane_fwd", "k_rho_reduce", "k_plane_bwd", "none",
                           "k_zbwd", "reduce_lambda", "sd_sync", "overlap", "ns", "rotate",
                           "resid", "step")
@@ -208,7 +220,8 @@ class CPContext:
     def timings(self):
         """Per-stage device ms averaged over the timed steps (HIP events on the
         context's stream); names follow the FFT mode in use."""
-        names = self.TIMING_NAMES_FUSED if self.config["engine"] == 2 else self.TIMING_NAMES
+        names = {2: self.TIMING_NAMES_FUSED, 3: self.TIMING_NAMES_PLANE}.get(
+            self.config["engine"], self.TIMING_NAMES)
         ms = np.zeros(len(names))
         self._chk(self._L.oa_timings(self.h, ptr(ms), len(ms)))
         out = dict(zip(names, ms.tolist()))

@@ -20,6 +20,13 @@ def _systems():
         "small": lambda: small_system(),
         "ragged37": lambda: small_system(nstates=37, ecut=18.0, seed=8),
         "water8": lambda: system.build("water8", seed=11, kvec_source=oracle.stored_kvecs),
+        # square x-y planes for the fused plane engine: odd Ny (27 = 3 x 9),
+        # and 128 = 8 x 16 with 17 kx columns (two y batches of 16) and four
+        # x rounds of pair-rows
+        "square27": lambda: small_system(box=(5.0, 5.2, 5.8), grid=(27, 27, 30), nstates=19,
+                                         seed=5),
+        "dense128": lambda: small_system(box=(6.0, 6.0, 5.0), ecut=80.0, grid=(128, 128, 30),
+                                         nstates=5, seed=9),
     }
 
 
@@ -33,6 +40,8 @@ def get_system(name):
 
 
 def make_ctx(s, precision="fp64", **kw):
+    if kw.get("fft_mode") == 4 and s["grid"][0] != s["grid"][1]:
+        pytest.skip("fused two-factor planes need nx == ny")
     return CPContext(s["grid"], s["hmat"], s["ecut_ry"], s["nstates"], s["kx"], s["ky"],
                      s["kz"], precision=precision, **kw)
 
@@ -46,12 +55,14 @@ def report(tag, **errs):
     print(tag, " ".join(f"{k}={v:.3e}" for k, v in errs.items()), flush=True)
 
 
-# FFT engines: 0 two-factor register engine, 1 Stockham per pass, 2 Stockham fused planes
-MODES = [0, 1, 2]
+# fft_mode: 0 auto, 1 Stockham per pass, 2 Stockham fused planes,
+# 3 two-factor per pass, 4 two-factor fused planes (square planes only)
+MODES = [0, 1, 2, 3, 4]
+FFT_SYSTEMS = ["small", "ragged37", "water8", "square27", "dense128"]
 
 
 @pytest.mark.parametrize("mode", MODES)
-@pytest.mark.parametrize("name", ["small", "ragged37", "water8"])
+@pytest.mark.parametrize("name", FFT_SYSTEMS)
 def test_g2r_density(gpu, name, mode):
     s = get_system(name)
     o = make_oracle(s)
@@ -66,7 +77,7 @@ def test_g2r_density(gpu, name, mode):
 
 
 @pytest.mark.parametrize("mode", MODES)
-@pytest.mark.parametrize("name", ["small", "ragged37", "water8"])
+@pytest.mark.parametrize("name", FFT_SYSTEMS)
 def test_r2g_force_eke(gpu, name, mode):
     s = get_system(name)
     o = make_oracle(s)
@@ -127,7 +138,7 @@ def test_lambda_sd_overlap_ns_rotate(gpu, name, ns_mode):
 
 
 @pytest.mark.parametrize("mode", MODES)
-@pytest.mark.parametrize("name", ["small", "water8"])
+@pytest.mark.parametrize("name", ["small", "water8", "square27"])
 def test_full_steps_match_oracle(gpu, name, mode):
     s = get_system(name)
     o = make_oracle(s)
@@ -148,14 +159,17 @@ def test_full_steps_match_oracle(gpu, name, mode):
             assert r["ns_iters"] >= 1
 
 
-def test_mixed_precision_errors_stated(gpu):
+@pytest.mark.parametrize("mode", [0, 3])
+@pytest.mark.parametrize("name", ["water8", "dense128"])
+def test_mixed_precision_errors_stated(gpu, name, mode):
     """Mixed mode (FP32 FFT chain, FP64 master coefficients and FP64 GEMMs):
-    errors against the FP64 oracle for one step."""
-    s = get_system("water8")
+    errors against the FP64 oracle for one step.  Bars: |dE| <= 1e-6 Ha per
+    atom, rho within 1e-5 relative, ||Psi^H Psi/2 - I|| <= 1e-10."""
+    s = get_system(name)
     o = make_oracle(s)
     psio = s["psi"].copy()
     ro = o.step(s["vks"], psio, dt=5.0, ns_tol=1e-15, ns_max_iter=30, use_lambda=True, want=("rho",))
-    with make_ctx(s, precision="mixed", dt=5.0) as c:
+    with make_ctx(s, precision="mixed", dt=5.0, fft_mode=mode) as c:
         c.set_psi(s["psi"])
         c.set_vks(s["vks"])
         r = c.step()
@@ -163,8 +177,8 @@ def test_mixed_precision_errors_stated(gpu):
         psi = c.get_psi()
     natoms = s["natoms"]
     dE_atom = abs(r["eelec"] - ro["eelec"]) / natoms
-    report("[water8 mixed]", dE_per_atom=dE_atom, rho=rel(rho, ro["rho"]), psi=rel(psi, psio),
-           resid=r["resid_max"])
+    report(f"[{name} mixed mode{mode}]", dE_per_atom=dE_atom, rho=rel(rho, ro["rho"]),
+           psi=rel(psi, psio), resid=r["resid_max"])
     assert dE_atom < 1e-6
     assert rel(rho, ro["rho"]) < 1e-5
     assert r["resid_max"] < 1e-10
