@@ -72,6 +72,10 @@ typedef struct {
                           3: two-factor register engine, one kernel per pass;
                           4: two-factor fused y+x planes, psi(r) recomputed on chip
                           (square x-y planes only; OA_EINVAL where not compiled) */
+  int use_graph;       /* 1: oa_step replays its kernel chain from HIP graphs (captured on
+                          the second call; the Newton-Schulz launch stays outside); stage
+                          timings then read: [0] everything before Newton-Schulz, [8]
+                          Newton-Schulz, [9] Psi T (+ residual), [12] whole step */
   int ns_mode;         /* 0: one cooperative launch when all tiles fit (default),
                           1: two launches per iteration with a device-side early exit */
 } oa_options;
@@ -186,10 +190,14 @@ oa_status oa_residual(oa_ctx *ctx, double *resid_max, double *resid_rms);
 oa_status oa_sync(oa_ctx *ctx);
 oa_status oa_step_result(oa_ctx *ctx, oa_step_out *out);
 
-/* Device time (ms) of the last oa_step once oa_set_timing(ctx, 1), from HIP
- * events on the context's stream: [0] z gather+FFT, [1] y-FFT, [2] x c2r +
- * rho, [3] V psi + x r2c, [4] y-FFT back, [5] z-FFT + pack + kinetic force,
- * [6] EKE/E_loc reductions + Lambda, [7] SD + redundancy sync, [8] S,
+/* Device time (ms) per stage of the timed oa_step calls once
+ * oa_set_timing(ctx, 1) (average over the last <= 64 steps), from HIP
+ * events on the context's stream: [0] z gather+FFT, [1] y-FFT (fused
+ * planes: y+x G->R), [2] x c2r + rho (fused planes: rho reduction),
+ * [3] V psi + x r2c (fused planes: V psi, x and y R->G), [4] y-FFT back
+ * (fused planes: empty), [5] z-FFT + pack + kinetic force,
+ * [6] EKE/E_loc reductions + Lambda (for n <= 128 with the SD update in the
+ * same pass), [7] SD (or its |F| reduction) + redundancy sync, [8] S,
  * [9] Newton-Schulz, [10] Psi T, [11] residual, [12] whole step.
  * Fills min(n, OA_NTIMINGS) entries. */
 #define OA_NTIMINGS 13

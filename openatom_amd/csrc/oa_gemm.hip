@@ -3,24 +3,32 @@
 //
 //   gram    : S = 2 Re Psi^H Psi over stored points with kx=0 halved
 //             (PairCalculator::multiplyForward, ckPairCalculator.C:682-802)
-//             and Lambda = Re Psi^H F (asymmetric PairCalculator, alpha = 1);
-//             split-K over the long point axis, the symmetric case computes
-//             the upper tiles only.
+//             and Lambda = Re Psi^H F (asymmetric PairCalculator, alpha = 1).
+//             One workgroup owns a 128 x 128 output block and one slice of
+//             the long point axis, so each slice of Psi is read once per
+//             block; the slices' partial blocks are summed in a fixed order.
 //   rotate  : Psi' = Psi T and F -= w Psi Lambda
 //             (bwMultiplyHelper, ckPairCalculator.C:1189-1425;
-//              unpackLambda, CP_State_GSpacePlane.C:2045-2225).
+//              unpackLambda, CP_State_GSpacePlane.C:2045-2225), optionally
+//             followed in the same pass by the steepest-descent update
+//             psi += dt/m F (CP_integrate_min_STD, cp_min_std.C:56-86).
 //   ns      : Newton-Schulz iteration for (S/2)^{-1/2}
 //             (Ortho::do_iteration/step_2/step_3/tolerance_check,
-//              ortho.C:862-985; serial twin cp_lowdin.C:180-265), either one
-//             cooperative launch that loops on the device with grid barriers
-//             or, as a fallback, two launches per iteration with a device-side
-//             early exit.
+//              ortho.C:862-985; serial twin cp_lowdin.C:180-265): one
+//             cooperative launch that loops on the device with grid
+//             barriers, operand panels staged in LDS; fallback: two launches
+//             per iteration with a device-side early exit.
 //
-// All three feed the matrix cores straight from global memory (no LDS
-// staging): inside one MFMA step the assignment of the four k values to the
-// four lane groups is free as long as A and B agree, so lane group g takes
-// the contiguous run k0+4g..k0+4g+3 and each lane issues 32-byte loads.
+// MFMA operand layout (measured on gfx950): lane l supplies A[l%16][l/16]
+// and B[l/16][l%16] and receives D[l/16 + 4v][l%16] in element v.  Within
+// one MFMA the assignment of the four k values to the four lane groups is
+// free as long as A and B agree: in the gram, lane group g takes the
+// contiguous run k0+4g..k0+4g+3 over four consecutive MFMAs, so every lane
+// issues one 32-byte load per operand row and 16 k values.
 #include <hip/hip_cooperative_groups.h>
+
+#include <algorithm>
+#include <type_traits>
 
 #include "oa_common.h"
 #include "oa_kernels.h"
@@ -29,218 +37,644 @@ namespace oa {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-// D(16x16) += A(16x4) B(4x16).  Lane l supplies A[l%16][l/16] and
-// B[l/16][l%16]; it receives D[l/16 + 4*v][l%16] in element v (layout
-// measured on gfx950 with a single-wave probe).
 __device__ __forceinline__ v4d mfma16(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// ------------------------------------------------------------------ gram
-// part[sp][i][j] = sum_{k in slice sp} w(k) A[i][k] B[j][k].  One workgroup
-// = one 32x32 output tile and one K slice; its four waves take a quarter of
-// the slice each and are summed through LDS.
-#define GT 32
+__device__ __forceinline__ v4d vzero() { return (v4d){0.0, 0.0, 0.0, 0.0}; }
 
-__global__ __launch_bounds__(256) void k_gram(GramArgs a, int ntile) {
-  __shared__ double red[4][GT][GT + 1];
-  int ti, tj;
-  if (a.sym) {
-    int b = blockIdx.x;
-    ti = 0;
-    while (b >= ntile - ti) { b -= ntile - ti; ti++; }
-    tj = ti + b;
-  } else {
-    ti = blockIdx.x / ntile;
-    tj = blockIdx.x - ti * ntile;
+// f(integral_constant<int, I>) for I in [B, E): compile-time indices into
+// register arrays
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
   }
-  const int sp = blockIdx.y;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int l16 = lane & 15, g = lane >> 4;
-  const double* B = a.sym ? a.A : a.B;
-  const int kbeg = sp * a.kslice;
-  const int kend = min(a.K, kbeg + a.kslice);
-  const int q = ((a.kslice / 4) + 15) & ~15;
-  const int wb = kbeg + wave * q, we = min(kend, wb + q);
-  int ra[2], rb[2];
-  bool va[2], vb[2];
+}
+
+// ================================================================== gram
+constexpr int kGB = 128;        // output block edge
+
+
+// Diagonal block of the symmetric product: its 36 upper tiles (ti <= tj) in
+// four groups with few distinct row fragments per wave:
+//   wave 0: ti <= tj < 4           (10 tiles, row tiles 0-3)
+//   wave 1: ti in {0,1}, tj >= 4   ( 8 tiles, row tiles 0,1,4-7)
+//   wave 2: ti in {2,3}, tj >= 4   ( 8 tiles, row tiles 2,3,4-7)
+//   wave 3: 4 <= ti <= tj          (10 tiles, row tiles 4-7)
+// Fragment slot f of wave W holds row tile dg_rt(W, f); tile t of wave W is
+// the product of slots dg_a(W, t) and dg_b(W, t).
+__host__ __device__ constexpr int up_i(int idx, int m) {
+  int i = 0;
+  while (idx >= m - i) {
+    idx -= m - i;
+    i++;
+  }
+  return i;
+}
+__host__ __device__ constexpr int up_j(int idx, int m) {
+  int i = 0;
+  while (idx >= m - i) {
+    idx -= m - i;
+    i++;
+  }
+  return i + idx;
+}
+__host__ __device__ constexpr int dg_nf(int W) { return (W == 0 || W == 3) ? 4 : 6; }
+__host__ __device__ constexpr int dg_nt(int W) { return (W == 0 || W == 3) ? 10 : 8; }
+__host__ __device__ constexpr int dg_rt(int W, int f) {
+  return W == 0 ? f : W == 3 ? 4 + f : W == 2 ? 2 + f : (f < 2 ? f : 2 + f);
+}
+__host__ __device__ constexpr int dg_a(int W, int t) {
+  return (W == 0 || W == 3) ? up_i(t, 4) : t / 4;
+}
+__host__ __device__ constexpr int dg_b(int W, int t) {
+  return (W == 0 || W == 3) ? up_j(t, 4) : 2 + t % 4;
+}
+
+// One K slice: chunks [c0, c1) of 16 k values starting at `base`; loads are
+// masked to the region [rs, re), which is either the kx=0 part [0, k_lo) or
+// the rest [k_lo, K), so a slice has one weight.
+struct GSlice {
+  int rs, re, base, c0, c1;
+  double w;
+};
+
+__device__ __forceinline__ GSlice gram_slice(const GramArgs& a, int sp, int nlo) {
+  GSlice s;
+  int idx, cnt;
+  const int klo = min(max(a.k_lo, 0), a.K);
+  if (sp < nlo) {
+    s.rs = 0;
+    s.re = klo;
+    idx = sp;
+    cnt = nlo;
+    s.w = a.w_lo;
+  } else {
+    s.rs = klo;
+    s.re = a.K;
+    idx = sp - nlo;
+    cnt = a.nslice - nlo;
+    s.w = a.w_hi;
+  }
+  s.base = s.rs & ~15;
+  const int nch = (s.re - s.base + 15) >> 4;
+  s.c0 = (int)(((long long)idx * nch) / cnt);
+  s.c1 = (int)(((long long)(idx + 1) * nch) / cnt);
+  return s;
+}
+
+// f[e] = X[row][kk + e] (kk a multiple of 4), zero outside [rs, re) or for an
+// invalid row; `full` says the whole 16-wide chunk lies inside [rs, re)
+__device__ __forceinline__ void gfrag(const double* __restrict__ X, int lda, int row, bool vr,
+                                      int kk, bool full, int rs, int re, double (&f)[4]) {
+  if (vr && full) {
+    const double4 v = *reinterpret_cast<const double4*>(X + (size_t)row * lda + kk);
+    f[0] = v.x;
+    f[1] = v.y;
+    f[2] = v.z;
+    f[3] = v.w;
+  } else {
 #pragma unroll
-  for (int x = 0; x < 2; x++) {
-    ra[x] = ti * GT + x * 16 + l16;
-    rb[x] = tj * GT + x * 16 + l16;
+    for (int e = 0; e < 4; e++) {
+      f[e] = 0.0;
+      if (vr && kk + e >= rs && kk + e < re) f[e] = X[(size_t)row * lda + kk + e];
+    }
+  }
+}
+
+// Register ring of D chunk buffers: the loads of chunk c + D are issued as
+// soon as the MFMAs of chunk c have been issued, so D - 1 chunks of MFMAs
+// cover the memory latency.  load(c, buf) fills a buffer for chunk c;
+// mma(buf) issues the chunk's MFMAs.
+template <int D, typename Buf, typename L, typename M>
+__device__ __forceinline__ void chunk_ring(int c0, int c1, Buf (&ring)[D], L&& load, M&& mma) {
+  static_for<0, D>([&](auto dc) __attribute__((always_inline)) {
+    constexpr int d = decltype(dc)::value;
+    if (c0 + d < c1) load(c0 + d, ring[d]);
+  });
+  for (int c = c0; c < c1; c += D) {
+    static_for<0, D>([&](auto dc) __attribute__((always_inline)) {
+      constexpr int d = decltype(dc)::value;
+      if (c + d < c1) {
+        mma(ring[d]);
+        if (c + d + D < c1) load(c + d + D, ring[d]);
+      }
+    });
+  }
+}
+
+template <int NF>
+struct Frag {
+  double f[NF][4];
+};
+constexpr int kGramRing = 3;
+
+// Diagonal block of the symmetric product, wave W (tile groups above); the
+// row fragments serve as both operands.
+template <int W>
+__device__ __forceinline__ void gram_diag_wave(const GramArgs& a, const GSlice& s, int sp,
+                                               int bi) {
+  constexpr int NF = dg_nf(W), NTW = dg_nt(W);
+  const int lane = threadIdx.x & 63, l16 = lane & 15, g = lane >> 4;
+  int row[NF];
+  bool vr[NF];
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    row[f] = bi * kGB + 16 * dg_rt(W, f) + l16;
+    vr[f] = row[f] < a.n;
+  }
+  v4d acc[NTW];
+#pragma unroll
+  for (int t = 0; t < NTW; t++) acc[t] = vzero();
+  Frag<NF> ring[kGramRing];
+  chunk_ring(
+      s.c0, s.c1, ring,
+      [&](int c, Frag<NF>& d) __attribute__((always_inline)) {
+        const int k0 = s.base + 16 * c;
+        const bool full = k0 >= s.rs && k0 + 16 <= s.re;
+        const int kk = k0 + 4 * g;
+#pragma unroll
+        for (int f = 0; f < NF; f++) gfrag(a.A, a.lda, row[f], vr[f], kk, full, s.rs, s.re, d.f[f]);
+      },
+      [&](const Frag<NF>& d) __attribute__((always_inline)) {
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+          static_for<0, NTW>([&](auto tc) __attribute__((always_inline)) {
+            constexpr int t = decltype(tc)::value;
+            constexpr int ia = dg_a(W, t), ib = dg_b(W, t);
+            acc[t] = mfma16(d.f[ia][e], d.f[ib][e], acc[t]);
+          });
+      });
+  double* part = a.part + (size_t)sp * a.n * a.n;
+#pragma unroll
+  for (int t = 0; t < NTW; t++) {
+    const int r0 = bi * kGB + 16 * dg_rt(W, dg_a(W, t));
+    const int c0 = bi * kGB + 16 * dg_rt(W, dg_b(W, t));
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int r = r0 + g + 4 * v, col = c0 + l16;
+      if (r < a.n && col < a.n) part[(size_t)r * a.n + col] = s.w * acc[t][v];
+    }
+  }
+}
+
+// Full block: wave (wr, wc) owns the 64 x 64 quadrant (4 x 4 tiles).
+__device__ __forceinline__ void gram_full_wave(const GramArgs& a, const double* __restrict__ B,
+                                               const GSlice& s, int sp, int bi, int bj) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, l16 = lane & 15, g = lane >> 4;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int rbase = bi * kGB + wr * 64, cbase = bj * kGB + wc * 64;
+  if (rbase >= a.n || cbase >= a.n) return;
+  int ra[4], rb[4];
+  bool va[4], vb[4];
+#pragma unroll
+  for (int x = 0; x < 4; x++) {
+    ra[x] = rbase + 16 * x + l16;
+    rb[x] = cbase + 16 * x + l16;
     va[x] = ra[x] < a.n;
     vb[x] = rb[x] < a.n;
   }
-  v4d acc[2][2];
+  v4d acc[4][4];
 #pragma unroll
-  for (int x = 0; x < 2; x++)
+  for (int x = 0; x < 4; x++)
 #pragma unroll
-    for (int y = 0; y < 2; y++) acc[x][y] = (v4d){0.0, 0.0, 0.0, 0.0};
-  for (int k0 = wb; k0 < we; k0 += 16) {
-    const int kk = k0 + 4 * g;
-    double av[2][4], bv[2][4];
-    if (k0 + 16 <= we) {
+    for (int y = 0; y < 4; y++) acc[x][y] = vzero();
+  Frag<8> ring[kGramRing];  // fragments 0-3: rows of A, 4-7: rows of B
+  chunk_ring(
+      s.c0, s.c1, ring,
+      [&](int c, Frag<8>& d) __attribute__((always_inline)) {
+        const int k0 = s.base + 16 * c;
+        const bool full = k0 >= s.rs && k0 + 16 <= s.re;
+        const int kk = k0 + 4 * g;
 #pragma unroll
-      for (int x = 0; x < 2; x++) {
-        const double* pa = a.A + (size_t)(va[x] ? ra[x] : 0) * a.lda + kk;
-        const double* pb = B + (size_t)(vb[x] ? rb[x] : 0) * a.lda + kk;
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-          av[x][e] = va[x] ? pa[e] : 0.0;
-          bv[x][e] = vb[x] ? pb[e] : 0.0;
+        for (int x = 0; x < 4; x++) {
+          gfrag(a.A, a.lda, ra[x], va[x], kk, full, s.rs, s.re, d.f[x]);
+          gfrag(B, a.lda, rb[x], vb[x], kk, full, s.rs, s.re, d.f[4 + x]);
         }
-      }
-    } else {
+      },
+      [&](const Frag<8>& d) __attribute__((always_inline)) {
 #pragma unroll
-      for (int x = 0; x < 2; x++)
+        for (int e = 0; e < 4; e++)
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-          const int k = kk + e;
-          av[x][e] = (va[x] && k < we) ? a.A[(size_t)ra[x] * a.lda + k] : 0.0;
-          bv[x][e] = (vb[x] && k < we) ? B[(size_t)rb[x] * a.lda + k] : 0.0;
-        }
-    }
+          for (int x = 0; x < 4; x++)
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const double w = (kk + e < a.k_lo) ? a.w_lo : a.w_hi;
+            for (int y = 0; y < 4; y++) acc[x][y] = mfma16(d.f[x][e], d.f[4 + y][e], acc[x][y]);
+      });
+  double* part = a.part + (size_t)sp * a.n * a.n;
 #pragma unroll
-      for (int x = 0; x < 2; x++) av[x][e] *= w;
-    }
+  for (int x = 0; x < 4; x++)
 #pragma unroll
-    for (int e = 0; e < 4; e++)
-#pragma unroll
-      for (int x = 0; x < 2; x++)
-#pragma unroll
-        for (int y = 0; y < 2; y++) acc[x][y] = mfma16(av[x][e], bv[y][e], acc[x][y]);
-  }
-#pragma unroll
-  for (int x = 0; x < 2; x++)
-#pragma unroll
-    for (int y = 0; y < 2; y++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) red[wave][x * 16 + g + 4 * v][y * 16 + l16] = acc[x][y][v];
-  __syncthreads();
-  for (int e = tid; e < GT * GT; e += 256) {
-    const int r = e / GT, c = e - r * GT;
-    const int row = ti * GT + r, col = tj * GT + c;
-    if (row < a.n && col < a.n)
-      a.part[((size_t)sp * a.n + row) * a.n + col] =
-          red[0][r][c] + red[1][r][c] + red[2][r][c] + red[3][r][c];
-  }
-}
-
-int gram_nsplit(int K, int kslice) { return (K + kslice - 1) / kslice; }
-
-hipError_t gram_f64(const GramArgs& a, hipStream_t st) {
-  const int ntile = (a.n + GT - 1) / GT;
-  const int ntiles = a.sym ? ntile * (ntile + 1) / 2 : ntile * ntile;
-  dim3 grid(ntiles, gram_nsplit(a.K, a.kslice));
-  hipLaunchKernelGGL(k_gram, grid, dim3(256), 0, st, a, ntile);
-  return hipGetLastError();
-}
-
-__global__ void k_gram_reduce(const double* __restrict__ part, int nsplit, int n, int sym,
-                              double alpha, double* __restrict__ C) {
-  const size_t nn = (size_t)n * n;
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nn;
-       e += (size_t)gridDim.x * blockDim.x) {
-    int i = (int)(e / n), j = (int)(e - (size_t)i * n);
-    int si = i, sj = j;
-    if (sym && (i / GT) > (j / GT)) { si = j; sj = i; }
-    const double* p = part + (size_t)si * n + sj;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int sp = 0;
-    for (; sp + 4 <= nsplit; sp += 4) {
-      a0 += p[(size_t)sp * nn];
-      a1 += p[(size_t)(sp + 1) * nn];
-      a2 += p[(size_t)(sp + 2) * nn];
-      a3 += p[(size_t)(sp + 3) * nn];
-    }
-    for (; sp < nsplit; sp++) a0 += p[(size_t)sp * nn];
-    C[e] = alpha * ((a0 + a1) + (a2 + a3));
-  }
-}
-
-hipError_t gram_reduce(const double* part, int nsplit, int n, int sym, double alpha, double* C,
-                       hipStream_t st) {
-  const size_t nn = (size_t)n * n;
-  int blocks = (int)((nn + 255) / 256);
-  if (blocks > 2048) blocks = 2048;
-  hipLaunchKernelGGL(k_gram_reduce, dim3(blocks), dim3(256), 0, st, part, nsplit, n, sym, alpha,
-                     C);
-  return hipGetLastError();
-}
-
-// ------------------------------------------------------------------ rotate
-// Y[j][k] = beta Y[j][k] + s(k) sum_i M[i][j] X[i][k].  Workgroup: 32 columns
-// of k and up to 128 rows of j; wave w owns j rows [32w, 32w+32).
-__global__ __launch_bounds__(256) void k_rotate(RotArgs a) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int l16 = lane & 15, g = lane >> 4;
-  const int k0 = blockIdx.x * 32;
-  const int jw = blockIdx.y * 128 + wave * 32;
-  if (jw >= a.n) return;
-  v4d acc[2][2];
-#pragma unroll
-  for (int x = 0; x < 2; x++)
-#pragma unroll
-    for (int y = 0; y < 2; y++) acc[x][y] = (v4d){0.0, 0.0, 0.0, 0.0};
-  const int kc[2] = {k0 + l16, k0 + 16 + l16};
-  const int jc[2] = {jw + l16, jw + 16 + l16};
-  for (int i0 = 0; i0 < a.n; i0 += 4) {
-    const int i = i0 + g;
-    const bool vi = i < a.n;
-    double am[2], bx[2];
-#pragma unroll
-    for (int x = 0; x < 2; x++) am[x] = (vi && jc[x] < a.n) ? a.M[(size_t)i * a.n + jc[x]] : 0.0;
-#pragma unroll
-    for (int y = 0; y < 2; y++) bx[y] = (vi && kc[y] < a.K) ? a.X[(size_t)i * a.ldx + kc[y]] : 0.0;
-#pragma unroll
-    for (int x = 0; x < 2; x++)
-#pragma unroll
-      for (int y = 0; y < 2; y++) acc[x][y] = mfma16(am[x], bx[y], acc[x][y]);
-  }
-#pragma unroll
-  for (int x = 0; x < 2; x++)
-#pragma unroll
-    for (int y = 0; y < 2; y++)
+    for (int y = 0; y < 4; y++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int j = jw + x * 16 + g + 4 * v;
-        const int k = k0 + y * 16 + l16;
+        const int r = rbase + 16 * x + g + 4 * v, col = cbase + 16 * y + l16;
+        if (r < a.n && col < a.n) part[(size_t)r * a.n + col] = s.w * acc[x][y][v];
+      }
+}
+
+// grid (output blocks, K slices); nbe = blocks per edge, nlo = slices of the
+// kx=0 region
+__global__ __launch_bounds__(256) void k_gram(GramArgs a, int nbe, int nlo) {
+  int bi, bj, b = blockIdx.x;
+  if (a.sym) {
+    bi = 0;
+    while (b >= nbe - bi) {
+      b -= nbe - bi;
+      bi++;
+    }
+    bj = bi + b;
+  } else {
+    bi = b / nbe;
+    bj = b - bi * nbe;
+  }
+  const int sp = blockIdx.y;
+  const GSlice s = gram_slice(a, sp, nlo);
+  if (a.sym && bi == bj) {
+    switch (threadIdx.x >> 6) {
+      case 0: gram_diag_wave<0>(a, s, sp, bi); break;
+      case 1: gram_diag_wave<1>(a, s, sp, bi); break;
+      case 2: gram_diag_wave<2>(a, s, sp, bi); break;
+      default: gram_diag_wave<3>(a, s, sp, bi); break;
+    }
+  } else {
+    gram_full_wave(a, a.sym ? a.A : a.B, s, sp, bi, bj);
+  }
+}
+
+static int gram_nlo(const GramArgs& a) {
+  if (a.k_lo <= 0) return 0;
+  if (a.k_lo >= a.K) return a.nslice;
+  const int nlo = (int)((double)a.nslice * a.k_lo / a.K + 0.5);
+  return std::min(std::max(nlo, 1), a.nslice - 1);
+}
+
+int gram_slices(int n, int K, int k_lo, int sym, int target) {
+  const int nbe = (n + kGB - 1) / kGB;
+  const int blocks = sym ? nbe * (nbe + 1) / 2 : nbe * nbe;
+  int ns = std::max(2, target / std::max(1, blocks));
+  ns = std::min(ns, std::max(2, K / 64));  // at least four chunks per slice
+  (void)k_lo;
+  return ns;
+}
+
+hipError_t gram_f64(const GramArgs& a, hipStream_t st) {
+  if (a.nslice < 1 || (a.k_lo > 0 && a.k_lo < a.K && a.nslice < 2) || (a.lda & 3))
+    return hipErrorInvalidValue;
+  const int nbe = (a.n + kGB - 1) / kGB;
+  const int blocks = a.sym ? nbe * (nbe + 1) / 2 : nbe * nbe;
+  hipLaunchKernelGGL(k_gram, dim3(blocks, a.nslice), dim3(256), 0, st, a, nbe, gram_nlo(a));
+  return hipGetLastError();
+}
+
+// C[e] = alpha * sum_sp part[sp][e'] with e' = e or its mirror; 64 outputs
+// per workgroup, 16 groups of slices summed through LDS in a fixed order
+__global__ __launch_bounds__(1024) void k_gram_reduce(const double* __restrict__ part,
+                                                      int nslice, int n, int sym, double alpha,
+                                                      double* __restrict__ C) {
+  __shared__ double red[16][64];
+  const int x = threadIdx.x & 63, y = threadIdx.x >> 6;
+  const size_t nn = (size_t)n * n;
+  const size_t e = (size_t)blockIdx.x * 64 + x;
+  double s0 = 0.0, s1 = 0.0;
+  if (e < nn) {
+    int i = (int)(e / n), j = (int)(e - (size_t)i * n);
+    if (sym && (i >> 4) > (j >> 4)) {
+      const int t = i;
+      i = j;
+      j = t;
+    }
+    const double* p = part + (size_t)i * n + j;
+    int sp = y;
+    for (; sp + 16 < nslice; sp += 32) {
+      s0 += p[(size_t)sp * nn];
+      s1 += p[(size_t)(sp + 16) * nn];
+    }
+    if (sp < nslice) s0 += p[(size_t)sp * nn];
+  }
+  red[y][x] = s0 + s1;
+  __syncthreads();
+  if (y == 0 && e < nn) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < 16; w++) t += red[w][x];
+    C[e] = alpha * t;
+  }
+}
+
+hipError_t gram_reduce(const double* part, int nslice, int n, int sym, double alpha, double* C,
+                       hipStream_t st) {
+  const size_t nn = (size_t)n * n;
+  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)((nn + 63) / 64)), dim3(1024), 0, st, part,
+                     nslice, n, sym, alpha, C);
+  return hipGetLastError();
+}
+
+// ================================================================== rotate
+// Y[j][k] = beta Y[j][k] + s(k) sum_i M[i][j] X[i][k] as D = M^T X: the
+// output tile rows are j and its columns k, so the stores run along k.
+// Workgroup: 64 columns k x 128 rows j; wave w owns j tiles 2w, 2w+1 of the
+// block.  The i loop runs in chunks of 16 with the next chunk's operands
+// loaded while the current chunk's 32 MFMAs issue.
+constexpr int kRK = 64;
+
+__global__ __launch_bounds__(256) void k_rotate(RotArgs a) {
+  __shared__ double red[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, g = lane >> 4;
+  const int kb = blockIdx.x * kRK;
+  const int jb = blockIdx.y * 128 + wave * 32;
+  int jc[2], kc[4];
+  bool vj[2], vk[4];
+#pragma unroll
+  for (int x = 0; x < 2; x++) {
+    jc[x] = jb + 16 * x + l16;
+    vj[x] = jc[x] < a.n;
+  }
+#pragma unroll
+  for (int y = 0; y < 4; y++) {
+    kc[y] = kb + 16 * y + l16;
+    vk[y] = kc[y] < a.K;
+  }
+  v4d acc[2][4];
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++) acc[x][y] = vzero();
+  if (jb < a.n) {
+    double am0[4][2], bx0[4][4], am1[4][2], bx1[4][4];
+    auto load = [&](int i0, double (&am)[4][2], double (&bx)[4][4]) {
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+        const int i = i0 + 4 * s + g;
+        const bool vi = i < a.n;
+#pragma unroll
+        for (int x = 0; x < 2; x++) am[s][x] = (vi && vj[x]) ? a.M[(size_t)i * a.n + jc[x]] : 0.0;
+#pragma unroll
+        for (int y = 0; y < 4; y++)
+          bx[s][y] = (vi && vk[y]) ? a.X[(size_t)i * a.ldx + kc[y]] : 0.0;
+      }
+    };
+    auto mma = [&](const double (&am)[4][2], const double (&bx)[4][4]) {
+#pragma unroll
+      for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int x = 0; x < 2; x++)
+#pragma unroll
+          for (int y = 0; y < 4; y++) acc[x][y] = mfma16(am[s][x], bx[s][y], acc[x][y]);
+    };
+    int i0 = 0;
+    load(0, am0, bx0);
+    while (i0 < a.n) {
+      if (i0 + 16 < a.n) load(i0 + 16, am1, bx1);
+      mma(am0, bx0);
+      i0 += 16;
+      if (i0 >= a.n) break;
+      if (i0 + 16 < a.n) load(i0 + 16, am0, bx0);
+      mma(am1, bx1);
+      i0 += 16;
+    }
+  }
+  // fused steepest descent: every read of X (= sd_psi) in this workgroup
+  // precedes the updates below; no other workgroup reads these columns
+  if (a.sd_psi) __syncthreads();
+  double fn = 0.0;
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int j = jb + 16 * x + g + 4 * v;
+        const int k = kc[y];
         if (j < a.n && k < a.K) {
           const double s = (k < a.k_lo) ? a.s_lo : a.s_hi;
           double* yp = a.Y + (size_t)j * a.ldx + k;
-          const double val = s * acc[x][y][v];
-          *yp = (a.beta != 0.0) ? a.beta * (*yp) + val : val;
+          double val = s * acc[x][y][v];
+          if (a.beta != 0.0) val = a.beta * (*yp) + val;
+          *yp = val;
+          if (a.sd_psi) {
+            a.sd_psi[(size_t)j * a.ldx + k] += a.dtm[k >> 1] * val;
+            fn += val * val;
+          }
         }
       }
+  if (a.sd_psi) {
+    for (int o = 32; o > 0; o >>= 1) fn += __shfl_down(fn, o, 64);
+    if (lane == 0) red[wave] = fn;
+    __syncthreads();
+    if (tid == 0)
+      a.fpart[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+  }
+}
+
+// n <= 128: persistent workgroups.  M is staged once per workgroup in LDS
+// (all 128 x 128, zero beyond n, row stride 129); workgroup b walks the
+// 64-column k-blocks b, b + gridDim.x, ...  The work items are (k-block,
+// chunk of 16 i); a ring of kRotRing X buffers keeps the loads of item q+4
+// in flight while the MFMAs of items q+1..q+3 issue, across k-block
+// boundaries.  Wave w owns the output rows j in [32w, 32w + 32).
+constexpr int kRotRing = 4;
+constexpr int kRotCh = 8;                      // i chunks per k-block (n <= 128)
+constexpr int kRotLd = 129;                    // LDS row stride of M
+constexpr size_t kRotSmem = (size_t)128 * kRotLd * sizeof(double);
+
+struct XFrag {
+  double v[4][4];  // [i step][k tile]
+};
+
+__global__ __launch_bounds__(256) void k_rotate_p(RotArgs a) {
+  extern __shared__ __align__(16) double msm[];
+  __shared__ double red[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, g = lane >> 4;
+  for (int e = tid; e < 128 * 128; e += 256) {
+    const int i = e >> 7, j = e & 127;
+    msm[i * kRotLd + j] = (i < a.n && j < a.n) ? a.M[(size_t)i * a.n + j] : 0.0;
+  }
+  __syncthreads();
+  const int jb = wave * 32;
+  const bool wact = jb < a.n;
+  const int nkb = (a.K + kRK - 1) / kRK;
+  const int nmine =
+      (int)blockIdx.x < nkb ? (nkb - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x : 0;
+  const int nitems = nmine * kRotCh;
+  auto kblock = [&](int q) { return (int)blockIdx.x + (q / kRotCh) * (int)gridDim.x; };
+  auto load = [&](int q, XFrag& d) __attribute__((always_inline)) {
+    const int kb = kblock(q) * kRK, ic = q % kRotCh;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const int i = ic * 16 + 4 * s + g;
+      const bool vi = i < a.n;
+#pragma unroll
+      for (int y = 0; y < 4; y++) {
+        const int k = kb + 16 * y + l16;
+        d.v[s][y] = (vi && k < a.K) ? a.X[(size_t)i * a.ldx + k] : 0.0;
+      }
+    }
+  };
+  v4d acc[2][4];
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++) acc[x][y] = vzero();
+  double fn = 0.0;
+  // output of one k-block: Y = beta Y + s(k) acc, optionally the SD update
+  auto epilogue = [&](int kb) __attribute__((always_inline)) {
+    if (a.sd_psi) __syncthreads();  // all waves are done reading these X columns
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+      for (int y = 0; y < 4; y++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int j = jb + 16 * x + g + 4 * v;
+          const int k = kb + 16 * y + l16;
+          if (j < a.n && k < a.K) {
+            const double s = (k < a.k_lo) ? a.s_lo : a.s_hi;
+            double* yp = a.Y + (size_t)j * a.ldx + k;
+            double val = s * acc[x][y][v];
+            if (a.beta != 0.0) val = a.beta * (*yp) + val;
+            *yp = val;
+            if (a.sd_psi) {
+              a.sd_psi[(size_t)j * a.ldx + k] += a.dtm[k >> 1] * val;
+              fn += val * val;
+            }
+          }
+          acc[x][y][v] = 0.0;
+        }
+  };
+  XFrag ring[kRotRing];
+  static_for<0, kRotRing>([&](auto dc) __attribute__((always_inline)) {
+    constexpr int d = decltype(dc)::value;
+    if (d < nitems) load(d, ring[d]);
+  });
+  for (int q0 = 0; q0 < nitems; q0 += kRotRing) {
+    static_for<0, kRotRing>([&](auto dc) __attribute__((always_inline)) {
+      constexpr int d = decltype(dc)::value;
+      const int q = q0 + d;
+      if (q < nitems) {
+        const int ic = q % kRotCh;
+        if (wact) {
+#pragma unroll
+          for (int s = 0; s < 4; s++) {
+            const double* mrow = msm + (ic * 16 + 4 * s + g) * kRotLd + jb + l16;
+            const double am0 = mrow[0], am1 = mrow[16];
+#pragma unroll
+            for (int y = 0; y < 4; y++) {
+              acc[0][y] = mfma16(am0, ring[d].v[s][y], acc[0][y]);
+              acc[1][y] = mfma16(am1, ring[d].v[s][y], acc[1][y]);
+            }
+          }
+        }
+        if (q + kRotRing < nitems) load(q + kRotRing, ring[d]);
+        if (ic == kRotCh - 1) epilogue(kblock(q) * kRK);
+      }
+    });
+  }
+  if (a.sd_psi) {
+    for (int o = 32; o > 0; o >>= 1) fn += __shfl_down(fn, o, 64);
+    if (lane == 0) red[wave] = fn;
+    __syncthreads();
+    if (tid == 0) a.fpart[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+  }
+}
+
+static int rotate_pgrid(int K) {
+  static int ncu = 0;
+  if (ncu <= 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        ncu <= 0)
+      ncu = 256;
+  }
+  return std::max(1, std::min(ncu, (K + kRK - 1) / kRK));
+}
+
+int rotate_nwg(int n, int K) {
+  return n <= 128 ? rotate_pgrid(K) : ((K + kRK - 1) / kRK) * ((n + 127) / 128);
 }
 
 hipError_t rotate_f64(const RotArgs& a, hipStream_t st) {
-  dim3 grid((a.K + 31) / 32, (a.n + 127) / 128);
+  if (a.sd_psi && (a.n > 128 || a.sd_psi != a.X || !a.dtm || !a.fpart))
+    return hipErrorInvalidValue;
+  if (a.n <= 128) {
+    static bool granted = false;
+    if (!granted) {
+      const hipError_t e = hipFuncSetAttribute(
+          (const void*)k_rotate_p, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRotSmem);
+      if (e != hipSuccess) return e;
+      granted = true;
+    }
+    hipLaunchKernelGGL(k_rotate_p, dim3(rotate_pgrid(a.K)), dim3(256), kRotSmem, st, a);
+    return hipGetLastError();
+  }
+  dim3 grid((a.K + kRK - 1) / kRK, (a.n + 127) / 128);
   hipLaunchKernelGGL(k_rotate, grid, dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
-// ------------------------------------------------------------------ Newton-Schulz
-// Workgroup = 32x32 tile of the n x n matrices, wave = 16x16 block.
+// ================================================================== Newton-Schulz
+// Workgroup = 32 x 32 tile of the n x n matrices, wave = 16 x 16 block.
 #define NT 32
+constexpr int kNsKP = 128;            // k panel width
+constexpr int kNsLd = kNsKP + 1;      // LDS row stride of a row panel
+constexpr int kNsCl = NT + 1;         // LDS row stride of a column panel
+constexpr int kNsRowPanel = NT * kNsLd;
+constexpr int kNsColPanel = kNsKP * kNsCl;
+constexpr size_t kNsSmem = (size_t)2 * (kNsRowPanel + kNsColPanel) * sizeof(double);
 
-// 16x16 block (row0, col0) of A B, A and B n x n row-major
-__device__ __forceinline__ v4d blk_gemm(const double* __restrict__ A,
-                                        const double* __restrict__ B, int n, int row0,
-                                        int col0) {
-  const int lane = threadIdx.x & 63, l16 = lane & 15, g = lane >> 4;
-  const int r = row0 + l16, cc = col0 + l16;
-  v4d acc = (v4d){0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < n; k0 += 4) {
-    const int k = k0 + g;
-    const double av = (r < n && k < n) ? A[(size_t)r * n + k] : 0.0;
-    const double bv = (k < n && cc < n) ? B[(size_t)k * n + cc] : 0.0;
-    acc = mfma16(av, bv, acc);
+// Stage A[r0.., k0..k0+KP) (row panel) and B[k0..k0+KP, c0..) (column panel)
+__device__ __forceinline__ void ns_stage(const double* __restrict__ A,
+                                         const double* __restrict__ B, int n, int r0, int c0,
+                                         int k0, double* As, double* Bs) {
+  const int tid = threadIdx.x;
+  for (int e = tid; e < NT * kNsKP; e += 256) {
+    const int r = e / kNsKP, k = e - r * kNsKP;
+    As[r * kNsLd + k] = (r0 + r < n && k0 + k < n) ? A[(size_t)(r0 + r) * n + k0 + k] : 0.0;
   }
-  return acc;
+  for (int e = tid; e < kNsKP * NT; e += 256) {
+    const int k = e / NT, cc = e - k * NT;
+    Bs[k * kNsCl + cc] = (k0 + k < n && c0 + cc < n) ? B[(size_t)(k0 + k) * n + c0 + cc] : 0.0;
+  }
+}
+
+__device__ __forceinline__ v4d ns_mma(const double* As, const double* Bs, v4d acc) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, l16 = lane & 15, g = lane >> 4;
+  const int wr = wave >> 1, wc = wave & 1;
+  const double* ap = As + (16 * wr + l16) * kNsLd + g;
+  const double* bp = Bs + g * kNsCl + 16 * wc + l16;
+  // four independent accumulator chains keep the matrix pipe busy
+  v4d p[4] = {acc, vzero(), vzero(), vzero()};
+#pragma unroll
+  for (int kk = 0; kk < kNsKP; kk += 16)
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      p[u] = mfma16(ap[kk + 4 * u], bp[(kk + 4 * u) * kNsCl], p[u]);
+  return (p[0] + p[1]) + (p[2] + p[3]);
+}
+
+// One (or two) tile products through LDS panels: acc1 = A1 B1 and, if A2,
+// acc2 = A2 B2, for the workgroup's tile (r0, c0).
+__device__ __forceinline__ void ns_tile2(const double* A1, const double* B1, const double* A2,
+                                         const double* B2, int n, int r0, int c0, double* sm,
+                                         v4d& acc1, v4d& acc2) {
+  double* As1 = sm;
+  double* Bs1 = As1 + kNsRowPanel;
+  double* As2 = Bs1 + kNsColPanel;
+  double* Bs2 = As2 + kNsRowPanel;
+  acc1 = vzero();
+  acc2 = vzero();
+  for (int k0 = 0; k0 < n; k0 += kNsKP) {
+    __syncthreads();  // the previous panels are consumed
+    ns_stage(A1, B1, n, r0, c0, k0, As1, Bs1);
+    if (A2) ns_stage(A2, B2, n, r0, c0, k0, As2, Bs2);
+    __syncthreads();
+    acc1 = ns_mma(As1, Bs1, acc1);
+    if (A2) acc2 = ns_mma(As2, Bs2, acc2);
+  }
 }
 
 __device__ __forceinline__ void blk_coords(int nt, int& row0, int& col0) {
@@ -299,12 +733,14 @@ __device__ __forceinline__ double ns_err(const double* __restrict__ errpart, int
 }
 
 // One launch for the whole iteration (cooperative, all workgroups resident):
-// X0 = S/2, T0 = I; loop { C = 3I - T X; X' = C X ... } with two grid
-// barriers per iteration; stops when the rms change of T <= tol.
+// X0 = S/2, T0 = I; loop { C = 3I - T X; X' = X C / 2; T' = C T / 2 } with
+// two grid barriers per iteration; stops when the rms change of T <= tol.
 __global__ __launch_bounds__(256) void k_ns_coop(NsArgs a) {
   namespace cg = cooperative_groups;
+  extern __shared__ __align__(16) double ns_smem[];
   cg::grid_group grid = cg::this_grid();
   const int n = a.n, nt = (n + NT - 1) / NT, nwg = nt * nt;
+  const int r0 = (blockIdx.x / nt) * NT, c0 = (blockIdx.x % nt) * NT;
   int row0, col0;
   blk_coords(nt, row0, col0);
   const int lane = threadIdx.x & 63, l16 = lane & 15, g = lane >> 4;
@@ -323,11 +759,13 @@ __global__ __launch_bounds__(256) void k_ns_coop(NsArgs a) {
   double err = 0.0;
   for (int it = 1; it <= a.max_iter; it++) {
     const int o = (it - 1) & 1, w = it & 1;
-    blk_store(a.C, blk_gemm(T[o], X[o], n, row0, col0), n, row0, col0, -1.0, 3.0, nullptr);
+    v4d p1, p2;
+    ns_tile2(T[o], X[o], nullptr, nullptr, n, r0, c0, ns_smem, p1, p2);
+    blk_store(a.C, p1, n, row0, col0, -1.0, 3.0, nullptr);
     grid.sync();
-    blk_store(X[w], blk_gemm(X[o], a.C, n, row0, col0), n, row0, col0, 0.5, 0.0, nullptr);
-    const double e =
-        blk_store(T[w], blk_gemm(a.C, T[o], n, row0, col0), n, row0, col0, 0.5, 0.0, T[o]);
+    ns_tile2(X[o], a.C, a.C, T[o], n, r0, c0, ns_smem, p1, p2);
+    blk_store(X[w], p1, n, row0, col0, 0.5, 0.0, nullptr);
+    const double e = blk_store(T[w], p2, n, row0, col0, 0.5, 0.0, T[o]);
     const double es = wg_sum(e);
     if (threadIdx.x == 0) a.errpart[(size_t)it * nwg + blockIdx.x] = es;
     grid.sync();
@@ -351,6 +789,22 @@ __global__ __launch_bounds__(256) void k_ns_coop(NsArgs a) {
 }
 
 // ---- fallback: two launches per iteration with a device-side early exit
+
+// 16x16 block (row0, col0) of A B straight from global memory
+__device__ __forceinline__ v4d blk_gemm(const double* __restrict__ A,
+                                        const double* __restrict__ B, int n, int row0,
+                                        int col0) {
+  const int lane = threadIdx.x & 63, l16 = lane & 15, g = lane >> 4;
+  const int r = row0 + l16, cc = col0 + l16;
+  v4d acc = vzero();
+  for (int k0 = 0; k0 < n; k0 += 4) {
+    const int k = k0 + g;
+    const double av = (r < n && k < n) ? A[(size_t)r * n + k] : 0.0;
+    const double bv = (k < n && cc < n) ? B[(size_t)k * n + cc] : 0.0;
+    acc = mfma16(av, bv, acc);
+  }
+  return acc;
+}
 
 __global__ void k_ns_init(const double* __restrict__ S, double* __restrict__ X0,
                           double* __restrict__ T0, int n, StepDev* res) {
@@ -468,6 +922,21 @@ static hipError_t ns_run_launches(const NsArgs& a, hipStream_t st) {
   return hipGetLastError();
 }
 
+static bool ns_smem_granted(int device) {
+  static int done = -1;
+  if (done == device) return true;
+  int lds_max = 0;
+  if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
+          hipSuccess ||
+      (size_t)lds_max < kNsSmem)
+    return false;
+  if (hipFuncSetAttribute((const void*)k_ns_coop, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)kNsSmem) != hipSuccess)
+    return false;
+  done = device;
+  return true;
+}
+
 // Can all n x n tiles be resident at once for the cooperative launch?
 bool ns_coop_possible(int n, int device) {
   const int nt = (n + NT - 1) / NT;
@@ -477,8 +946,9 @@ bool ns_coop_possible(int n, int device) {
     return false;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
     return false;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_ns_coop, 256, 0) !=
-      hipSuccess)
+  if (!ns_smem_granted(device)) return false;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_ns_coop, 256,
+                                                   kNsSmem) != hipSuccess)
     return false;
   return (long)per_cu * ncu >= (long)nt * nt;
 }
@@ -488,8 +958,8 @@ hipError_t ns_run(const NsArgs& a, hipStream_t st) {
   const int nt = (a.n + NT - 1) / NT;
   NsArgs args = a;
   void* params[] = {&args};
-  return hipLaunchCooperativeKernel((const void*)k_ns_coop, dim3(nt * nt), dim3(256), params, 0,
-                                    st);
+  return hipLaunchCooperativeKernel((const void*)k_ns_coop, dim3(nt * nt), dim3(256), params,
+                                    kNsSmem, st);
 }
 
 }  // namespace oa

@@ -56,6 +56,9 @@ int fft2_ntr(int n);  // transforms per workgroup for length n, 0 if not compile
 // y back to zout), 2 G->R writing psi(r) of c.psir_state
 hipError_t plane3_launch(int kind, int n, bool fp32, const FftCtx& c, hipStream_t st);
 size_t plane3_smem(int n, int nplane_x, bool fp32);  // 0 if n is not compiled
+// row parts of the G->R grid and resident workgroups per CU of both passes
+hipError_t plane3_info(int n, int nplane_x, bool fp32, int* rowparts, int* occ_g2r,
+                       int* occ_r2g);
 
 // fused y+x plane passes (one z-plane, a chunk of states per workgroup)
 hipError_t plane_fwd(const FftCtx& c, bool fp32, hipStream_t st);
@@ -83,33 +86,43 @@ struct StepDev {
 
 // ---- dense linear algebra (oa_gemm.hip) ----
 // part[sp][i][j] = sum_{k in slice sp} w(k) A[i][k] B[j][k]; A, B are [n][lda]
-// rows of K reals; w(k) = w_lo for k < k_lo else w_hi.  sym: B == A, only
-// upper tiles computed.
+// rows of K reals (lda a multiple of 4, rows 32-byte aligned); w(k) = w_lo
+// for k < k_lo else w_hi.  Every 128 x 128 output block is split into nslice
+// K slices (slices never straddle k_lo).  sym: B == A, only the 16 x 16
+// tiles with i/16 <= j/16 are computed.
 struct GramArgs {
   const double* A;
   const double* B;
   int n, lda, K, k_lo;
   double w_lo, w_hi;
   int sym;
-  int kslice;
-  double* part;
+  int nslice;
+  double* part;  // [nslice][n][n]
 };
-int gram_nsplit(int K, int kslice);
+// K slices per output block for about `target` workgroups in all
+int gram_slices(int n, int K, int k_lo, int sym, int target);
 hipError_t gram_f64(const GramArgs& a, hipStream_t st);
-// C = alpha * sum_sp part[sp]; mirrored from the upper triangle if sym
-hipError_t gram_reduce(const double* part, int nsplit, int n, int sym, double alpha,
+// C = alpha * sum_sp part[sp]; mirrored from the computed tiles if sym
+hipError_t gram_reduce(const double* part, int nslice, int n, int sym, double alpha,
                        double* C, hipStream_t st);
 
 // Y[j][k] = beta * Y[j][k] + s(k) * sum_i M[i][j] X[i][k]; s(k) = s_lo for
 // k < k_lo else s_hi.  X, Y: [n][ldx] (Y may not alias X).
+// With sd_psi set (n <= 128 only) the epilogue also applies the steepest-
+// descent update to the rows of X: sd_psi[j][k] += dtm[k/2] * Ynew[j][k]
+// (sd_psi == X), and writes per-workgroup partials of sum Ynew^2 to fpart.
 struct RotArgs {
   const double* X;
   const double* M;
   double* Y;
   int n, ldx, K, k_lo;
   double s_lo, s_hi, beta;
+  double* sd_psi;
+  const double* dtm;
+  double* fpart;
 };
 hipError_t rotate_f64(const RotArgs& a, hipStream_t st);
+int rotate_nwg(int n, int K);  // workgroups of a rotate launch (fpart entries)
 
 // Newton-Schulz for T = (S/2)^{-1/2} with device-side convergence test
 // (ortho.C:862-985).  Work: X[2], T[2] (n*n each), errpart[(max_iter+1) * nwg].

@@ -367,13 +367,43 @@ enum PlaneKind { G2R = 0, R2G = 1, G2R_PSIR = 2 };
 
 constexpr int kPlaneThreads = 256;
 
+// x rounds per k3_g2r workgroup: the rho accumulators are 2*X2 doubles per
+// round and thread; FP64 keeps one round
+template <typename T>
+constexpr int row_rounds() {
+  return sizeof(T) == 8 ? 1 : 2;
+}
+
+// Launch facts of one size for the host's chunk choice: row parts of the
+// G->R grid and resident workgroups per CU of both kernels.
+template <typename T, int X1, int X2, int Y1, int Y2>
+hipError_t plane3_info(int npx, int* rowparts, int* occ_g2r, int* occ_r2g) {
+  constexpr int NT = kPlaneThreads;
+  using S = PS<X1, X2, Y1, Y2, NT>;
+  constexpr bool VREG = sizeof(T) == 4 && S::ROUNDS * 2 * X2 <= 48;
+  const size_t smem = plane3_smem<T, X1, X2, Y1, Y2, NT>(npx);
+  *rowparts = (S::ROUNDS + row_rounds<T>() - 1) / row_rounds<T>();
+  const void* g = (const void*)k3_g2r<T, X1, X2, Y1, Y2, NT, row_rounds<T>(), false>;
+  const void* r = (const void*)k3_r2g<T, X1, X2, Y1, Y2, NT, VREG>;
+  hipError_t e;
+  if (smem > 65536) {
+    if ((e = hipFuncSetAttribute(g, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem)) !=
+        hipSuccess)
+      return e;
+    if ((e = hipFuncSetAttribute(r, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem)) !=
+        hipSuccess)
+      return e;
+  }
+  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(occ_g2r, g, NT, smem)) != hipSuccess)
+    return e;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ_r2g, r, NT, smem);
+}
+
 template <typename T, int X1, int X2, int Y1, int Y2>
 hipError_t launch_plane3(int kind, const FftCtx& c, hipStream_t st) {
   constexpr int NT = kPlaneThreads;
   using S = PS<X1, X2, Y1, Y2, NT>;
-  // x rounds per k3_g2r workgroup: the rho accumulators are 2*X2 doubles per
-  // round and thread; FP64 keeps one round to stay within 128 VGPRs
-  constexpr int kRowRounds = sizeof(T) == 8 ? 1 : 2;
+  constexpr int kRowRounds = row_rounds<T>();
   constexpr int RPn = (S::ROUNDS + kRowRounds - 1) / kRowRounds;
   constexpr bool VREG = sizeof(T) == 4 && S::ROUNDS * 2 * X2 <= 48;
   const size_t smem = plane3_smem<T, X1, X2, Y1, Y2, NT>(c.g.nplane_x);

@@ -25,4 +25,10 @@ size_t OA_CAT(plane3_smem_n, OA_FFT2_N)(int npx, bool fp32) {
   return fp32 ? plane3_smem<float, kX1, kX2, kY1, kY2, kPlaneThreads>(npx)
               : plane3_smem<double, kX1, kX2, kY1, kY2, kPlaneThreads>(npx);
 }
+
+hipError_t OA_CAT(plane3_info_n, OA_FFT2_N)(int npx, bool fp32, int* rowparts, int* occ_g2r,
+                                           int* occ_r2g) {
+  return fp32 ? plane3_info<float, kX1, kX2, kY1, kY2>(npx, rowparts, occ_g2r, occ_r2g)
+              : plane3_info<double, kX1, kX2, kY1, kY2>(npx, rowparts, occ_g2r, occ_r2g);
+}
 }  // namespace oa3

@@ -148,7 +148,7 @@ struct oa_ctx {
   double* eke_part = nullptr;
   int nblk = 0;
   double *S = nullptr, *T = nullptr, *lam = nullptr, *gpart = nullptr, *S2 = nullptr;
-  int gsplit_max = 0, kslice = 0;
+  int nslice_sym = 0, nslice_full = 0;  // gram K slices (symmetric S, full Lambda)
   double *nsX0 = nullptr, *nsX1 = nullptr, *nsT0 = nullptr, *nsT1 = nullptr, *nsC = nullptr,
          *errpart = nullptr;
   double *fpart = nullptr, *dpart = nullptr;
@@ -165,6 +165,11 @@ struct oa_ctx {
   bool have_S = false, have_T = false, have_v = false;
   bool ns_coop = false;  // Newton-Schulz as one cooperative launch
   int engine = 0;     // kEngFft2, kEngSep, kEngFused or kEngPlane
+  // graph mode: the two coefficient buffers, and per parity (which buffer
+  // holds psi) the chain before Newton-Schulz and the one after it
+  double2* psi_buf[2] = {nullptr, nullptr};
+  hipGraphExec_t gpre[2] = {nullptr, nullptr}, gpost[2] = {nullptr, nullptr};
+  int steps_run = 0;
   int nzblk = 0;      // EKE partials per state of the active z kernel
   int4* lines = nullptr;
 };
@@ -222,6 +227,28 @@ std::vector<double2> twiddles(int n) {
     t[j] = make_double2(std::cos(a), -std::sin(a));
   }
   return t;
+}
+
+// States per chunk for a grid of units x nchunk workgroups of which `slots`
+// are resident at once: minimise rounds * (states per workgroup + 1), the
+// +1 standing for a workgroup's fixed start-up cost; ties go to fewer
+// chunks.  At most max_chunks chunks.
+void pick_chunks(int units, int slots, int nst, int max_chunks, int& cs, int& nchunk) {
+  double best = 1e300;
+  cs = nst;
+  nchunk = 1;
+  for (int nc = 1; nc <= std::min(nst, std::max(1, max_chunks)); nc++) {
+    const int s = (nst + nc - 1) / nc, ncr = (nst + s - 1) / s;
+    if (ncr != nc) continue;
+    const long wg = (long)units * ncr;
+    const long rounds = (wg + slots - 1) / slots;
+    const double cost = (double)rounds * (s + 1);
+    if (cost < best) {
+      best = cost;
+      cs = s;
+      nchunk = ncr;
+    }
+  }
 }
 
 // Validate the stored layout and build the device tables.
@@ -335,14 +362,16 @@ oa_status build_layout(oa_ctx* c) {
     c->nzblk = (c->nlines + oa::fft2_ntr(c->nz) - 1) / oa::fft2_ntr(c->nz);
   }
   if (c->engine == kEngPlane) {
-    // G->R: about 2 workgroups per CU over (z, row parts, chunks); every
-    // chunk writes one rho partial.  R->G: no partials, finer chunks.
-    const int nchunk = std::max(1, std::min(c->nst, (512 + c->nz - 1) / c->nz));
-    f.cs = (c->nst + nchunk - 1) / nchunk;
-    f.nchunk = (c->nst + f.cs - 1) / f.cs;
-    const int nchunk_r = std::max(1, std::min(c->nst, (1024 + c->nz - 1) / c->nz));
-    f.cs_r = (c->nst + nchunk_r - 1) / nchunk_r;
-    f.nchunk_r = (c->nst + f.cs_r - 1) / f.cs_r;
+    // State chunks of both plane passes from the resident-workgroup count
+    // (every G->R chunk also writes one rho partial of N doubles)
+    int rowparts = 1, occ_g = 1, occ_r = 1, ncu = 0;
+    if (oa::plane3_info(c->nx, c->nplane_x, c->fp32, &rowparts, &occ_g, &occ_r) != hipSuccess)
+      return fail(c, OA_EHIP, "plane pass occupancy query failed");
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) !=
+            hipSuccess || ncu <= 0)
+      ncu = 256;
+    pick_chunks(c->nz * rowparts, std::max(1, occ_g) * ncu, c->nst, 16, f.cs, f.nchunk);
+    pick_chunks(c->nz, std::max(1, occ_r) * ncu, c->nst, c->nst, f.cs_r, f.nchunk_r);
     c->nzblk = (c->nlines + oa::fft2_ntr(c->nz) - 1) / oa::fft2_ntr(c->nz);
   }
   if (c->engine == kEngFused) {
@@ -428,6 +457,8 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
   const size_t csz = 2 * rsz;
   TRY(dalloc(c, &c->psi, gsz));
   TRY(dalloc(c, &c->psi2, gsz));
+  c->psi_buf[0] = c->psi;
+  c->psi_buf[1] = c->psi2;
   TRY(dalloc(c, &c->force, gsz));
   const size_t zsz = (size_t)c->nst * c->nz * c->nplane_x * c->ny * csz;
   TRY(dalloc(c, &c->zbuf, zsz));
@@ -452,16 +483,17 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
   TRY(dalloc(c, &c->S2, nn * sizeof(double)));
   TRY(dalloc(c, &c->T, nn * sizeof(double)));
   TRY(dalloc(c, &c->lam, nn * sizeof(double)));
-  // split-K shape for the gram (32x32 tiles): about 1024 workgroups for the
-  // symmetric product, K slices a multiple of 64 (16 per wave)
+  // gram: one workgroup per (128 x 128 output block, K slice), about one
+  // workgroup per CU in all; the same slice count serves S and Lambda
   const int K = 2 * c->npts;
-  const int ntile = (n + 31) / 32;
-  const int nsplit = std::max(1, 1024 / (ntile * (ntile + 1) / 2));
-  int ks = (K + nsplit - 1) / nsplit;
-  ks = std::max(64, ((ks + 63) / 64) * 64);
-  c->kslice = ks;
-  c->gsplit_max = oa::gram_nsplit(K, ks);
-  TRY(dalloc(c, &c->gpart, (size_t)c->gsplit_max * nn * sizeof(double)));
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) !=
+          hipSuccess || ncu <= 0)
+    ncu = 256;
+  c->nslice_sym = oa::gram_slices(n, K, 2 * c->nkx0, 1, ncu);
+  c->nslice_full = oa::gram_slices(n, K, 2 * c->nkx0, 0, ncu);
+  TRY(dalloc(c, &c->gpart,
+             (size_t)std::max(c->nslice_sym, c->nslice_full) * nn * sizeof(double)));
   TRY(dalloc(c, &c->nsX0, nn * sizeof(double)));
   TRY(dalloc(c, &c->nsX1, nn * sizeof(double)));
   TRY(dalloc(c, &c->nsT0, nn * sizeof(double)));
@@ -470,7 +502,9 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
   const int nt = (n + 31) / 32;
   TRY(dalloc(c, &c->errpart, (size_t)(c->opt.ns_max_iter + 2) * nt * nt * sizeof(double)));
   c->ns_coop = (c->opt.ns_mode == 0) && oa::ns_coop_possible(n, c->device);
-  c->nfpart = 1024;
+  // |F|^2 partials: one per workgroup of the SD kernel or of the rotate that
+  // applies Lambda and the SD update together
+  c->nfpart = std::max(1024, oa::rotate_nwg(n, K));
   c->ndpart = 512;
   TRY(dalloc(c, &c->fpart, c->nfpart * sizeof(double)));
   TRY(dalloc(c, &c->dpart, c->ndpart * sizeof(double)));
@@ -615,17 +649,17 @@ oa_status st_gram(oa_ctx* c, const double2* A, const double2* B, bool sym, doubl
   g.w_lo = wlo;
   g.w_hi = whi;
   g.sym = sym ? 1 : 0;
-  g.kslice = c->kslice;
+  g.nslice = sym ? c->nslice_sym : c->nslice_full;
   g.part = c->gpart;
   HIPCHK(oa::gram_f64(g, c->stream));
-  HIPCHK(oa::gram_reduce(c->gpart, oa::gram_nsplit(g.K, g.kslice), c->nst, g.sym, 1.0, C,
-                         c->stream));
+  HIPCHK(oa::gram_reduce(c->gpart, g.nslice, c->nst, g.sym, 1.0, C, c->stream));
   return OA_OK;
 }
 
+// sd: also apply psi += dt/m F to X (= psi) in the same pass (n <= 128)
 oa_status st_rotate(oa_ctx* c, const double2* X, const double* M, double2* Y, double beta,
-                    double slo, double shi) {
-  oa::RotArgs r;
+                    double slo, double shi, bool sd = false) {
+  oa::RotArgs r{};
   r.X = reinterpret_cast<const double*>(X);
   r.M = M;
   r.Y = reinterpret_cast<double*>(Y);
@@ -636,15 +670,32 @@ oa_status st_rotate(oa_ctx* c, const double2* X, const double* M, double2* Y, do
   r.s_lo = slo;
   r.s_hi = shi;
   r.beta = beta;
+  if (sd) {
+    r.sd_psi = reinterpret_cast<double*>(const_cast<double2*>(X));
+    r.dtm = c->dtm;
+    r.fpart = c->fpart;
+  }
   HIPCHK(oa::rotate_f64(r, c->stream));
   return OA_OK;
 }
 
-oa_status st_lambda(oa_ctx* c) {
-  // Lambda_ij = sum_stored Re conj(psi_i) F_j (alpha = 1, cpaimd.C:2527)
+// Lambda_ij = sum_stored Re conj(psi_i) F_j (alpha = 1, cpaimd.C:2527) and
+// F_j -= w sum_i psi_i Lambda_ij, w = 1/2 on kx=0
+// (CP_State_GSpacePlane.C:2076-2086).  with_sd: the same pass also applies
+// the SD update and leaves the |F|^2 partials (then st_sd_finish).
+oa_status st_lambda(oa_ctx* c, bool with_sd = false) {
   TRY(st_gram(c, c->psi, c->force, false, 1.0, 1.0, c->lam));
-  // F_j -= w sum_i psi_i Lambda_ij, w = 1/2 on kx=0 (CP_State_GSpacePlane.C:2076-2086)
-  TRY(st_rotate(c, c->psi, c->lam, c->force, 1.0, -0.5, -1.0));
+  TRY(st_rotate(c, c->psi, c->lam, c->force, 1.0, -0.5, -1.0, with_sd));
+  return OA_OK;
+}
+
+bool lambda_sd_fusable(const oa_ctx* c) { return c->nst <= 128; }
+
+// |F|^2 from the partials of the fused rotate, then the redundancy sync
+oa_status st_sd_finish(oa_ctx* c) {
+  HIPCHK(oa::reduce_to(c->fpart, oa::rotate_nwg(c->nst, 2 * c->npts), 1.0, &c->res->fnorm2,
+                       c->stream));
+  HIPCHK(oa::red_sync(c->psi, c->red, c->partner, c->nred, c->nst, c->ld, c->stream));
   return OA_OK;
 }
 
@@ -805,6 +856,10 @@ oa_status oa_destroy(oa_ctx* c) {
   if (!c) return OA_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (int p = 0; p < 2; p++) {
+    if (c->gpre[p]) (void)hipGraphExecDestroy(c->gpre[p]);
+    if (c->gpost[p]) (void)hipGraphExecDestroy(c->gpost[p]);
+  }
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->res_host) (void)hipHostFree(c->res_host);
   for (auto& e : c->ev)
@@ -998,29 +1053,100 @@ oa_status oa_rotate(oa_ctx* c) {
   return OA_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The step before Newton-Schulz (GSpaceDriver order, gspace.ci:106-190):
+// G->R, rho, V psi, R->G, EKE and kinetic force, [Lambda], SD update,
+// redundancy sync, S.  marks: record the stage events.
+oa_status enqueue_pre(oa_ctx* c, bool marks) {
+  auto mk = [&](int i) {
+    if (marks) mark(c, i);
+  };
+  TRY(st_g2r(c, marks ? 1 : -1));  // marks 1..3
+  TRY(st_r2g(c, marks ? 1 : -1));  // marks 4..6, then the EKE / E_loc reductions
+  if (c->opt.use_lambda && lambda_sd_fusable(c)) {
+    TRY(st_lambda(c, true));  // Lambda projection and SD update in one pass
+    mk(7);
+    TRY(st_sd_finish(c));
+  } else {
+    if (c->opt.use_lambda) TRY(st_lambda(c));
+    mk(7);
+    TRY(st_sd(c));
+  }
+  mk(8);
+  TRY(st_overlap(c));
+  return OA_OK;
+}
+
+// Psi T into the other buffer (the host pointers swap), then the residual
+oa_status enqueue_post(oa_ctx* c, bool marks) {
+  TRY(st_rot_psi(c));
+  if (marks) mark(c, 11);
+  if (c->opt.want_resid) {
+    TRY(st_gram(c, c->psi, c->psi, true, 1.0, 2.0, c->S2));
+    HIPCHK(oa::resid_stats(c->S2, c->nst, c->res, c->stream));
+  }
+  return OA_OK;
+}
+
+// Capture one half of the step for the current parity into an executable
+// graph.  Nothing runs; the psi/psi2 swap of the post half is undone.
+oa_status capture(oa_ctx* c, bool post, hipGraphExec_t* exec) {
+  double2 *p0 = c->psi, *p1 = c->psi2;
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const oa_status s = post ? enqueue_post(c, false) : enqueue_pre(c, false);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  c->psi = p0;
+  c->psi2 = p1;
+  if (s != OA_OK) {
+    if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+    return s;
+  }
+  HIPCHK(e);
+  const hipError_t ei = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HIPCHK(ei);
+  return OA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 oa_status oa_step(oa_ctx* c, oa_step_out* out) {
   if (!c) return OA_EINVAL;
   if (!c->have_v) return fail(c, OA_ESTATE, "oa_step before oa_vks_set");
   HIPCHK(hipSetDevice(c->device));
   if (c->timing) c->ev_cur = c->ev_count % oa_ctx::kRing;
+  // graph mode from the second step on (the first one runs eagerly and sets
+  // up every kernel attribute outside any capture)
+  const bool graph = c->opt.use_graph && c->steps_run > 0;
   mark(c, 0);
-  TRY(st_g2r(c, 1));   // marks 1..3
-  TRY(st_r2g(c, 1));   // marks 4..6, then the EKE / E_loc reductions
-  if (c->opt.use_lambda) TRY(st_lambda(c));
-  mark(c, 7);
-  TRY(st_sd(c));
-  mark(c, 8);
-  TRY(st_overlap(c));
-  mark(c, 9);
-  TRY(st_ns(c, c->opt.ns_tol, c->opt.ns_max_iter));
-  mark(c, 10);
-  TRY(st_rot_psi(c));
-  mark(c, 11);
-  if (c->opt.want_resid) {
-    TRY(st_gram(c, c->psi, c->psi, true, 1.0, 2.0, c->S2));
-    HIPCHK(oa::resid_stats(c->S2, c->nst, c->res, c->stream));
+  if (graph) {
+    const int p = (c->psi == c->psi_buf[0]) ? 0 : 1;
+    if (!c->gpre[p]) TRY(capture(c, false, &c->gpre[p]));
+    if (!c->gpost[p]) TRY(capture(c, true, &c->gpost[p]));
+    HIPCHK(hipGraphLaunch(c->gpre[p], c->stream));
+    for (int i = 1; i <= 8; i++) mark(c, i);
+    TRY(st_ns(c, c->opt.ns_tol, c->opt.ns_max_iter));
+    mark(c, 9);
+    HIPCHK(hipGraphLaunch(c->gpost[p], c->stream));
+    std::swap(c->psi, c->psi2);  // what the captured rotate did
+    c->have_S = c->have_T = true;
+    mark(c, 10);
+    mark(c, 11);
+  } else {
+    TRY(enqueue_pre(c, true));
+    mark(c, 9);
+    TRY(st_ns(c, c->opt.ns_tol, c->opt.ns_max_iter));
+    mark(c, 10);
+    TRY(enqueue_post(c, true));
   }
   mark(c, 12);
+  c->steps_run++;
   if (c->timing) c->ev_count++;
   if (out) {
     TRY(fetch_result(c, out));

@@ -126,11 +126,14 @@ def initial_coeffs(kx, ky, kz, hmat, nstates, seed, norm=2.0):
 
 def gaussian_vloc(pos, spec, hmat, grid, seed):
     """V(r) = sum_a A_a exp(-|r - R_a|^2 / (2 sigma_a^2)) with per-species
-    (A, sigma) drawn from the seed; orthorhombic minimum image.
-    Returns (V[Nz][Ny][Nx], params)."""
+    (A, sigma) drawn from the seed; orthorhombic minimum image.  The ranges
+    give each molecule four bound levels below a gap of about 0.7 Ha (for
+    8 H2O at 30 Ry: 32 levels spread over 1.8 Ha, then the gap), like the
+    four doubly occupied valence levels of water, so minimisations converge
+    as they do for a real insulator.  Returns (V[Nz][Ny][Nx], params)."""
     rng = np.random.Generator(np.random.PCG64(seed + 104729))
-    params = {"O": (float(rng.uniform(-2.0, -1.0)), float(rng.uniform(0.6, 0.9))),
-              "H": (float(rng.uniform(-0.8, -0.4)), float(rng.uniform(0.4, 0.6)))}
+    params = {"O": (float(rng.uniform(-6.5, -5.5)), float(rng.uniform(0.95, 1.05))),
+              "H": (float(rng.uniform(-1.3, -1.1)), float(rng.uniform(0.55, 0.65)))}
     h = np.asarray(hmat, np.float64).reshape(3, 3)
     Lx, Ly, Lz = h[0, 0], h[1, 1], h[2, 2]
     nx, ny, nz = grid

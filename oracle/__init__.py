@@ -67,16 +67,28 @@ def lib():
         L.oo_step.argtypes = [vp, ctypes.POINTER(StepOpts), _dp, _dp, vp, vp,
                               vp, vp, ctypes.POINTER(StepOut)]
         L.oo_step.restype = c_int
+        L.oo_cg_gamma.argtypes = [ctypes.POINTER(CgState), c_double]
+        L.oo_cg_gamma.restype = c_double
+        L.oo_cg_update.argtypes = [vp, _dp, c_double, c_double, _dp, _dp, _dp]
         L.oo_set_threads.argtypes = [c_int]
         L.oo_get_threads.restype = c_int
         _lib = L
     return _lib
 
 
+class CgState(ctypes.Structure):
+    """oo_cg_state: CG iteration counter, resets, |F|^2 and |F| now and before."""
+    _fields_ = [("iter", ctypes.c_int), ("num_reset", ctypes.c_int),
+                ("fovlap", ctypes.c_double), ("fovlap_old", ctypes.c_double),
+                ("fmag", ctypes.c_double), ("fmag_old", ctypes.c_double)]
+
+
 class StepOpts(ctypes.Structure):
     _fields_ = [("dt", ctypes.c_double), ("tau_fs", ctypes.c_double),
                 ("ecut_mass", ctypes.c_double), ("ns_tol", ctypes.c_double),
-                ("ns_max_iter", ctypes.c_int), ("use_lambda", ctypes.c_int)]
+                ("ns_max_iter", ctypes.c_int), ("use_lambda", ctypes.c_int),
+                ("use_cg", ctypes.c_int), ("cg_state", ctypes.POINTER(CgState)),
+                ("cg", ctypes.c_void_p)]
 
 
 class StepOut(ctypes.Structure):
@@ -233,10 +245,22 @@ class System:
         return lib().oo_eloc(self.h, np.ascontiguousarray(v).reshape(-1),
                              np.ascontiguousarray(rho).reshape(-1))
 
+    def cg_update(self, cmass, dt, gamma, f, cg, psi):
+        """cg = gamma cg + F (|G|^2 <= ecut), psi += dt/m cg, in place."""
+        lib().oo_cg_update(self.h, np.ascontiguousarray(cmass, np.float64), float(dt),
+                           float(gamma), f.view(np.float64).reshape(-1),
+                           cg.view(np.float64).reshape(-1), psi.view(np.float64).reshape(-1))
+
     def step(self, v, psi, dt=5.0, tau_fs=25.0, ecut_mass=2.0, ns_tol=1e-15,
-             ns_max_iter=100, use_lambda=True, want=()):
-        """One minimisation step; psi (complex [nst][npts]) updated in place."""
+             ns_max_iter=100, use_lambda=True, want=(), cg=None):
+        """One minimisation step; psi (complex [nst][npts]) updated in place.
+        cg: None for steepest descent, or a Minimizer from new_cg() whose
+        direction and state the step updates (conjugate gradient)."""
         o = StepOpts(dt, tau_fs, ecut_mass, ns_tol, ns_max_iter, int(use_lambda))
+        if cg is not None:
+            o.use_cg = 1
+            o.cg_state = ctypes.pointer(cg.state)
+            o.cg = cg.direction.ctypes.data_as(ctypes.c_void_p)
         out = StepOut()
         extra = {}
         rho = np.zeros((self.nz, self.ny, self.nx)) if "rho" in want else None
@@ -254,6 +278,18 @@ class System:
                    ns_iters=out.ns_iters, ns_err=out.ns_err, rc=rc)
         res.update(extra)
         return res
+
+
+class CgMinimizer:
+    """Persistent conjugate-gradient direction and state of one minimisation."""
+
+    def __init__(self, nst, npts):
+        self.state = CgState()
+        self.direction = np.zeros((nst, npts), np.complex128)
+
+
+def cg_gamma(state, fovlap):
+    return lib().oo_cg_gamma(ctypes.byref(state), float(fovlap))
 
 
 def ns(S, tol=1e-15, max_iter=100):

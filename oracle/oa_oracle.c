@@ -476,6 +476,40 @@ void oo_sd_update(const oo_sys *s, const double *cmass, double dt, const double 
   }
 }
 
+double oo_cg_gamma(oo_cg_state *st, double fovlap) {
+  /* acceptCgOverlap, CP_State_GSpacePlane.C:131-135 */
+  st->fovlap_old = st->fovlap;
+  st->fmag_old = st->fmag;
+  st->fovlap = fovlap;
+  st->fmag = sqrt(fovlap);
+  st->iter++;
+  /* reset rule, CP_State_GSpacePlane.C:2651-2671 */
+  int reset = 0;
+  if (st->iter == 1) {
+    reset = 1;
+    st->num_reset = 0;
+  } else if ((st->fmag > 1.05 * st->fmag_old && st->num_reset >= 5) ||
+             st->fmag > 2.0 * st->fmag_old) {
+    reset = 1;
+    st->num_reset = 0;
+  }
+  st->num_reset++;
+  return reset ? 0.0 : st->fovlap / st->fovlap_old;
+}
+
+void oo_cg_update(const oo_sys *s, const double *cmass, double dt, double gamma,
+                  const double *f_, double *cg_, double *psi_) {
+  for (int st = 0; st < s->nst; st++) {
+    const cplx *f = (const cplx *)f_ + (size_t)st * s->npts;
+    cplx *cg = (cplx *)cg_ + (size_t)st * s->npts;
+    cplx *psi = (cplx *)psi_ + (size_t)st * s->npts;
+    for (int i = 0; i < s->npts; i++) {
+      if (s->g2[i] <= s->ecut_ry) cg[i] = cg[i] * gamma + f[i];
+      psi[i] += cg[i] * (dt / cmass[i]);
+    }
+  }
+}
+
 void oo_red_sync(const oo_sys *s, double *psi_) {
   for (int st = 0; st < s->nst; st++) {
     cplx *psi = (cplx *)psi_ + (size_t)st * s->npts;
@@ -588,7 +622,12 @@ int oo_step(const oo_sys *s, const oo_step_opts *o, const double *v, double *psi
   double fn = 0.0;
   for (size_t i = 0; i < K * n; i++) fn += f[i] * f[i];
   oo_cmass(s, o->tau_fs, o->ecut_mass, cm);
-  oo_sd_update(s, cm, o->dt, f, psi);
+  if (o->use_cg && o->cg_state && o->cg) {
+    const double gamma = oo_cg_gamma(o->cg_state, fn);
+    oo_cg_update(s, cm, o->dt, gamma, f, o->cg, psi);
+  } else {
+    oo_sd_update(s, cm, o->dt, f, psi);
+  }
   oo_red_sync(s, psi);
   oo_overlap(s, psi, S);
   double err = 0.0;

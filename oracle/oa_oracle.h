@@ -110,6 +110,23 @@ void oo_sd_update(const oo_sys *s, const double *cmass, double dt,
  * (CP_State_GSpacePlane.C:2862-3061) */
 void oo_red_sync(const oo_sys *s, double *psi);
 
+/* Conjugate-gradient minimisation state (CP_State_GSpacePlane.C:120-136,
+ * 2646-2671): iteration counter, resets since the last one, |F|^2 and
+ * |F| of this and the previous step. */
+typedef struct {
+  int iter, num_reset;
+  double fovlap, fovlap_old, fmag, fmag_old;
+} oo_cg_state;
+/* Take the new sum |F|^2 (acceptCgOverlap) and return gamma of the
+ * conjugate direction: 0 on the first iteration or on a reset
+ * (|F| > 1.05 |F|_old after >= 5 steps without reset, or |F| > 2 |F|_old),
+ * else fovlap / fovlap_old. */
+double oo_cg_gamma(oo_cg_state *st, double fovlap);
+/* cg = gamma cg + F where |G|^2 <= ecut; psi += dt/m cg
+ * (CPINTEGRATE::CP_integrate_min_CG, cp_min_CG.C:55-105) */
+void oo_cg_update(const oo_sys *s, const double *cmass, double dt, double gamma,
+                  const double *f, double *cg, double *psi);
+
 /* S = 2 sum_k A_ik A_jk over stored points with kx=0 scaled by 1/sqrt2
  * (sendPsi CP_State_GSpacePlane.C:3118-3125 + symm PC forward alpha=2,
  * ckPairCalculator.C:682-802) */
@@ -135,6 +152,9 @@ typedef struct {
   double ns_tol;      /* invsqr_tolerance                      */
   int ns_max_iter;    /* invsqr_max_iter                       */
   int use_lambda;     /* project forces with Lambda            */
+  int use_cg;         /* conjugate gradient instead of SD      */
+  oo_cg_state *cg_state;  /* CG only: persistent state         */
+  double *cg;         /* CG only: [nst][npts] complex direction */
 } oo_step_opts;
 
 typedef struct {

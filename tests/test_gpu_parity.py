@@ -184,6 +184,34 @@ def test_mixed_precision_errors_stated(gpu, name, mode):
     assert r["resid_max"] < 1e-10
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replay_equals_eager(gpu, precision):
+    """use_graph replays the captured kernel chain (both psi-buffer parities,
+    several replays with host synchronisation between them): every step must
+    equal the eager step bit for bit, and the FP64 chain the oracle."""
+    s = get_system("water8")
+    o = make_oracle(s)
+    psio = s["psi"].copy()
+    with make_ctx(s, precision=precision, dt=5.0) as ce, \
+            make_ctx(s, precision=precision, dt=5.0, use_graph=1) as cg:
+        for c in (ce, cg):
+            c.set_psi(s["psi"])
+            c.set_vks(s["vks"])
+        for k in range(5):
+            re_ = ce.step()
+            rg = cg.step()
+            pe, pg = ce.get_psi(), cg.get_psi()
+            report(f"[water8 {precision} graph] step{k}", dpsi=float(np.abs(pe - pg).max()),
+                   dE=abs(re_["eelec"] - rg["eelec"]))
+            assert np.array_equal(pe, pg)
+            assert re_["eelec"] == rg["eelec"] and re_["fnorm"] == rg["fnorm"]
+            if precision == "fp64":
+                ro = o.step(s["vks"], psio, dt=5.0, ns_tol=1e-15, ns_max_iter=30,
+                            use_lambda=True)
+                assert abs(rg["eelec"] - ro["eelec"]) < TOL * abs(ro["eelec"])
+                assert rel(pg, psio) < TOL
+
+
 def test_enumerated_layout_equals_explicit(gpu):
     """npts = 0 (library enumerates) gives the same k-vectors as passing them."""
     s = get_system("water8")
