@@ -65,6 +65,10 @@ typedef struct {
   double tau_mass;     /* cp_mass_tau_def, fs (default 25) */
   double ecut_mass;    /* cp_mass_cut_def (default 2) */
   int use_lambda;      /* project F with Lambda (cp_min_opt = 1) */
+  int use_cg;          /* conjugate gradient instead of steepest descent (cp_min_cg = 1;
+                          CP_integrate_min_CG, cp_min_CG.C:55-105, with the reset rule of
+                          CP_State_GSpacePlane.C:2646-2671); needs use_lambda.  oa_psi_set
+                          restarts the direction. */
   int want_resid;      /* compute ||Psi^H Psi/2 - I|| after each step */
   int fft_mode;        /* 0 (default): the fastest engine compiled for the grid (two-factor
                           fused planes, else two-factor per pass, else Stockham per pass);

@@ -38,7 +38,8 @@ class OaOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("ns_max_iter", ctypes.c_int),
                 ("ns_tol", ctypes.c_double), ("dt", ctypes.c_double),
                 ("tau_mass", ctypes.c_double), ("ecut_mass", ctypes.c_double),
-                ("use_lambda", ctypes.c_int), ("want_resid", ctypes.c_int),
+                ("use_lambda", ctypes.c_int), ("use_cg", ctypes.c_int),
+                ("want_resid", ctypes.c_int),
                 ("fft_mode", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("ns_mode", ctypes.c_int)]
 

@@ -469,32 +469,46 @@ __global__ __launch_bounds__(256) void k_rotate(RotArgs a) {
   }
 }
 
-// n <= 128: persistent workgroups.  M is staged once per workgroup in LDS
-// (all 128 x 128, zero beyond n, row stride 129); workgroup b walks the
-// 64-column k-blocks b, b + gridDim.x, ...  The work items are (k-block,
-// chunk of 16 i); a ring of kRotRing X buffers keeps the loads of item q+4
-// in flight while the MFMAs of items q+1..q+3 issue, across k-block
-// boundaries.  Wave w owns the output rows j in [32w, 32w + 32).
+// n <= 128: persistent workgroups of 8 waves (two per SIMD).  M is staged
+// once per workgroup in LDS (all 128 x 128, zero beyond n, row stride 129);
+// workgroup b walks the 64-column k-blocks b, b + gridDim.x, ...  Wave w
+// owns the output rows j in [32 (w & 3), +32) and the k tiles 2 (w >> 2),
+// 2 (w >> 2) + 1 of a block.  The work items are (k-block, chunk of 16 i);
+// a ring of kRotRing X buffers keeps the loads of item q+4 in flight while
+// the MFMAs of items q+1..q+3 issue, across k-block boundaries.  The
+// epilogue's operands (Y when beta != 0, psi and dt/m for the SD update)
+// are loaded when a block starts, behind its MFMAs.
 constexpr int kRotRing = 4;
 constexpr int kRotCh = 8;                      // i chunks per k-block (n <= 128)
 constexpr int kRotLd = 129;                    // LDS row stride of M
+constexpr int kRotThreads = 512;
 constexpr size_t kRotSmem = (size_t)128 * kRotLd * sizeof(double);
 
 struct XFrag {
-  double v[4][4];  // [i step][k tile]
+  double v[4][2];  // [i step][k tile]
 };
 
-__global__ __launch_bounds__(256) void k_rotate_p(RotArgs a) {
+__global__ __launch_bounds__(kRotThreads) void k_rotate_p(RotArgs a) {
   extern __shared__ __align__(16) double msm[];
-  __shared__ double red[4];
+  __shared__ double red[kRotThreads / 64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int l16 = lane & 15, g = lane >> 4;
-  for (int e = tid; e < 128 * 128; e += 256) {
-    const int i = e >> 7, j = e & 127;
-    msm[i * kRotLd + j] = (i < a.n && j < a.n) ? a.M[(size_t)i * a.n + j] : 0.0;
+  {  // stage M: every load of a thread is issued before its LDS stores
+    constexpr int PER = 128 * 128 / kRotThreads;
+    double mv[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int e = tid + kRotThreads * u, i = e >> 7, j = e & 127;
+      mv[u] = (i < a.n && j < a.n) ? a.M[(size_t)i * a.n + j] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int e = tid + kRotThreads * u, i = e >> 7, j = e & 127;
+      msm[i * kRotLd + j] = mv[u];
+    }
   }
   __syncthreads();
-  const int jb = wave * 32;
+  const int jb = (wave & 3) * 32, kt0 = 2 * (wave >> 2);
   const bool wact = jb < a.n;
   const int nkb = (a.K + kRK - 1) / kRK;
   const int nmine =
@@ -508,17 +522,34 @@ __global__ __launch_bounds__(256) void k_rotate_p(RotArgs a) {
       const int i = ic * 16 + 4 * s + g;
       const bool vi = i < a.n;
 #pragma unroll
-      for (int y = 0; y < 4; y++) {
-        const int k = kb + 16 * y + l16;
+      for (int y = 0; y < 2; y++) {
+        const int k = kb + 16 * (kt0 + y) + l16;
         d.v[s][y] = (vi && k < a.K) ? a.X[(size_t)i * a.ldx + k] : 0.0;
       }
     }
   };
-  v4d acc[2][4];
+  v4d acc[2][2];
 #pragma unroll
   for (int x = 0; x < 2; x++)
 #pragma unroll
-    for (int y = 0; y < 4; y++) acc[x][y] = vzero();
+    for (int y = 0; y < 2; y++) acc[x][y] = vzero();
+  double yold[2][2][4], pold[2][2][4], dm[2];
+  auto preload = [&](int kb) __attribute__((always_inline)) {
+#pragma unroll
+    for (int y = 0; y < 2; y++) {
+      const int k = kb + 16 * (kt0 + y) + l16;
+      dm[y] = (a.sd_psi && k < a.K) ? a.dtm[k >> 1] : 0.0;
+#pragma unroll
+      for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int j = jb + 16 * x + g + 4 * v;
+          const bool ok = j < a.n && k < a.K;
+          yold[x][y][v] = (ok && a.beta != 0.0) ? a.Y[(size_t)j * a.ldx + k] : 0.0;
+          pold[x][y][v] = (ok && a.sd_psi) ? a.sd_psi[(size_t)j * a.ldx + k] : 0.0;
+        }
+    }
+  };
   double fn = 0.0;
   // output of one k-block: Y = beta Y + s(k) acc, optionally the SD update
   auto epilogue = [&](int kb) __attribute__((always_inline)) {
@@ -526,21 +557,18 @@ __global__ __launch_bounds__(256) void k_rotate_p(RotArgs a) {
 #pragma unroll
     for (int x = 0; x < 2; x++)
 #pragma unroll
-      for (int y = 0; y < 4; y++)
+      for (int y = 0; y < 2; y++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
           const int j = jb + 16 * x + g + 4 * v;
-          const int k = kb + 16 * y + l16;
+          const int k = kb + 16 * (kt0 + y) + l16;
           if (j < a.n && k < a.K) {
             const double s = (k < a.k_lo) ? a.s_lo : a.s_hi;
-            double* yp = a.Y + (size_t)j * a.ldx + k;
             double val = s * acc[x][y][v];
-            if (a.beta != 0.0) val = a.beta * (*yp) + val;
-            *yp = val;
-            if (a.sd_psi) {
-              a.sd_psi[(size_t)j * a.ldx + k] += a.dtm[k >> 1] * val;
-              fn += val * val;
-            }
+            if (a.beta != 0.0) val = a.beta * yold[x][y][v] + val;
+            a.Y[(size_t)j * a.ldx + k] = val;
+            if (a.sd_psi) a.sd_psi[(size_t)j * a.ldx + k] = pold[x][y][v] + dm[y] * val;
+            if (a.fpart) fn += val * val;
           }
           acc[x][y][v] = 0.0;
         }
@@ -550,6 +578,7 @@ __global__ __launch_bounds__(256) void k_rotate_p(RotArgs a) {
     constexpr int d = decltype(dc)::value;
     if (d < nitems) load(d, ring[d]);
   });
+  if (nitems > 0) preload(kblock(0) * kRK);
   for (int q0 = 0; q0 < nitems; q0 += kRotRing) {
     static_for<0, kRotRing>([&](auto dc) __attribute__((always_inline)) {
       constexpr int d = decltype(dc)::value;
@@ -562,22 +591,30 @@ __global__ __launch_bounds__(256) void k_rotate_p(RotArgs a) {
             const double* mrow = msm + (ic * 16 + 4 * s + g) * kRotLd + jb + l16;
             const double am0 = mrow[0], am1 = mrow[16];
 #pragma unroll
-            for (int y = 0; y < 4; y++) {
+            for (int y = 0; y < 2; y++) {
               acc[0][y] = mfma16(am0, ring[d].v[s][y], acc[0][y]);
               acc[1][y] = mfma16(am1, ring[d].v[s][y], acc[1][y]);
             }
           }
         }
         if (q + kRotRing < nitems) load(q + kRotRing, ring[d]);
-        if (ic == kRotCh - 1) epilogue(kblock(q) * kRK);
+        if (ic == kRotCh - 1) {
+          epilogue(kblock(q) * kRK);
+          if (q + 1 < nitems) preload(kblock(q + 1) * kRK);
+        }
       }
     });
   }
-  if (a.sd_psi) {
+  if (a.fpart) {
     for (int o = 32; o > 0; o >>= 1) fn += __shfl_down(fn, o, 64);
     if (lane == 0) red[wave] = fn;
     __syncthreads();
-    if (tid == 0) a.fpart[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+    if (tid == 0) {
+      double t = 0.0;
+#pragma unroll
+      for (int w = 0; w < kRotThreads / 64; w++) t += red[w];
+      a.fpart[blockIdx.x] = t;
+    }
   }
 }
 
@@ -598,8 +635,8 @@ int rotate_nwg(int n, int K) {
 }
 
 hipError_t rotate_f64(const RotArgs& a, hipStream_t st) {
-  if (a.sd_psi && (a.n > 128 || a.sd_psi != a.X || !a.dtm || !a.fpart))
-    return hipErrorInvalidValue;
+  if ((a.sd_psi || a.fpart) && a.n > 128) return hipErrorInvalidValue;
+  if (a.sd_psi && (a.sd_psi != a.X || !a.dtm || !a.fpart)) return hipErrorInvalidValue;
   if (a.n <= 128) {
     static bool granted = false;
     if (!granted) {
@@ -608,7 +645,7 @@ hipError_t rotate_f64(const RotArgs& a, hipStream_t st) {
       if (e != hipSuccess) return e;
       granted = true;
     }
-    hipLaunchKernelGGL(k_rotate_p, dim3(rotate_pgrid(a.K)), dim3(256), kRotSmem, st, a);
+    hipLaunchKernelGGL(k_rotate_p, dim3(rotate_pgrid(a.K)), dim3(kRotThreads), kRotSmem, st, a);
     return hipGetLastError();
   }
   dim3 grid((a.K + kRK - 1) / kRK, (a.n + 127) / 128);
@@ -627,17 +664,34 @@ constexpr int kNsColPanel = kNsKP * kNsCl;
 constexpr size_t kNsSmem = (size_t)2 * (kNsRowPanel + kNsColPanel) * sizeof(double);
 
 // Stage A[r0.., k0..k0+KP) (row panel) and B[k0..k0+KP, c0..) (column panel)
-__device__ __forceinline__ void ns_stage(const double* __restrict__ A,
-                                         const double* __restrict__ B, int n, int r0, int c0,
-                                         int k0, double* As, double* Bs) {
+// All of a thread's loads are issued before any LDS store, so a panel costs
+// one memory latency rather than one per element.
+constexpr int kNsPer = NT * kNsKP / 256;  // elements per thread and panel
+struct NsPanel {
+  double va[kNsPer], vb[kNsPer];
+};
+__device__ __forceinline__ void ns_load(const double* __restrict__ A,
+                                        const double* __restrict__ B, int n, int r0, int c0,
+                                        int k0, NsPanel& p) {
   const int tid = threadIdx.x;
-  for (int e = tid; e < NT * kNsKP; e += 256) {
+#pragma unroll
+  for (int u = 0; u < kNsPer; u++) {
+    const int e = tid + 256 * u;
     const int r = e / kNsKP, k = e - r * kNsKP;
-    As[r * kNsLd + k] = (r0 + r < n && k0 + k < n) ? A[(size_t)(r0 + r) * n + k0 + k] : 0.0;
+    p.va[u] = (r0 + r < n && k0 + k < n) ? A[(size_t)(r0 + r) * n + k0 + k] : 0.0;
+    const int kb = e / NT, cc = e - kb * NT;
+    p.vb[u] = (k0 + kb < n && c0 + cc < n) ? B[(size_t)(k0 + kb) * n + c0 + cc] : 0.0;
   }
-  for (int e = tid; e < kNsKP * NT; e += 256) {
-    const int k = e / NT, cc = e - k * NT;
-    Bs[k * kNsCl + cc] = (k0 + k < n && c0 + cc < n) ? B[(size_t)(k0 + k) * n + c0 + cc] : 0.0;
+}
+__device__ __forceinline__ void ns_store(const NsPanel& p, double* As, double* Bs) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < kNsPer; u++) {
+    const int e = tid + 256 * u;
+    const int r = e / kNsKP, k = e - r * kNsKP;
+    As[r * kNsLd + k] = p.va[u];
+    const int kb = e / NT, cc = e - kb * NT;
+    Bs[kb * kNsCl + cc] = p.vb[u];
   }
 }
 
@@ -668,9 +722,14 @@ __device__ __forceinline__ void ns_tile2(const double* A1, const double* B1, con
   acc1 = vzero();
   acc2 = vzero();
   for (int k0 = 0; k0 < n; k0 += kNsKP) {
+    NsPanel p;
+    ns_load(A1, B1, n, r0, c0, k0, p);
     __syncthreads();  // the previous panels are consumed
-    ns_stage(A1, B1, n, r0, c0, k0, As1, Bs1);
-    if (A2) ns_stage(A2, B2, n, r0, c0, k0, As2, Bs2);
+    ns_store(p, As1, Bs1);
+    if (A2) {
+      ns_load(A2, B2, n, r0, c0, k0, p);
+      ns_store(p, As2, Bs2);
+    }
     __syncthreads();
     acc1 = ns_mma(As1, Bs1, acc1);
     if (A2) acc2 = ns_mma(As2, Bs2, acc2);

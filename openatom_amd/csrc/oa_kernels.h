@@ -141,7 +141,20 @@ struct NsArgs {
 hipError_t ns_run(const NsArgs& a, hipStream_t st);
 bool ns_coop_possible(int n, int device);
 
+// Conjugate-gradient state of a minimisation, on the device
+// (CP_State_GSpacePlane.C:120-136, 2646-2671)
+struct CgDev {
+  double fovlap, fovlap_old, fmag, fmag_old, gamma;
+  int iter, num_reset;
+};
+
 // ---- pointwise G-space kernels (oa_pointwise.hip) ----
+// gamma of the new direction from res->fnorm2 (one thread); the state advances
+hipError_t cg_gamma(const StepDev* res, CgDev* cg, hipStream_t st);
+// cg = gamma cg + F where g2 <= ecut; psi += dtm cg (cp_min_CG.C:55-105)
+hipError_t cg_update(double2* psi, double2* cgdir, const double2* f, const double* dtm,
+                     const double* g2, double ecut, int nst, int npts, int ld, const CgDev* cg,
+                     hipStream_t st);
 // psi += dtm[p] * F (cp_min_std.C:56-86); accumulates |F|^2 partials
 hipError_t sd_update(double2* psi, const double2* f, const double* dtm, int nst, int npts,
                      int ld, double* fpart, int nfpart, hipStream_t st);

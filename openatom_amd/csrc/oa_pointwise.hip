@@ -1,10 +1,15 @@
 // oa_pointwise.hip -- G-space pointwise kernels and small reductions of the
 // CP electronic step.
 //   k_sd       : psi += dt/m F  (CP_integrate_min_STD, cp_min_std.C:56-86)
+//   k_cg_gamma, k_cg_update : conjugate-gradient direction and update
+//                (CP_integrate_min_CG, cp_min_CG.C:55-105; reset rule
+//                 CP_State_GSpacePlane.C:2646-2671)
 //   k_red_sync : redundant kx=0 points <- conj(partner)
 //                (CP_State_GSpacePlane.C:2862-3061)
 //   k_reduce_to, k_dot, k_resid, k_fill_zero: deterministic reductions and
 //   graph-safe zero fill.
+#include <algorithm>
+
 #include "oa_common.h"
 #include "oa_kernels.h"
 
@@ -45,6 +50,72 @@ __global__ void k_sd(double2* __restrict__ psi, const double2* __restrict__ f,
 hipError_t sd_update(double2* psi, const double2* f, const double* dtm, int nst, int npts, int ld,
                      double* fpart, int nfpart, hipStream_t st) {
   hipLaunchKernelGGL(k_sd, dim3(nfpart), dim3(256), 0, st, psi, f, dtm, nst, npts, ld, fpart);
+  return hipGetLastError();
+}
+
+// acceptCgOverlap (CP_State_GSpacePlane.C:131-135) and the reset rule
+// (:2651-2671): gamma = 0 on the first iteration, or when |F| grew by more
+// than 5% after at least five steps without a reset, or more than doubled;
+// else fovlap / fovlap_old.
+__global__ void k_cg_gamma(const StepDev* __restrict__ res, CgDev* __restrict__ cg) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  CgDev s = *cg;
+  s.fovlap_old = s.fovlap;
+  s.fmag_old = s.fmag;
+  s.fovlap = res->fnorm2;
+  s.fmag = sqrt(s.fovlap);
+  s.iter++;
+  int reset = 0;
+  if (s.iter == 1) {
+    reset = 1;
+    s.num_reset = 0;
+  } else if ((s.fmag > 1.05 * s.fmag_old && s.num_reset >= 5) || s.fmag > 2.0 * s.fmag_old) {
+    reset = 1;
+    s.num_reset = 0;
+  }
+  s.num_reset++;
+  s.gamma = reset ? 0.0 : s.fovlap / s.fovlap_old;
+  *cg = s;
+}
+
+hipError_t cg_gamma(const StepDev* res, CgDev* cg, hipStream_t st) {
+  hipLaunchKernelGGL(k_cg_gamma, dim3(1), dim3(64), 0, st, res, cg);
+  return hipGetLastError();
+}
+
+// cg = gamma cg + F where g2 <= ecut; psi += dt/m cg (cp_min_CG.C:55-105)
+__global__ void k_cg_update(double2* __restrict__ psi, double2* __restrict__ cgd,
+                            const double2* __restrict__ f, const double* __restrict__ dtm,
+                            const double* __restrict__ g2, double ecut, int nst, int npts, int ld,
+                            const CgDev* __restrict__ cg) {
+  const double gamma = cg->gamma;
+  const size_t tot = (size_t)nst * npts;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int s = (int)(e / npts), p = (int)(e - (size_t)s * npts);
+    const size_t o = (size_t)s * ld + p;
+    double2 d = cgd[o];
+    if (g2[p] <= ecut) {
+      const double2 fv = f[o];
+      d.x = d.x * gamma + fv.x;
+      d.y = d.y * gamma + fv.y;
+      cgd[o] = d;
+    }
+    const double m = dtm[p];
+    double2 c = psi[o];
+    c.x += d.x * m;
+    c.y += d.y * m;
+    psi[o] = c;
+  }
+}
+
+hipError_t cg_update(double2* psi, double2* cgdir, const double2* f, const double* dtm,
+                     const double* g2, double ecut, int nst, int npts, int ld, const CgDev* cg,
+                     hipStream_t st) {
+  const size_t tot = (size_t)nst * npts;
+  int blocks = (int)std::min<size_t>((tot + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_cg_update, dim3(blocks), dim3(256), 0, st, psi, cgdir, f, dtm, g2, ecut,
+                     nst, npts, ld, cg);
   return hipGetLastError();
 }
 

@@ -168,6 +168,9 @@ struct oa_ctx {
   // graph mode: the two coefficient buffers, and per parity (which buffer
   // holds psi) the chain before Newton-Schulz and the one after it
   double2* psi_buf[2] = {nullptr, nullptr};
+  // conjugate gradient: direction [nst][ld] and device state
+  double2* cgdir = nullptr;
+  oa::CgDev* cgs = nullptr;
   hipGraphExec_t gpre[2] = {nullptr, nullptr}, gpost[2] = {nullptr, nullptr};
   int steps_run = 0;
   int nzblk = 0;      // EKE partials per state of the active z kernel
@@ -459,6 +462,12 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
   TRY(dalloc(c, &c->psi2, gsz));
   c->psi_buf[0] = c->psi;
   c->psi_buf[1] = c->psi2;
+  if (c->opt.use_cg) {
+    TRY(dalloc(c, &c->cgdir, gsz));
+    TRY(dalloc(c, &c->cgs, sizeof(oa::CgDev)));
+    HIPCHK(oa::fill_zero(c->cgdir, gsz, c->stream));
+    HIPCHK(oa::fill_zero(c->cgs, sizeof(oa::CgDev), c->stream));
+  }
   TRY(dalloc(c, &c->force, gsz));
   const size_t zsz = (size_t)c->nst * c->nz * c->nplane_x * c->ny * csz;
   TRY(dalloc(c, &c->zbuf, zsz));
@@ -657,8 +666,9 @@ oa_status st_gram(oa_ctx* c, const double2* A, const double2* B, bool sym, doubl
 }
 
 // sd: also apply psi += dt/m F to X (= psi) in the same pass (n <= 128)
+// fnorm: also leave per-workgroup partials of sum Y^2 in fpart (n <= 128)
 oa_status st_rotate(oa_ctx* c, const double2* X, const double* M, double2* Y, double beta,
-                    double slo, double shi, bool sd = false) {
+                    double slo, double shi, bool sd = false, bool fnorm = false) {
   oa::RotArgs r{};
   r.X = reinterpret_cast<const double*>(X);
   r.M = M;
@@ -673,8 +683,8 @@ oa_status st_rotate(oa_ctx* c, const double2* X, const double* M, double2* Y, do
   if (sd) {
     r.sd_psi = reinterpret_cast<double*>(const_cast<double2*>(X));
     r.dtm = c->dtm;
-    r.fpart = c->fpart;
   }
+  if (sd || fnorm) r.fpart = c->fpart;
   HIPCHK(oa::rotate_f64(r, c->stream));
   return OA_OK;
 }
@@ -695,6 +705,28 @@ bool lambda_sd_fusable(const oa_ctx* c) { return c->nst <= 128; }
 oa_status st_sd_finish(oa_ctx* c) {
   HIPCHK(oa::reduce_to(c->fpart, oa::rotate_nwg(c->nst, 2 * c->npts), 1.0, &c->res->fnorm2,
                        c->stream));
+  HIPCHK(oa::red_sync(c->psi, c->red, c->partner, c->nred, c->nst, c->ld, c->stream));
+  return OA_OK;
+}
+
+// Lambda projection, then the CG direction and update: |F|^2 (fovlap) must
+// be complete before gamma, so the update is a pass of its own
+oa_status st_lambda_cg(oa_ctx* c) {
+  const size_t N2 = (size_t)2 * c->nst * c->ld;  // reals in F, zero padding included
+  TRY(st_gram(c, c->psi, c->force, false, 1.0, 1.0, c->lam));
+  if (c->nst <= 128) {
+    TRY(st_rotate(c, c->psi, c->lam, c->force, 1.0, -0.5, -1.0, false, true));
+    HIPCHK(oa::reduce_to(c->fpart, oa::rotate_nwg(c->nst, 2 * c->npts), 1.0, &c->res->fnorm2,
+                         c->stream));
+  } else {
+    TRY(st_rotate(c, c->psi, c->lam, c->force, 1.0, -0.5, -1.0));
+    const double* fr = reinterpret_cast<const double*>(c->force);
+    HIPCHK(oa::dot_parts(fr, fr, N2, c->fpart, c->nfpart, c->stream));
+    HIPCHK(oa::reduce_to(c->fpart, c->nfpart, 1.0, &c->res->fnorm2, c->stream));
+  }
+  HIPCHK(oa::cg_gamma(c->res, c->cgs, c->stream));
+  HIPCHK(oa::cg_update(c->psi, c->cgdir, c->force, c->dtm, c->g2, c->ecut_ry, c->nst, c->npts,
+                       c->ld, c->cgs, c->stream));
   HIPCHK(oa::red_sync(c->psi, c->red, c->partner, c->nred, c->nst, c->ld, c->stream));
   return OA_OK;
 }
@@ -777,6 +809,7 @@ void oa_default_options(oa_options* o) {
   o->tau_mass = 25.0;
   o->ecut_mass = 2.0;
   o->use_lambda = 1;
+  o->use_cg = 0;
   o->want_resid = 1;
   o->fft_mode = 0;
   o->ns_mode = 0;
@@ -816,6 +849,11 @@ oa_status oa_create(int device, const oa_system* sys, const oa_options* opt, oa_
   c->vol = std::fabs(inv3(c->hmat, c->hmati));
   if (c->nx < 2 || c->ny < 2 || c->nz < 2 || c->nst < 1) {
     c->err = "bad grid or state count";
+    return bail(OA_EINVAL);
+  }
+  if (c->opt.use_cg && !c->opt.use_lambda) {
+    // computeCgOverlap needs the Lambda-corrected forces (CP_State_GSpacePlane.C:2349-2353)
+    c->err = "conjugate gradient needs use_lambda";
     return bail(OA_EINVAL);
   }
   if (sys->npts > 0) {
@@ -918,6 +956,12 @@ static oa_status copy_g(oa_ctx* c, double2* dev, const double* host, bool in) {
 
 oa_status oa_psi_set(oa_ctx* c, const double* psi) {
   if (!c || !psi) return OA_EINVAL;
+  if (c->opt.use_cg) {  // a new minimisation: zero direction, iteration 0
+    HIPCHK(hipSetDevice(c->device));
+    const size_t gsz = (size_t)c->nst * c->ld * sizeof(double2);
+    HIPCHK(oa::fill_zero(c->cgdir, gsz, c->stream));
+    HIPCHK(oa::fill_zero(c->cgs, sizeof(oa::CgDev), c->stream));
+  }
   return copy_g(c, c->psi, psi, true);
 }
 oa_status oa_psi_get(oa_ctx* c, double* psi) {
@@ -1066,7 +1110,10 @@ oa_status enqueue_pre(oa_ctx* c, bool marks) {
   };
   TRY(st_g2r(c, marks ? 1 : -1));  // marks 1..3
   TRY(st_r2g(c, marks ? 1 : -1));  // marks 4..6, then the EKE / E_loc reductions
-  if (c->opt.use_lambda && lambda_sd_fusable(c)) {
+  if (c->opt.use_cg) {
+    TRY(st_lambda_cg(c));
+    mk(7);
+  } else if (c->opt.use_lambda && lambda_sd_fusable(c)) {
     TRY(st_lambda(c, true));  // Lambda projection and SD update in one pass
     mk(7);
     TRY(st_sd_finish(c));

@@ -27,6 +27,10 @@ def _systems():
                                          seed=5),
         "dense128": lambda: small_system(box=(6.0, 6.0, 5.0), ecut=80.0, grid=(128, 128, 30),
                                          nstates=5, seed=9),
+        # more than 128 states: two 128-row gram blocks, the non-persistent
+        # rotate and the separate |F|^2 reduction
+        "states150": lambda: small_system(box=(5.0, 5.2, 5.6), ecut=36.0, grid=(40, 40, 45),
+                                          nstates=150, seed=13),
     }
 
 
@@ -182,6 +186,63 @@ def test_mixed_precision_errors_stated(gpu, name, mode):
     assert dE_atom < 1e-6
     assert rel(rho, ro["rho"]) < 1e-5
     assert r["resid_max"] < 1e-10
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+@pytest.mark.parametrize("name", ["small", "water8", "states150"])
+def test_cg_steps_match_oracle(gpu, name, graph):
+    """Conjugate-gradient minimisation (use_cg) against the oracle's CG over
+    six steps: coefficients and energies at 1e-10, the same |F| sequence."""
+    s = get_system(name)
+    o = make_oracle(s)
+    psio = s["psi"].copy()
+    cgo = oracle.CgMinimizer(s["nstates"], len(s["kx"]))
+    with make_ctx(s, dt=20.0, use_cg=1, use_graph=graph) as c:
+        c.set_psi(s["psi"])
+        c.set_vks(s["vks"])
+        for k in range(6):
+            r = c.step()
+            ro = o.step(s["vks"], psio, dt=20.0, ns_tol=1e-15, ns_max_iter=30, use_lambda=True,
+                        cg=cgo)
+            psi = c.get_psi()
+            report(f"[{name} cg graph{graph}] step{k}", dE=abs(r["eelec"] - ro["eelec"]),
+                   psi=rel(psi, psio), f=abs(r["fnorm"] - ro["fnorm"]) / ro["fnorm"])
+            assert abs(r["eelec"] - ro["eelec"]) < TOL * abs(ro["eelec"])
+            assert abs(r["fnorm"] - ro["fnorm"]) < TOL * ro["fnorm"]
+            assert rel(psi, psio) < TOL
+        # oa_psi_set restarts the minimisation: the first step repeats
+        c.set_psi(s["psi"])
+        r0 = c.step()
+    o2 = make_oracle(s)
+    p2 = s["psi"].copy()
+    ro0 = o2.step(s["vks"], p2, dt=20.0, ns_tol=1e-15, ns_max_iter=30, use_lambda=True,
+                  cg=oracle.CgMinimizer(s["nstates"], len(s["kx"])))
+    assert abs(r0["eelec"] - ro0["eelec"]) < TOL * abs(ro0["eelec"])
+
+
+def test_cg_needs_lambda(gpu):
+    s = get_system("small")
+    from openatom_amd import OaError
+    with pytest.raises(OaError):
+        make_ctx(s, use_cg=1, use_lambda=0)
+
+
+@pytest.mark.parametrize("name", ["states150"])
+def test_many_states_steps_match_oracle(gpu, name):
+    s = get_system(name)
+    o = make_oracle(s)
+    psio = s["psi"].copy()
+    with make_ctx(s, dt=5.0) as c:
+        c.set_psi(s["psi"])
+        c.set_vks(s["vks"])
+        for k in range(2):
+            r = c.step()
+            ro = o.step(s["vks"], psio, dt=5.0, ns_tol=1e-15, ns_max_iter=30, use_lambda=True)
+            report(f"[{name}] step{k}", dE=abs(r["eelec"] - ro["eelec"]),
+                   psi=rel(c.get_psi(), psio), resid=r["resid_max"])
+            assert abs(r["eelec"] - ro["eelec"]) < TOL * abs(ro["eelec"])
+            assert rel(c.get_psi(), psio) < TOL
+            assert r["resid_max"] < 1e-12
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

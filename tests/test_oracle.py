@@ -248,3 +248,62 @@ def test_step_orthonormal_and_descends(small):
     assert all(b < a for a, b in zip(es, es[1:]))
     S = o.overlap(psi)
     assert np.abs(S - 2 * np.eye(s["nstates"])).max() < 1e-12
+
+
+def test_cg_gamma_reset_rule():
+    """oo_cg_gamma restates CP_State_GSpacePlane.C:2651-2671: gamma = 0 on the
+    first iteration and on a reset (|F| > 2|F|_old, or |F| > 1.05|F|_old once
+    five steps ran without a reset), else fovlap/fovlap_old."""
+    st = oracle.CgState()
+    assert oracle.cg_gamma(st, 4.0) == 0.0 and st.iter == 1 and st.num_reset == 1
+    assert oracle.cg_gamma(st, 1.0) == pytest.approx(0.25)       # decreasing |F|
+    assert oracle.cg_gamma(st, 1.0) == pytest.approx(1.0)        # flat: no reset
+    assert oracle.cg_gamma(st, 1.1) == pytest.approx(1.1)        # +5%, only 3 steps since reset
+    assert oracle.cg_gamma(st, 5.0) == 0.0                       # |F| more than doubles
+    assert st.num_reset == 1
+    for _ in range(4):
+        oracle.cg_gamma(st, 5.0)
+    assert st.num_reset == 5
+    assert oracle.cg_gamma(st, 5.0 * 1.2) == 0.0                 # > 1.05 |F|_old after 5 steps
+    st2 = oracle.CgState()
+    oracle.cg_gamma(st2, 1.0)
+    for _ in range(5):
+        oracle.cg_gamma(st2, 1.0)
+    assert oracle.cg_gamma(st2, 1.0 * 1.05 ** 2 * 0.99) == pytest.approx(1.05 ** 2 * 0.99)
+
+
+def test_cg_update_closed_form_and_sd_limit(small):
+    """cg = gamma cg + F (|G|^2 <= ecut), psi += dt/m cg (cp_min_CG.C:55-105);
+    with gamma = 0 and a zero direction it is the SD update."""
+    s, o = small
+    rng = np.random.default_rng(12)
+    cm = o.cmass(25.0, 2.0)
+    F = rng.normal(size=s["psi"].shape) + 1j * rng.normal(size=s["psi"].shape)
+    cg0 = rng.normal(size=s["psi"].shape) + 1j * rng.normal(size=s["psi"].shape)
+    psi, cg = s["psi"].copy(), cg0.copy()
+    o.cg_update(cm, 7.0, 0.3, F, cg, psi)
+    cg_ref = 0.3 * cg0 + F   # every stored point has |G|^2 <= Ecut
+    assert rel(cg, cg_ref) < 1e-15
+    assert rel(psi, s["psi"] + cg_ref * (7.0 / cm)[None, :]) < 1e-14
+    psi_cg, psi_sd = s["psi"].copy(), s["psi"].copy()
+    o.cg_update(cm, 7.0, 0.0, F, np.zeros_like(F), psi_cg)
+    o.sd_update(cm, 7.0, F, psi_sd)
+    assert np.array_equal(psi_cg, psi_sd)
+
+
+def test_cg_minimisation_beats_sd(small):
+    """At the same dt, conjugate gradient lowers E_elec faster than steepest
+    descent and the force norm falls by orders of magnitude."""
+    s, o = small
+    psi_sd, psi_cg = s["psi"].copy(), s["psi"].copy()
+    cg = oracle.CgMinimizer(s["nstates"], len(s["kx"]))
+    f0 = None
+    for _ in range(50):
+        rs = o.step(s["vks"], psi_sd, dt=20.0, ns_tol=1e-14, ns_max_iter=50, use_lambda=True)
+        rc = o.step(s["vks"], psi_cg, dt=20.0, ns_tol=1e-14, ns_max_iter=50, use_lambda=True,
+                    cg=cg)
+        f0 = f0 or rc["fnorm"]
+    assert rc["eelec"] < rs["eelec"]
+    assert rc["fnorm"] < 0.1 * f0 < rs["fnorm"]
+    S = o.overlap(psi_cg)
+    assert np.abs(S - 2 * np.eye(s["nstates"])).max() < 1e-12
