@@ -130,6 +130,71 @@ def roofline(res, nst, traffic=None):
     return r
 
 
+def _serving_system(args):
+    from openatom_amd import system
+    import oracle
+    nmol, ecut, seed = args
+    return system.build(system.serving_workload(nmol, ecut), seed=seed,
+                        kvec_source=oracle.stored_kvecs)
+
+
+def serving_mix(nreq, rate, seed, max_mol, workers=8):
+    """The request mix of the serving workload: water clusters of 4..max_mol
+    molecules (log-uniform), 4 states per molecule, Ecut uniform in
+    {50, 60, 70} Ry, Poisson arrivals at `rate` per second.  Systems are
+    built in worker processes before any GPU work."""
+    import numpy as np
+    from concurrent.futures import ProcessPoolExecutor
+    import multiprocessing as mp
+    rng = np.random.default_rng(seed)
+    nmol = np.floor(np.exp(rng.uniform(np.log(4), np.log(max_mol + 1), nreq))).astype(int)
+    nmol = np.clip(nmol, 4, max_mol)
+    ecut = rng.choice([50.0, 60.0, 70.0], nreq)
+    arrive = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    jobs = [(int(n), float(e), int(seed * 1000 + i)) for i, (n, e) in enumerate(zip(nmol, ecut))]
+    with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("fork")) as ex:
+        systems = list(ex.map(_serving_system, jobs))
+    return systems, arrive
+
+
+def serving_run(systems, arrive, precision, f_tol, max_steps, budget, max_active, opts):
+    """Submit each request at its arrival time, collect results as they
+    finish.  Latency = finish - arrival (queueing included)."""
+    import numpy as np
+    from openatom_amd import serve
+    n = len(systems)
+    res = {}
+    lag = np.zeros(n)
+    with serve.Server(hbm_budget=budget, max_active=max_active) as srv:
+        t0 = time.perf_counter()
+        i = 0
+        while len(res) < n:
+            now = time.perf_counter() - t0
+            while i < n and arrive[i] <= now:
+                srv.submit(systems[i], i, f_tol=f_tol, max_steps=max_steps, precision=precision,
+                           **opts)
+                lag[i] = time.perf_counter() - t0 - arrive[i]
+                i += 1
+            for r in srv.poll():
+                res[r["tag"]] = r
+            time.sleep(5e-4)
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    lat = np.array([res[k]["total_ms"] + lag[k] * 1e3 for k in range(n)])
+    steps = np.array([res[k]["steps"] for k in range(n)])
+    ok = [k for k in range(n) if res[k]["status"] == 0]
+    return dict(res=res, summary={
+        "solves_per_s": round(len(ok) / wall, 3),
+        "p50_ms": round(float(np.percentile(lat, 50)), 2),
+        "p99_ms": round(float(np.percentile(lat, 99)), 2),
+        "max_ms": round(float(lat.max()), 2),
+        "mean_queue_ms": round(float(np.mean([res[k]["queue_ms"] for k in range(n)])), 2),
+        "converged": len(ok), "requests": n,
+        "steps_mean": round(float(steps.mean()), 1), "steps_max": int(steps.max()),
+        "wall_s": round(wall, 3), "peak_active": int(st["peak_active"]),
+        "peak_hbm_gib": round(st["peak_hbm_bytes"] / 2**30, 3)})
+
+
 def cpu_baseline(sysd, nsteps):
     import numpy as np
     import oracle
@@ -163,6 +228,14 @@ def main():
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="before the runs, copy a buffer of this many MiB three times "
                          "(a plain copy kernel that scales the HBM byte counters)")
+    ap.add_argument("--serve-requests", type=int, default=24,
+                    help="requests of the serving sample (0: skip it)")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-max-mol", type=int, default=64)
+    ap.add_argument("--serve-ftol", type=float, default=1e-5)
+    ap.add_argument("--serve-max-steps", type=int, default=1500)
+    ap.add_argument("--serve-active", type=int, default=8, help="requests on the GPU at once")
+    ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the step on a single "
@@ -173,6 +246,11 @@ def main():
     import oracle
     from openatom_amd import system
 
+    mix = None
+    if args.serve_requests > 0:
+        log(f"[bench] building the serving mix: {args.serve_requests} requests")
+        mix = serving_mix(args.serve_requests, args.serve_rate, args.seed, args.serve_max_mol)
+
     if args.copy_calib_mb > 0:
         copy_calibration(args.copy_calib_mb)
     log(f"[bench] building workload {args.workload} (seed {args.seed})")
@@ -180,9 +258,9 @@ def main():
     nst = sysd["nstates"]
     natoms = sysd["natoms"]
 
-    # per precision: an eager run (per-stage HIP-event timings for the
-    # roofline) and, unless --no-graph, a run that replays the captured step
-    # graph (the throughput reported)
+    # per precision: an eager run (the throughput reported, with per-stage
+    # HIP-event timings for the roofline) and, unless --no-graph, a run that
+    # replays the captured step graph, reported beside it
     runs = {}
     for prec in ("fp64", "mixed"):
         for g in ((0, 1) if not args.no_graph else (0,)):
@@ -191,8 +269,9 @@ def main():
             log(f"[bench] {prec} {'graph' if g else 'eager'} {r['ms']:.3f} ms/step")
             runs[(prec, g)] = r
     r64, rmx = runs[("fp64", 0)], runs[("mixed", 0)]
-    r64v = runs.get(("fp64", 1), r64)
-    rmxv = runs.get(("mixed", 1), rmx)
+    r64v, rmxv = r64, rmx
+    graph = {p: round(runs[(p, 1)]["steps_per_s"], 3) for p in ("fp64", "mixed")
+             if (p, 1) in runs}
 
     # quality: same start, same number of steps
     de_atom = abs(rmx["last"]["eelec"] - r64["last"]["eelec"]) / natoms
@@ -207,6 +286,31 @@ def main():
         traffic = tr.get("mixed", {}).get(rl["kernel"])
         rl["traffic"] = traffic
     rl64 = roofline(r64, nst)
+
+    serving = None
+    if mix is not None:
+        systems, arrive = mix
+        sopts = dict(dt=20.0, use_cg=1, ns_mode=1)
+        serving = {"workload": f"{len(systems)} independent minimisations of water clusters "
+                               f"(4-{args.serve_max_mol} H2O log-uniform, 4 states/molecule, "
+                               f"Ecut in {{50,60,70}} Ry), Poisson arrivals at "
+                               f"{args.serve_rate:g}/s, conjugate gradient to |F| <= "
+                               f"{args.serve_ftol:g}",
+                   "nmol": [int(s["nmol"]) for s in systems],
+                   "max_active": args.serve_active, "hbm_budget_gib": args.serve_budget_gib}
+        runs_s = {}
+        for prec in ("fp64", "mixed"):
+            log(f"[bench] serving sample ({prec})")
+            r = serving_run(systems, arrive, prec, args.serve_ftol, args.serve_max_steps,
+                            int(args.serve_budget_gib * 2**30), args.serve_active, sopts)
+            log(f"[bench] serving {prec}: {r['summary']}")
+            serving[prec] = r["summary"]
+            runs_s[prec] = r["res"]
+        de = [abs(runs_s["mixed"][k]["eelec"] - runs_s["fp64"][k]["eelec"]) /
+              (3 * systems[k]["nmol"]) for k in range(len(systems))
+              if runs_s["mixed"][k]["status"] == 0 and runs_s["fp64"][k]["status"] == 0]
+        serving["max_dE_per_atom_mixed_vs_fp64"] = max(de) if de else None
+        del mix, systems
 
     cpu = None
     if not args.no_cpu:
@@ -235,9 +339,8 @@ def main():
                                f"{sysd['ecut_ry']:.0f} Ry, {sysd['grid'][0]}^3 grid, "
                                f"{r64['geo']['npts']} stored plane waves/state",
                    "global_batch": nst, "seq_len": r64["geo"]["npts"], "parallelism": "none (1 GPU)"},
-        "step_launch": "HIP graph replay" if rmxv is not rmx else "eager launches",
-        "eager": {"mixed_steps_per_s": round(rmx["steps_per_s"], 3),
-                  "fp64_steps_per_s": round(r64["steps_per_s"], 3)},
+        "step_launch": "eager launches on one stream",
+        "graph_replay_steps_per_s": graph,
         "fp64": {"steps_per_s": round(r64v["steps_per_s"], 3), "ms_per_step": round(r64v["ms"], 4),
                  "ns_iters": r64["last"]["ns_iters"], "resid_max": r64["last"]["resid_max"],
                  "roofline": rl64, "timings_ms": r64["timings"], "launch": r64["launch"]},
@@ -251,6 +354,7 @@ def main():
         "launch": rmx["launch"],
         "roofline": rl,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

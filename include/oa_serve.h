@@ -1,0 +1,82 @@
+/*
+ * oa_serve.h -- request loop for independent electronic minimisations on one
+ * GPU (openatom_amd/liboa_step.so).
+ *
+ * OpenAtom runs several independent CP instances in one job -- path-integral
+ * beads, k-points, tempering replicas, spins -- each a full set of chares
+ * addressed by an Uber index and scheduled together by the Charm++ runtime
+ * (uber/Uber.h, UberCollection; cpaimd.C instance set-up).  This interface
+ * is the one-GPU counterpart for independent fixed-ion minimisations of
+ * different sizes: a native scheduler thread keeps every admitted request's
+ * wavefunctions resident in HBM, admits queued requests while the HBM budget
+ * and the slot count allow, advances every admitted request one CP step at a
+ * time on its own HIP stream (so their kernels overlap on the device), and
+ * retires a request when |F| <= f_tol (the tol_cp_min exit of
+ * CP_State_GSpacePlane.C:158-191) or after max_steps.  A request's result
+ * does not depend on what else runs: each context owns its buffers and
+ * stream, and every kernel is deterministic.
+ *
+ * Threading: submit/poll/wait/drain may be called from any host thread; the
+ * contexts are owned by the scheduler thread.
+ */
+#ifndef OA_SERVE_H
+#define OA_SERVE_H
+
+#include <stddef.h>
+
+#include "oa_step.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oa_server oa_server;
+
+typedef struct {
+  oa_system sys;        /* the request's cell, grid, cutoff, states (k-vectors copied) */
+  oa_options opt;       /* step options (precision, dt, use_cg, ...) */
+  const double *psi0;   /* [nstates][npts] complex starting coefficients (copied) */
+  const double *vks;    /* [Nz][Ny][Nx] V(r) (copied) */
+  double f_tol;         /* converged when |F| <= f_tol */
+  int max_steps;        /* OA_ENOCONV after this many steps */
+  double *psi_out;      /* optional [nstates][npts] final coefficients; must stay valid
+                           until the request's result is returned */
+  long long tag;        /* caller's id, returned with the result */
+} oa_request;
+
+typedef struct {
+  long long tag;
+  int status;           /* OA_OK converged, OA_ENOCONV step cap reached, else an error */
+  int steps;
+  double eelec, fnorm;  /* of the last step */
+  double queue_ms;      /* submit -> admitted */
+  double run_ms;        /* admitted -> done */
+  double total_ms;      /* submit -> done */
+  size_t hbm_bytes;     /* device memory the request held */
+} oa_result;
+
+/* HBM one context of this system needs (what admission charges) */
+size_t oa_ctx_bytes(const oa_system *sys, const oa_options *opt);
+
+/* hbm_budget_bytes: device memory the server may hold for admitted
+ * requests; max_active: admitted requests at most (streams in flight). */
+oa_status oa_server_create(int device, size_t hbm_budget_bytes, int max_active,
+                           oa_server **out);
+/* Queue a request (inputs copied).  OA_EINVAL if it can never fit the
+ * budget or its layout is invalid. */
+oa_status oa_server_submit(oa_server *s, const oa_request *req);
+/* Move up to max finished results into res; returns how many. */
+int oa_server_poll(oa_server *s, oa_result *res, int max);
+/* Block until every submitted request has finished (results stay queued
+ * for poll). */
+oa_status oa_server_drain(oa_server *s);
+/* [0] admitted now, [1] queued, [2] finished, [3] HBM bytes held,
+ * [4] peak admitted, [5] peak HBM bytes, [6] steps run, [7] submitted */
+oa_status oa_server_stats(oa_server *s, double stats[8]);
+oa_status oa_server_destroy(oa_server *s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OA_SERVE_H */

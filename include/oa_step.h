@@ -30,6 +30,8 @@
 #ifndef OA_STEP_H
 #define OA_STEP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -82,19 +84,42 @@ typedef struct {
                           Newton-Schulz, [9] Psi T (+ residual), [12] whole step */
   int ns_mode;         /* 0: one cooperative launch when all tiles fit (default),
                           1: two launches per iteration with a device-side early exit */
+  int vks_from_rho;    /* 1: each step builds V(r) = V_KS[rho] from its own density
+                          (PZ-LDA exchange-correlation, Hartree, local pseudopotential of
+                          oa_local_pseudo_set) between rho and V psi, as the density chares
+                          do; 0 (default): V(r) is the input of oa_vks_set */
 } oa_options;
 
 typedef struct {
   double eke;          /* kinetic energy (cp_eke.C:128) */
-  double eloc;         /* (Omega/N) sum_r V rho */
-  double eelec;        /* eke + eloc */
+  double eloc;         /* (Omega/N) sum_r V rho; with vks_from_rho the local
+                          pseudopotential energy E_ext (cp_hart_ext.C:289, 323) */
+  double eelec;        /* eke + eloc (+ ehart + exc with vks_from_rho) */
   double fnorm;        /* sqrt(sum |F|^2) after projection */
   int ns_iters;        /* Newton-Schulz iterations used */
   double ns_err;       /* final rms change of T (ortho.C:117) */
   double resid_max;    /* max |(Psi^H Psi)/2 - I| after the step (if asked) */
   double resid_rms;
   double ms;           /* device time of the step, ms */
+  double ehart;        /* vks_from_rho: Hartree energy (cp_hart_ext.C:207, 385), else 0 */
+  double exc;          /* vks_from_rho: PZ-LDA exchange-correlation energy
+                          (cp_pz_lda.C:143-167), else 0 */
 } oa_step_out;
+
+/* Local pseudopotential of fixed ions for a vks_from_rho context: per-species
+ * form factors v_s(|G|) as cubic splines on the uniform grid g_i = gmin + i dg,
+ * in the layout of PINY's vps0..vps3 tables, evaluated as CP_get_vpsnow does
+ * (cp_hart_ext.C:433-475): row i = min(max(floor((g - gmin)/dg) + 1, 1),
+ * nspline) (1-based), h = g - ((i-1) dg + gmin),
+ * v = ((c3 h + c2) h + c1) h + c0; vps_g0 is the G = 0 value (gzvps). */
+typedef struct {
+  int natoms, nspecies, nspline;
+  const double *pos;     /* [natoms][3] Cartesian positions, bohr */
+  const int *species;    /* [natoms] in 0 .. nspecies-1 */
+  double gmin, dg;
+  const double *vps;     /* [nspecies][4][nspline]: rows c0, c1, c2, c3 */
+  const double *vps_g0;  /* [nspecies] */
+} oa_local_pseudo;
 
 /* default options: FP64, ns_tol 1e-15, ns_max_iter 30, dt 5, tau 25 fs,
  * ecut_mass 2, Lambda on, residual on */
@@ -115,6 +140,9 @@ const char *oa_last_error(const oa_ctx *ctx);
 
 /* sizes of the context's layout: npts, nlines, nplane_x, nkx0 */
 oa_status oa_layout_info(const oa_ctx *ctx, int info[4]);
+/* device memory the context holds (the fftw_malloc'd slabs and plan
+ * scratch of GStateSlab/RealStateSlab/FFTcache, stateSlab.C:88-106) */
+size_t oa_ctx_device_bytes(const oa_ctx *ctx);
 /* launch shapes in use: [0] FFT engine (0 two-factor per pass, 1 Stockham
  * per pass, 2 Stockham fused y+x planes, 3 two-factor fused y+x planes),
  * [1] z-lines, [2] y columns, [3] x pair-rows per workgroup (or per round),
@@ -186,6 +214,22 @@ oa_status oa_rotate(oa_ctx *ctx);
  * redundancy sync, S, Newton-Schulz, Psi T.  out may be NULL (no host
  * synchronisation); oa_step_result() fetches it later. */
 oa_status oa_step(oa_ctx *ctx, oa_step_out *out);
+
+/* V_KS from rho on the device (vks_from_rho contexts).
+ * oa_local_pseudo_set computes V_loc(G) = (1/Omega) sum_I v_I(|G|)
+ * exp(-i G.R_I) once for the fixed ions: the structure-factor loop of
+ * CP_hart_eext_calc (cp_hart_ext.C:186-330) with the ion positions of
+ * CP_Rho_GHartExt (CP_Rho_GHartExt.C:461-552).  Without it V_loc = 0.
+ * oa_vks_build turns the context's rho (after oa_fft_g2r) into
+ * V(r) = V_xc(r) + sum_{|G|^2 <= 4 ecut_ry} [4 pi rho(G)/(G^2 Omega) + V_loc(G)]
+ * exp(+i G.r), rho(G) = (Omega/N) sum_r rho(r) exp(-i G.r): the work of
+ * CP_Rho_RealSpacePlane (energyComputation, fftRhoRtoRhoG, acceptHartVks:
+ * CP_Rho_RealSpacePlane.C:739-864, 1365-1390), CP_Rho_GSpacePlane and
+ * CP_exc_calc (cp_pz_lda.C:23-170); its energies land in the next
+ * oa_step_result (ehart, exc, eloc = E_ext).  oa_vks_get copies V(r) out. */
+oa_status oa_local_pseudo_set(oa_ctx *ctx, const oa_local_pseudo *lp);
+oa_status oa_vks_build(oa_ctx *ctx);
+oa_status oa_vks_get(oa_ctx *ctx, double *v);
 /* ||Psi^H Psi / 2 - I|| of the current coefficients (max-abs and rms over
  * the n x n entries): the orthonormality that Ortho's tolerance_check
  * (ortho.C:109-146) controls through T; computed on demand, not per step

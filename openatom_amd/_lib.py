@@ -16,14 +16,18 @@ OA_FP64, OA_MIXED = 0, 1
 STATUS_NAMES = {0: "OA_OK", 1: "OA_EINVAL", 2: "OA_EHIP", 3: "OA_ENOMEM",
                 4: "OA_ENOCONV", 5: "OA_ESTATE"}
 
-# every symbol include/oa_step.h declares
+# every symbol include/oa_step.h and include/oa_serve.h declare
 EXPORTS = (
     "oa_default_options", "oa_kvecs", "oa_create", "oa_destroy", "oa_last_error",
-    "oa_layout_info", "oa_get_config", "oa_get_kvecs", "oa_psi_set", "oa_psi_get", "oa_vks_set",
+    "oa_layout_info", "oa_ctx_device_bytes", "oa_get_config", "oa_get_kvecs", "oa_psi_set",
+    "oa_psi_get", "oa_vks_set",
     "oa_fft_g2r", "oa_psir_get", "oa_density_get", "oa_vpsi_r2g", "oa_force_get",
     "oa_force_set", "oa_lambda", "oa_sd_update", "oa_overlap", "oa_ortho_ns",
     "oa_tmat_set", "oa_rotate", "oa_step", "oa_residual", "oa_sync", "oa_step_result", "oa_timings",
-    "oa_set_timing",
+    "oa_set_timing", "oa_local_pseudo_set", "oa_vks_build", "oa_vks_get",
+    # request loop (oa_serve.h)
+    "oa_ctx_bytes", "oa_server_create", "oa_server_submit", "oa_server_poll", "oa_server_drain",
+    "oa_server_stats", "oa_server_destroy",
 )
 
 
@@ -41,7 +45,7 @@ class OaOptions(ctypes.Structure):
                 ("use_lambda", ctypes.c_int), ("use_cg", ctypes.c_int),
                 ("want_resid", ctypes.c_int),
                 ("fft_mode", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("ns_mode", ctypes.c_int)]
+                ("ns_mode", ctypes.c_int), ("vks_from_rho", ctypes.c_int)]
 
 
 class OaStepOut(ctypes.Structure):
@@ -49,10 +53,17 @@ class OaStepOut(ctypes.Structure):
                 ("eelec", ctypes.c_double), ("fnorm", ctypes.c_double),
                 ("ns_iters", ctypes.c_int), ("ns_err", ctypes.c_double),
                 ("resid_max", ctypes.c_double), ("resid_rms", ctypes.c_double),
-                ("ms", ctypes.c_double)]
+                ("ms", ctypes.c_double), ("ehart", ctypes.c_double), ("exc", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class OaLocalPseudo(ctypes.Structure):
+    _fields_ = [("natoms", ctypes.c_int), ("nspecies", ctypes.c_int), ("nspline", ctypes.c_int),
+                ("pos", ctypes.c_void_p), ("species", ctypes.c_void_p),
+                ("gmin", ctypes.c_double), ("dg", ctypes.c_double), ("vps", ctypes.c_void_p),
+                ("vps_g0", ctypes.c_void_p)]
 
 
 class OaError(RuntimeError):
@@ -95,7 +106,9 @@ def load():
                        ("oa_tmat_set", [vp, vp]), ("oa_rotate", [vp]),
                        ("oa_step", [vp, vp]), ("oa_residual", [vp, vp, vp]), ("oa_sync", [vp]),
                        ("oa_step_result", [vp, vp]), ("oa_timings", [vp, vp, ci]),
-                       ("oa_set_timing", [vp, ci])):
+                       ("oa_set_timing", [vp, ci]),
+                       ("oa_local_pseudo_set", [vp, ctypes.POINTER(OaLocalPseudo)]),
+                       ("oa_vks_build", [vp]), ("oa_vks_get", [vp, vp])):
         f = getattr(L, name)
         f.argtypes = args
         f.restype = ci

@@ -82,7 +82,43 @@ struct StepDev {
   int ns_iter;      // >0 converged at this iteration, <0 not converged
   int pad;
   double resid_max, resid_rms;
+  double ehart, exc, muxc;  // V_KS from rho: Hartree, exchange-correlation, sum V_xc rho dV
 };
+
+// ---- V_KS from rho (oa_vks.hip) ----
+// Dense real 3D transform of rho restricted to the density sphere's box:
+// kx in [0, nkx), ky in [-kyr, kyr] (nky = 2 kyr + 1), all kz.
+struct VksArgs {
+  int nx, ny, nz;
+  int nkx, kyr, nky;
+  int pr, cb, lb;          // pair-rows / kx columns / z-lines per workgroup
+  FftPlan px, py, pz;
+  const double2 *twx, *twy, *twz;  // exp(-2 pi i j / n)
+  double hmati[9];
+  double vol, ecut4;       // Omega; density cutoff 4 ecut_ry on |G|^2
+  const double* rho;       // [nz][ny][nx]
+  double* v;               // [nz][ny][nx] out: V_KS
+  double2* hx;             // [nz][nkx][ny] after the x pass
+  double2* hl;             // [nz][nkx][nky] after the y pass
+  const double2* vloc;     // [nkx][nky][nz] V_loc(G), zero off the sphere
+  double* part_x;          // [nbx][3] exchange-correlation partial sums
+  double* part_l;          // [nbl][2] Hartree and E_ext partial sums
+};
+// workgroup counts of the x and line passes
+int vks_nbx(const VksArgs& a);
+int vks_nbl(const VksArgs& a);
+size_t vks_smem(const VksArgs& a);  // largest dynamic LDS of the passes
+hipError_t vks_build(const VksArgs& a, StepDev* res, hipStream_t st);
+// V_loc(G) of fixed ions on the line box from per-species spline tables
+struct PseudoDev {
+  int natoms, nspecies, nspline;
+  const double* pos;       // [natoms][3]
+  const int* species;      // [natoms]
+  double gmin, dg;
+  const double* vps;       // [nspecies][4][nspline]
+  const double* vps_g0;    // [nspecies]
+};
+hipError_t vloc_build(const VksArgs& a, const PseudoDev& p, double2* vloc, hipStream_t st);
 
 // ---- dense linear algebra (oa_gemm.hip) ----
 // part[sp][i][j] = sum_{k in slice sp} w(k) A[i][k] B[j][k]; A, B are [n][lda]

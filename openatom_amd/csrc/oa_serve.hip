@@ -1,0 +1,305 @@
+// oa_serve.hip -- request loop for independent electronic minimisations on
+// one GPU (include/oa_serve.h).
+//
+// One scheduler thread owns every admitted context.  Each pass of its loop:
+//   1. admits queued requests in arrival order, skipping over those that do
+//      not fit yet, while the slot count and the HBM budget allow (the charge
+//      is the context's exact device footprint, oa_ctx_bytes);
+//   2. enqueues the next CP step of every admitted request that has none in
+//      flight, on that request's own stream, followed by an event;
+//   3. retires finished steps (event complete): reads the step's energies and
+//      |F|, and finishes the request at |F| <= f_tol (tol_cp_min,
+//      CP_State_GSpacePlane.C:158-191), at max_steps, or on an error,
+//      releasing its memory so that queued requests can be admitted.
+// Steps of different requests overlap on the device; no request waits for
+// another one to finish.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <list>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "oa_internal.h"
+#include "oa_serve.h"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+double ms_since(clk::time_point a, clk::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+// A request with its own copies of every input.
+struct Job {
+  oa_request r{};
+  std::vector<int> kx, ky, kz;
+  std::vector<double> occ, psi0, vks;
+  size_t bytes = 0;
+  clk::time_point t_submit;
+};
+
+struct Active {
+  Job job;
+  oa_ctx* ctx = nullptr;
+  hipEvent_t ev = nullptr;
+  bool inflight = false;
+  bool issue_failed = false;
+  int steps = 0;
+  clk::time_point t_admit;
+  oa_step_out last{};
+};
+
+}  // namespace
+
+struct oa_server {
+  int device = 0;
+  size_t budget = 0;
+  int max_active = 1;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Job> queue;
+  std::deque<oa_result> done;
+  size_t used = 0, peak_used = 0;
+  int n_active = 0, peak_active = 0;
+  long long steps_run = 0, submitted = 0, finished = 0;
+  bool stop = false;
+  std::thread th;
+};
+
+namespace {
+
+void finish(oa_server* s, Active* a, oa_status st) {
+  oa_result res{};
+  res.tag = a->job.r.tag;
+  res.status = st;
+  res.steps = a->steps;
+  res.eelec = a->last.eelec;
+  res.fnorm = a->last.fnorm;
+  if (a->ctx && a->job.r.psi_out && (st == OA_OK || st == OA_ENOCONV))
+    (void)oa_psi_get(a->ctx, a->job.r.psi_out);
+  const auto now = clk::now();
+  res.queue_ms = ms_since(a->job.t_submit, a->t_admit);
+  res.run_ms = ms_since(a->t_admit, now);
+  res.total_ms = ms_since(a->job.t_submit, now);
+  res.hbm_bytes = a->job.bytes;
+  if (a->ev) (void)hipEventDestroy(a->ev);
+  if (a->ctx) oa_destroy(a->ctx);
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->used -= a->job.bytes;
+  s->n_active--;
+  s->finished++;
+  s->done.push_back(res);
+  s->cv_done.notify_all();
+}
+
+// create and load one admitted request's context; false: it finished with an error
+bool start(oa_server* s, Active* a) {
+  oa_system sys = a->job.r.sys;
+  sys.kx = a->job.kx.empty() ? nullptr : a->job.kx.data();
+  sys.ky = a->job.ky.empty() ? nullptr : a->job.ky.data();
+  sys.kz = a->job.kz.empty() ? nullptr : a->job.kz.data();
+  sys.occ = a->job.occ.empty() ? nullptr : a->job.occ.data();
+  oa_status st = oa_create(s->device, &sys, &a->job.r.opt, &a->ctx);
+  if (st == OA_OK) st = oa_psi_set(a->ctx, a->job.psi0.data());
+  if (st == OA_OK) st = oa_vks_set(a->ctx, a->job.vks.data());
+  if (st == OA_OK && hipEventCreateWithFlags(&a->ev, hipEventDisableTiming) != hipSuccess)
+    st = OA_EHIP;
+  if (st != OA_OK) {
+    finish(s, a, st);
+    return false;
+  }
+  return true;
+}
+
+void scheduler(oa_server* s) {
+  (void)hipSetDevice(s->device);
+  std::list<Active*> active;
+  for (;;) {
+    // 1. admission (arrival order, skipping requests that do not fit yet)
+    std::vector<Active*> admitted;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      if (active.empty() && s->queue.empty()) {
+        s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
+        if (s->stop && s->queue.empty()) break;
+      }
+      for (auto it = s->queue.begin(); it != s->queue.end() && s->n_active < s->max_active;) {
+        if (s->used + it->bytes <= s->budget) {
+          Active* a = new Active();
+          a->job = std::move(*it);
+          it = s->queue.erase(it);
+          s->used += a->job.bytes;
+          s->n_active++;
+          s->peak_used = std::max(s->peak_used, s->used);
+          s->peak_active = std::max(s->peak_active, s->n_active);
+          admitted.push_back(a);
+        } else {
+          ++it;
+        }
+      }
+    }
+    for (Active* a : admitted) {
+      a->t_admit = clk::now();
+      if (start(s, a))
+        active.push_back(a);
+      else
+        delete a;
+    }
+    // 2. issue the next step of every idle request
+    for (Active* a : active) {
+      if (a->inflight) continue;
+      const oa_status st = oa_step(a->ctx, nullptr);
+      if (st != OA_OK || hipEventRecord(a->ev, oa_ctx_stream(a->ctx)) != hipSuccess) {
+        a->issue_failed = true;  // retired below
+        continue;
+      }
+      a->inflight = true;
+    }
+    // 3. retire finished steps
+    bool progressed = !admitted.empty();
+    for (auto it = active.begin(); it != active.end();) {
+      Active* a = *it;
+      oa_status st = OA_OK;
+      bool end = false;
+      if (a->issue_failed) {  // the step could not be enqueued
+        st = OA_EHIP;
+        end = true;
+      } else if (a->inflight && hipEventQuery(a->ev) == hipSuccess) {
+        a->inflight = false;
+        a->steps++;
+        progressed = true;
+        {
+          std::lock_guard<std::mutex> lk(s->mu);
+          s->steps_run++;
+        }
+        st = oa_step_result(a->ctx, &a->last);
+        if (st != OA_OK) {
+          end = true;
+        } else if (a->last.fnorm <= a->job.r.f_tol) {
+          end = true;
+        } else if (a->steps >= a->job.r.max_steps) {
+          st = OA_ENOCONV;
+          end = true;
+        }
+      }
+      if (end) {
+        finish(s, a, st);
+        delete a;
+        it = active.erase(it);
+        progressed = true;
+      } else {
+        ++it;
+      }
+    }
+    if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+oa_status oa_server_create(int device, size_t hbm_budget_bytes, int max_active,
+                           oa_server** out) {
+  if (!out || max_active < 1 || hbm_budget_bytes == 0) return OA_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return OA_EHIP;
+  oa_server* s = new oa_server();
+  s->device = device;
+  s->budget = hbm_budget_bytes;
+  s->max_active = max_active;
+  s->th = std::thread(scheduler, s);
+  *out = s;
+  return OA_OK;
+}
+
+oa_status oa_server_submit(oa_server* s, const oa_request* req) {
+  if (!s || !req || !req->psi0 || !req->vks || req->max_steps < 1) return OA_EINVAL;
+  Job j;
+  j.r = *req;
+  const oa_system& sys = req->sys;
+  (void)hipSetDevice(s->device);
+  j.bytes = oa_ctx_bytes(&sys, &req->opt);
+  if (j.bytes == 0 || j.bytes > s->budget) return OA_EINVAL;
+  int npts = sys.npts;
+  if (npts > 0) {
+    if (!sys.kx || !sys.ky || !sys.kz) return OA_EINVAL;
+    j.kx.assign(sys.kx, sys.kx + npts);
+    j.ky.assign(sys.ky, sys.ky + npts);
+    j.kz.assign(sys.kz, sys.kz + npts);
+  } else {
+    npts = oa_kvecs(sys.ecut_ry, sys.hmat, nullptr, nullptr, nullptr);
+  }
+  if (sys.occ) j.occ.assign(sys.occ, sys.occ + sys.nstates);
+  j.psi0.assign(req->psi0, req->psi0 + (size_t)2 * sys.nstates * npts);
+  const size_t N = (size_t)sys.grid[0] * sys.grid[1] * sys.grid[2];
+  j.vks.assign(req->vks, req->vks + N);
+  j.t_submit = clk::now();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->queue.push_back(std::move(j));
+    s->submitted++;
+  }
+  s->cv_work.notify_one();
+  return OA_OK;
+}
+
+int oa_server_poll(oa_server* s, oa_result* res, int max) {
+  if (!s || !res || max < 1) return 0;
+  std::lock_guard<std::mutex> lk(s->mu);
+  int n = 0;
+  while (n < max && !s->done.empty()) {
+    res[n++] = s->done.front();
+    s->done.pop_front();
+  }
+  return n;
+}
+
+oa_status oa_server_drain(oa_server* s) {
+  if (!s) return OA_EINVAL;
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->cv_done.wait(lk, [&] { return s->finished == s->submitted; });
+  return OA_OK;
+}
+
+oa_status oa_server_stats(oa_server* s, double st[8]) {
+  if (!s || !st) return OA_EINVAL;
+  std::lock_guard<std::mutex> lk(s->mu);
+  st[0] = s->n_active;
+  st[1] = (double)s->queue.size();
+  st[2] = (double)s->finished;
+  st[3] = (double)s->used;
+  st[4] = s->peak_active;
+  st[5] = (double)s->peak_used;
+  st[6] = (double)s->steps_run;
+  st[7] = (double)s->submitted;
+  return OA_OK;
+}
+
+oa_status oa_server_destroy(oa_server* s) {
+  if (!s) return OA_EINVAL;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  (void)oa_server_drain(s);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->th.joinable()) s->th.join();
+  delete s;
+  return OA_OK;
+}
+
+}  // extern "C"

@@ -11,6 +11,7 @@
 #include <utility>
 #include <vector>
 
+#include "oa_internal.h"
 #include "oa_kernels.h"
 #include "oa_step.h"
 
@@ -168,6 +169,9 @@ struct oa_ctx {
   // graph mode: the two coefficient buffers, and per parity (which buffer
   // holds psi) the chain before Newton-Schulz and the one after it
   double2* psi_buf[2] = {nullptr, nullptr};
+  // sizing: device bytes allocated (256-byte granules); dry = count, allocate nothing
+  size_t bytes = 0;
+  bool dry = false;
   // conjugate gradient: direction [nst][ld] and device state
   double2* cgdir = nullptr;
   oa::CgDev* cgs = nullptr;
@@ -175,6 +179,9 @@ struct oa_ctx {
   int steps_run = 0;
   int nzblk = 0;      // EKE partials per state of the active z kernel
   int4* lines = nullptr;
+  // V_KS from rho (opt.vks_from_rho): the density-sphere box and its buffers
+  oa::VksArgs va{};
+  double2* vloc = nullptr;
 };
 
 namespace {
@@ -202,6 +209,11 @@ template <typename P>
 oa_status dalloc(oa_ctx* c, P** p, size_t bytes) {
   void* q = nullptr;
   if (bytes == 0) bytes = 16;
+  c->bytes += (bytes + 255) & ~(size_t)255;
+  if (c->dry) {  // sizing run: count only
+    *p = nullptr;
+    return OA_OK;
+  }
   if (hipMalloc(&q, bytes) != hipSuccess) {
     (void)hipGetLastError();
     return fail(c, OA_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
@@ -218,6 +230,7 @@ oa_status dalloc(oa_ctx* c, P** p, size_t bytes) {
   } while (0)
 
 oa_status upload(oa_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (c->dry) return OA_OK;
   HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return OA_OK;
@@ -453,6 +466,53 @@ oa_status build_layout(oa_ctx* c) {
   return OA_OK;
 }
 
+// The density sphere |G|^2 <= 4 ecut_ry (cp_hart_ext.C:111, 175) and its box
+// kx in [0, kxr], |ky| <= kyr, |kz| <= kzr; the grid must hold it without
+// aliasing (2 k + 1 <= N on every axis).
+oa_status alloc_vks(oa_ctx* c) {
+  oa::VksArgs& a = c->va;
+  const double ecut4 = 4.0 * c->ecut_ry;
+  int kxr = 0, kyr = 0, kzr = 0;
+  for (int i = 0; i <= c->nx / 2; i++)
+    for (int j = -c->ny / 2; j <= c->ny / 2; j++)
+      for (int k = -c->nz / 2; k <= c->nz / 2; k++)
+        if (gsq(c->hmati, i, j, k) <= ecut4) {
+          kxr = std::max(kxr, i);
+          kyr = std::max(kyr, std::abs(j));
+          kzr = std::max(kzr, std::abs(k));
+        }
+  if (2 * kxr + 1 > c->nx || 2 * kyr + 1 > c->ny || 2 * kzr + 1 > c->nz)
+    return fail(c, OA_EINVAL,
+                "grid too small for the density sphere |G|^2 <= 4 ecut (needs N >= " +
+                    std::to_string(2 * kxr + 1) + ", " + std::to_string(2 * kyr + 1) + ", " +
+                    std::to_string(2 * kzr + 1) + ")");
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.nz = c->nz;
+  a.nkx = kxr + 1;
+  a.kyr = kyr;
+  a.nky = 2 * kyr + 1;
+  a.pr = 8;
+  a.cb = 8;
+  a.lb = 8;
+  a.px = c->fc.px;
+  a.py = c->fc.py;
+  a.pz = c->fc.pz;
+  std::memcpy(a.hmati, c->hmati, sizeof(a.hmati));
+  a.vol = c->vol;
+  a.ecut4 = ecut4;
+  if (oa::vks_smem(a) > 64 * 1024)
+    return fail(c, OA_EINVAL, "V_KS pass needs more than 64 KiB of LDS");
+  const size_t nbox = (size_t)a.nkx * a.nky * c->nz;
+  TRY(dalloc(c, &a.hx, (size_t)c->nz * a.nkx * c->ny * sizeof(double2)));
+  TRY(dalloc(c, &a.hl, nbox * sizeof(double2)));
+  TRY(dalloc(c, &c->vloc, nbox * sizeof(double2)));
+  TRY(dalloc(c, &a.part_x, (size_t)oa::vks_nbx(a) * 3 * sizeof(double)));
+  TRY(dalloc(c, &a.part_l, (size_t)oa::vks_nbl(a) * 2 * sizeof(double)));
+  a.vloc = c->vloc;
+  return OA_OK;
+}
+
 oa_status alloc_state(oa_ctx* c, const double* occ) {
   const size_t N = (size_t)c->nx * c->ny * c->nz;
   const size_t gsz = (size_t)c->nst * c->ld * sizeof(double2);
@@ -465,8 +525,10 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
   if (c->opt.use_cg) {
     TRY(dalloc(c, &c->cgdir, gsz));
     TRY(dalloc(c, &c->cgs, sizeof(oa::CgDev)));
-    HIPCHK(oa::fill_zero(c->cgdir, gsz, c->stream));
-    HIPCHK(oa::fill_zero(c->cgs, sizeof(oa::CgDev), c->stream));
+    if (!c->dry) {
+      HIPCHK(oa::fill_zero(c->cgdir, gsz, c->stream));
+      HIPCHK(oa::fill_zero(c->cgs, sizeof(oa::CgDev), c->stream));
+    }
   }
   TRY(dalloc(c, &c->force, gsz));
   const size_t zsz = (size_t)c->nst * c->nz * c->nplane_x * c->ny * csz;
@@ -518,6 +580,8 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
   TRY(dalloc(c, &c->fpart, c->nfpart * sizeof(double)));
   TRY(dalloc(c, &c->dpart, c->ndpart * sizeof(double)));
   TRY(dalloc(c, &c->res, sizeof(StepDev)));
+  if (c->opt.vks_from_rho) TRY(alloc_vks(c));
+  if (c->dry) return OA_OK;  // sizing only
   HIPCHK(hipHostMalloc((void**)&c->res_host, sizeof(StepDev), hipHostMallocDefault));
   std::memset(c->res_host, 0, sizeof(StepDev));
   HIPCHK(oa::fill_zero(c->psi, gsz, c->stream));
@@ -525,6 +589,9 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
   HIPCHK(oa::fill_zero(c->force, gsz, c->stream));
   HIPCHK(oa::fill_zero(c->vks, N * sizeof(double), c->stream));
   HIPCHK(oa::fill_zero(c->res, sizeof(StepDev), c->stream));
+  if (c->vloc)
+    HIPCHK(oa::fill_zero(c->vloc, (size_t)c->va.nkx * c->va.nky * c->nz * sizeof(double2),
+                         c->stream));
   std::vector<double> o(c->nst, 1.0);
   if (occ)
     for (int i = 0; i < c->nst; i++) o[i] = occ[i];
@@ -572,6 +639,18 @@ void mark(oa_ctx* c, int i) {
 }
 
 // ---- stages (enqueue only) ----
+// V_KS[rho] into vks, and E_H, E_xc, E_ext into the step results
+oa_status st_vks(oa_ctx* c) {
+  oa::VksArgs a = c->va;
+  a.rho = c->rho;
+  a.v = c->vks;
+  a.twx = c->twx;
+  a.twy = c->twy;
+  a.twz = c->twz;
+  HIPCHK(oa::vks_build(a, c->res, c->stream));
+  return OA_OK;
+}
+
 // Two-factor engine: z, y, x c2r (rho partials), then the rho reduction with
 // partial sums of V rho.  Fused Stockham: z, plane forward, rho reduction.
 // Separate Stockham: z, y, x c2r + rho.  Marks ev0+0..2.
@@ -583,6 +662,7 @@ oa_status st_g2r(oa_ctx* c, int ev0 = -1) {
     HIPCHK(oa::plane3_launch(0, c->nx, c->fp32, fctx(c), c->stream));
     mk(1);
     HIPCHK(oa::rho_reduce(fctx(c), c->dpart, c->ndpart, c->stream));
+    if (c->opt.vks_from_rho) TRY(st_vks(c));
     mk(2);
   } else if (c->engine == kEngFft2) {
     HIPCHK(oa::fft2_launch(oa::FFT_ZFWD, c->nz, c->fp32, fctx(c), c->stream));
@@ -591,6 +671,7 @@ oa_status st_g2r(oa_ctx* c, int ev0 = -1) {
     mk(1);
     HIPCHK(oa::fft2_launch(oa::FFT_XC2R, c->nx, c->fp32, fctx(c), c->stream));
     HIPCHK(oa::rho_reduce(fctx(c), c->dpart, c->ndpart, c->stream));
+    if (c->opt.vks_from_rho) TRY(st_vks(c));
     mk(2);
   } else if (c->engine == kEngFused) {
     HIPCHK(oa::fft_stage(fctx(c), c->fp32, oa::FFT_ZFWD, c->stream));
@@ -598,12 +679,15 @@ oa_status st_g2r(oa_ctx* c, int ev0 = -1) {
     HIPCHK(oa::plane_fwd(fctx(c), c->fp32, c->stream));
     mk(1);
     HIPCHK(oa::rho_reduce(fctx(c), c->dpart, c->ndpart, c->stream));
+    if (c->opt.vks_from_rho) TRY(st_vks(c));
     mk(2);
   } else {
     for (int s = oa::FFT_ZFWD; s <= oa::FFT_XC2R; s++) {
       HIPCHK(oa::fft_stage(fctx(c), c->fp32, s, c->stream));
-      mk(s);
+      if (s < oa::FFT_XC2R) mk(s);
     }
+    if (c->opt.vks_from_rho) TRY(st_vks(c));
+    mk(2);
   }
   return OA_OK;
 }
@@ -642,7 +726,8 @@ oa_status st_r2g(oa_ctx* c, int ev0 = -1) {
     HIPCHK(oa::dot_parts(c->vks, c->rho, N, c->dpart, c->ndpart, c->stream));
   }
   HIPCHK(oa::reduce_to(c->eke_part, c->nst * c->nzblk, 1.0, &c->res->eke, c->stream));
-  HIPCHK(oa::reduce_to(c->dpart, c->ndpart, c->vol / (double)N, &c->res->eloc, c->stream));
+  if (!c->opt.vks_from_rho)  // else E_ext came with V_KS
+    HIPCHK(oa::reduce_to(c->dpart, c->ndpart, c->vol / (double)N, &c->res->eloc, c->stream));
   return OA_OK;
 }
 
@@ -781,7 +866,9 @@ oa_status fetch_result(oa_ctx* c, oa_step_out* out) {
     const StepDev& r = *c->res_host;
     out->eke = r.eke;
     out->eloc = r.eloc;
-    out->eelec = r.eke + r.eloc;
+    out->ehart = c->opt.vks_from_rho ? r.ehart : 0.0;
+    out->exc = c->opt.vks_from_rho ? r.exc : 0.0;
+    out->eelec = r.eke + r.eloc + out->ehart + out->exc;
     out->fnorm = std::sqrt(r.fnorm2);
     out->ns_iters = r.ns_iter < 0 ? -r.ns_iter : r.ns_iter;
     out->ns_err = r.ns_err;
@@ -812,7 +899,9 @@ void oa_default_options(oa_options* o) {
   o->use_cg = 0;
   o->want_resid = 1;
   o->fft_mode = 0;
+  o->use_graph = 0;
   o->ns_mode = 0;
+  o->vks_from_rho = 0;
 }
 
 int oa_kvecs(double ecut_ry, const double hmat[9], int* kx, int* ky, int* kz) {
@@ -826,12 +915,18 @@ int oa_kvecs(double ecut_ry, const double hmat[9], int* kx, int* ky, int* kz) {
   return (int)a.size();
 }
 
-oa_status oa_create(int device, const oa_system* sys, const oa_options* opt, oa_ctx** out) {
+}  // extern "C"
+
+// dry: lay the context out and count its device bytes without allocating,
+// uploading or launching anything (oa_ctx_bytes)
+static oa_status create_impl(int device, const oa_system* sys, const oa_options* opt, bool dry,
+                             oa_ctx** out) {
   if (!sys || !out) return OA_EINVAL;
   *out = nullptr;
   oa_ctx* c = new oa_ctx();
+  c->dry = dry;
   auto bail = [&](oa_status s) {
-    std::fprintf(stderr, "oa_create: %s\n", c->err.c_str());
+    if (!dry) std::fprintf(stderr, "oa_create: %s\n", c->err.c_str());
     oa_destroy(c);
     return s;
   };
@@ -873,7 +968,7 @@ oa_status oa_create(int device, const oa_system* sys, const oa_options* opt, oa_
     c->err = "hipSetDevice failed";
     return bail(OA_EHIP);
   }
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (!dry && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     c->err = "stream creation failed";
     return bail(OA_EHIP);
   }
@@ -889,6 +984,30 @@ oa_status oa_create(int device, const oa_system* sys, const oa_options* opt, oa_
   *out = c;
   return OA_OK;
 }
+
+extern "C" {
+
+oa_status oa_create(int device, const oa_system* sys, const oa_options* opt, oa_ctx** out) {
+  return create_impl(device, sys, opt, false, out);
+}
+
+size_t oa_ctx_bytes(const oa_system* sys, const oa_options* opt) {
+  oa_ctx* c = nullptr;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (create_impl(dev, sys, opt, true, &c) != OA_OK || !c) return 0;
+  const size_t b = c->bytes;
+  oa_destroy(c);
+  return b;
+}
+
+size_t oa_ctx_device_bytes(const oa_ctx* c) { return c ? c->bytes : 0; }
+
+}  // extern "C"
+
+hipStream_t oa_ctx_stream(oa_ctx* c) { return c ? c->stream : nullptr; }
+
+extern "C" {
 
 oa_status oa_destroy(oa_ctx* c) {
   if (!c) return OA_OK;
@@ -979,9 +1098,73 @@ oa_status oa_force_set(oa_ctx* c, const double* f) {
 
 oa_status oa_vks_set(oa_ctx* c, const double* v) {
   if (!c || !v) return OA_EINVAL;
+  if (c->opt.vks_from_rho)
+    return fail(c, OA_ESTATE, "this context builds V from rho (vks_from_rho)");
   HIPCHK(hipSetDevice(c->device));
   TRY(upload(c, c->vks, v, (size_t)c->nx * c->ny * c->nz * sizeof(double)));
   c->have_v = true;
+  return OA_OK;
+}
+
+oa_status oa_local_pseudo_set(oa_ctx* c, const oa_local_pseudo* lp) {
+  if (!c || !lp) return OA_EINVAL;
+  if (!c->opt.vks_from_rho)
+    return fail(c, OA_ESTATE, "oa_local_pseudo_set needs a vks_from_rho context");
+  if (lp->natoms < 1 || lp->nspecies < 1 || lp->nspline < 1 || !(lp->dg > 0.0) || !lp->pos ||
+      !lp->species || !lp->vps || !lp->vps_g0)
+    return fail(c, OA_EINVAL, "bad local pseudopotential description");
+  for (int i = 0; i < lp->natoms; i++)
+    if (lp->species[i] < 0 || lp->species[i] >= lp->nspecies)
+      return fail(c, OA_EINVAL, "atom species out of range");
+  // the density sphere's largest |G| must lie on the spline grid
+  if (lp->gmin + (lp->nspline - 1) * lp->dg < std::sqrt(c->va.ecut4))
+    return fail(c, OA_EINVAL, "spline table ends below the density cutoff");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t bpos = sizeof(double) * 3 * lp->natoms, bsp = sizeof(int) * lp->natoms;
+  const size_t bvps = sizeof(double) * 4 * (size_t)lp->nspecies * lp->nspline;
+  const size_t bg0 = sizeof(double) * lp->nspecies;
+  char* tmp = nullptr;
+  const size_t off_sp = (bpos + 255) & ~(size_t)255;
+  const size_t off_vps = off_sp + ((bsp + 255) & ~(size_t)255);
+  const size_t off_g0 = off_vps + ((bvps + 255) & ~(size_t)255);
+  HIPCHK(hipMalloc((void**)&tmp, off_g0 + bg0));
+  oa::PseudoDev p{};
+  p.natoms = lp->natoms;
+  p.nspecies = lp->nspecies;
+  p.nspline = lp->nspline;
+  p.gmin = lp->gmin;
+  p.dg = lp->dg;
+  p.pos = reinterpret_cast<const double*>(tmp);
+  p.species = reinterpret_cast<const int*>(tmp + off_sp);
+  p.vps = reinterpret_cast<const double*>(tmp + off_vps);
+  p.vps_g0 = reinterpret_cast<const double*>(tmp + off_g0);
+  hipError_t e = hipMemcpy(tmp, lp->pos, bpos, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(tmp + off_sp, lp->species, bsp, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(tmp + off_vps, lp->vps, bvps, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(tmp + off_g0, lp->vps_g0, bg0, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = oa::vloc_build(c->va, p, c->vloc, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(tmp);
+  HIPCHK(e);
+  return OA_OK;
+}
+
+oa_status oa_vks_build(oa_ctx* c) {
+  if (!c) return OA_EINVAL;
+  if (!c->opt.vks_from_rho)
+    return fail(c, OA_ESTATE, "oa_vks_build needs a vks_from_rho context");
+  HIPCHK(hipSetDevice(c->device));
+  TRY(st_vks(c));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return OA_OK;
+}
+
+oa_status oa_vks_get(oa_ctx* c, double* v) {
+  if (!c || !v) return OA_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(v, c->vks, (size_t)c->nx * c->ny * c->nz * sizeof(double),
+                   hipMemcpyDeviceToHost));
   return OA_OK;
 }
 
@@ -1165,7 +1348,8 @@ extern "C" {
 
 oa_status oa_step(oa_ctx* c, oa_step_out* out) {
   if (!c) return OA_EINVAL;
-  if (!c->have_v) return fail(c, OA_ESTATE, "oa_step before oa_vks_set");
+  if (!c->have_v && !c->opt.vks_from_rho)
+    return fail(c, OA_ESTATE, "oa_step before oa_vks_set");
   HIPCHK(hipSetDevice(c->device));
   if (c->timing) c->ev_cur = c->ev_count % oa_ctx::kRing;
   // graph mode from the second step on (the first one runs eagerly and sets

@@ -1,0 +1,166 @@
+"""Python side of the request loop (include/oa_serve.h): independent
+fixed-ion electronic minimisations of different sizes on one GPU, admitted
+by HBM budget and advanced concurrently by the library's scheduler thread."""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import OaError, OaOptions, OaSystem, ptr
+
+
+class OaRequest(ctypes.Structure):
+    _fields_ = [("sys", OaSystem), ("opt", OaOptions), ("psi0", ctypes.c_void_p),
+                ("vks", ctypes.c_void_p), ("f_tol", ctypes.c_double),
+                ("max_steps", ctypes.c_int), ("psi_out", ctypes.c_void_p),
+                ("tag", ctypes.c_longlong)]
+
+
+class OaResult(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_longlong), ("status", ctypes.c_int), ("steps", ctypes.c_int),
+                ("eelec", ctypes.c_double), ("fnorm", ctypes.c_double),
+                ("queue_ms", ctypes.c_double), ("run_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double), ("hbm_bytes", ctypes.c_size_t)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _bind(L):
+    if getattr(L, "_serve_bound", False):
+        return L
+    vp = ctypes.c_void_p
+    L.oa_ctx_bytes.argtypes = [ctypes.POINTER(OaSystem), ctypes.POINTER(OaOptions)]
+    L.oa_ctx_bytes.restype = ctypes.c_size_t
+    L.oa_server_create.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_int,
+                                   ctypes.POINTER(vp)]
+    L.oa_server_create.restype = ctypes.c_int
+    L.oa_server_submit.argtypes = [vp, ctypes.POINTER(OaRequest)]
+    L.oa_server_submit.restype = ctypes.c_int
+    L.oa_server_poll.argtypes = [vp, ctypes.POINTER(OaResult), ctypes.c_int]
+    L.oa_server_poll.restype = ctypes.c_int
+    L.oa_server_drain.argtypes = [vp]
+    L.oa_server_drain.restype = ctypes.c_int
+    L.oa_server_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    L.oa_server_stats.restype = ctypes.c_int
+    L.oa_server_destroy.argtypes = [vp]
+    L.oa_server_destroy.restype = ctypes.c_int
+    L._serve_bound = True
+    return L
+
+
+def _system_struct(s, keep):
+    sysd = OaSystem()
+    for i, v in enumerate(np.asarray(s["hmat"], np.float64).reshape(9)):
+        sysd.hmat[i] = float(v)
+    for i, v in enumerate(s["grid"]):
+        sysd.grid[i] = int(v)
+    sysd.ecut_ry = float(s["ecut_ry"])
+    sysd.nstates = int(s["nstates"])
+    k = [np.ascontiguousarray(s[a], np.int32) for a in ("kx", "ky", "kz")]
+    keep.extend(k)
+    sysd.npts = int(k[0].size)
+    sysd.kx, sysd.ky, sysd.kz = (ptr(a) for a in k)
+    sysd.occ = None
+    return sysd
+
+
+def _options(precision="fp64", **opts):
+    o = _lib.default_options()
+    o.precision = _lib.OA_MIXED if precision == "mixed" else _lib.OA_FP64
+    for key, v in opts.items():
+        if not hasattr(o, key):
+            raise TypeError(f"unknown option {key}")
+        setattr(o, key, v)
+    return o
+
+
+def ctx_bytes(system, precision="fp64", **opts):
+    """Device bytes one context of this system would hold."""
+    L = _bind(_lib.load())
+    keep = []
+    sysd = _system_struct(system, keep)
+    o = _options(precision, **opts)
+    return int(L.oa_ctx_bytes(ctypes.byref(sysd), ctypes.byref(o)))
+
+
+class Server:
+    """Request loop on one GPU.  submit() returns immediately; results come
+    back from poll()/drain() in completion order, keyed by tag."""
+
+    def __init__(self, device=0, hbm_budget=32 << 30, max_active=4):
+        self._L = _bind(_lib.load())
+        h = ctypes.c_void_p()
+        st = self._L.oa_server_create(int(device), int(hbm_budget), int(max_active),
+                                      ctypes.byref(h))
+        if st != _lib.OA_OK:
+            raise OaError(st, "oa_server_create failed")
+        self.h = h
+        self._pending = {}  # tag -> arrays kept alive until the result returns
+
+    def submit(self, system, tag, f_tol=1e-5, max_steps=1000, precision="fp64", want_psi=False,
+               **opts):
+        keep = []
+        req = OaRequest()
+        req.sys = _system_struct(system, keep)
+        req.opt = _options(precision, **opts)
+        psi0 = np.ascontiguousarray(system["psi"], np.complex128)
+        vks = np.ascontiguousarray(system["vks"], np.float64)
+        keep += [psi0, vks]
+        req.psi0 = ptr(psi0)
+        req.vks = ptr(vks)
+        req.f_tol = float(f_tol)
+        req.max_steps = int(max_steps)
+        out = np.zeros_like(psi0) if want_psi else None
+        req.psi_out = ptr(out)
+        req.tag = int(tag)
+        st = self._L.oa_server_submit(self.h, ctypes.byref(req))
+        if st != _lib.OA_OK:
+            raise OaError(st, "oa_server_submit failed")
+        self._pending[int(tag)] = (keep, out)
+
+    def poll(self, max_results=64):
+        buf = (OaResult * max_results)()
+        n = self._L.oa_server_poll(self.h, buf, max_results)
+        out = []
+        for i in range(n):
+            d = buf[i].as_dict()
+            _, psi = self._pending.pop(d["tag"], (None, None))
+            d["psi"] = psi
+            out.append(d)
+        return out
+
+    def drain(self):
+        st = self._L.oa_server_drain(self.h)
+        if st != _lib.OA_OK:
+            raise OaError(st, "oa_server_drain failed")
+        res = []
+        while True:
+            r = self.poll()
+            if not r:
+                return res
+            res += r
+
+    def stats(self):
+        a = (ctypes.c_double * 8)()
+        self._L.oa_server_stats(self.h, a)
+        keys = ("active", "queued", "finished", "hbm_bytes", "peak_active", "peak_hbm_bytes",
+                "steps_run", "submitted")
+        return dict(zip(keys, list(a)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.oa_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

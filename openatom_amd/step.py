@@ -128,6 +128,27 @@ class CPContext:
         self._chk(self._L.oa_force_get(self.h, ptr(a)))
         return a
 
+    def set_local_pseudo(self, pos, species, gmin, dg, vps, vps_g0):
+        """Fixed ions of a vks_from_rho context: positions (bohr), species
+        indices and per-species spline tables vps[nspecies][4][nspline]."""
+        keep = [np.ascontiguousarray(pos, np.float64).reshape(-1, 3),
+                np.ascontiguousarray(species, np.int32),
+                np.ascontiguousarray(vps, np.float64),
+                np.ascontiguousarray(vps_g0, np.float64)]
+        lp = _lib.OaLocalPseudo(keep[0].shape[0], keep[2].shape[0], keep[2].shape[2],
+                                ptr(keep[0]), ptr(keep[1]), float(gmin), float(dg),
+                                ptr(keep[2]), ptr(keep[3]))
+        self._chk(self._L.oa_local_pseudo_set(self.h, ctypes.byref(lp)))
+
+    def vks_build(self):
+        """V_KS from the context's current rho (after fft_g2r)."""
+        self._chk(self._L.oa_vks_build(self.h))
+
+    def get_vks(self):
+        a = np.zeros((self.nz, self.ny, self.nx))
+        self._chk(self._L.oa_vks_get(self.h, ptr(a)))
+        return a
+
     def set_force(self, f):
         a = np.ascontiguousarray(f, np.complex128)
         self._chk(self._L.oa_force_set(self.h, ptr(a)))

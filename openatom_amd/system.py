@@ -26,6 +26,25 @@ WORKLOADS = {
     "water128": dict(nmol=128, box_ang=15.65, nstates=512, ecut_ry=70.0, grid=160),
 }
 
+# FFT lengths with compiled two-factor kernels (openatom_amd/csrc/oa_fft2_sizes.h)
+FFT_SIZES = (24, 27, 30, 32, 36, 40, 45, 48, 50, 54, 60, 64, 72, 75, 80, 81, 90, 96, 100, 108,
+             120, 128, 144, 150, 160)
+
+
+def serving_workload(nmol, ecut_ry):
+    """Water box of the request mix: nmol molecules at the density of the
+    32-molecule box (side 9.86 (nmol/32)^(1/3) A), four doubly occupied states
+    per molecule, and the smallest compiled cubic grid that holds the density
+    sphere (N >= 4 kmax + 1, kmax the largest |k| component of the
+    wavefunction sphere)."""
+    box = 9.86 * (nmol / 32.0) ** (1.0 / 3.0)
+    L = box * BOHR_PER_ANG
+    kmax = int(np.floor(np.sqrt(ecut_ry) * L / (2.0 * np.pi)))
+    need = 4 * kmax + 1
+    grid = next((n for n in FFT_SIZES if n >= need), FFT_SIZES[-1])
+    return dict(nmol=int(nmol), box_ang=box, nstates=4 * int(nmol), ecut_ry=float(ecut_ry),
+                grid=grid)
+
 
 def _rand_rotation(rng):
     q = rng.normal(size=4)
@@ -154,6 +173,35 @@ def gaussian_vloc(pos, spec, hmat, grid, seed):
     return V, params
 
 
+def gaussian_form_factor(A, sigma, g):
+    """Fourier transform of A exp(-r^2 / (2 sigma^2)): A (2 pi sigma^2)^(3/2)
+    exp(-sigma^2 g^2 / 2), and its derivative in g."""
+    f = A * (2.0 * np.pi * sigma * sigma) ** 1.5 * np.exp(-0.5 * sigma * sigma * g * g)
+    return f, -sigma * sigma * g * f
+
+
+def local_pseudo_tables(params, ecut_ry, dg=0.002, species_order=("O", "H")):
+    """Spline tables of the Gaussian local potential in the layout of PINY's
+    vps0..vps3 (cubic Hermite on g_i = i dg up to the density cutoff
+    sqrt(4 ecut_ry)): returns dict(gmin, dg, vps[nspecies][4][nspline],
+    vps_g0[nspecies], order)."""
+    gmax = np.sqrt(4.0 * ecut_ry) + 2 * dg
+    nspl = int(np.ceil(gmax / dg)) + 2
+    g = np.arange(nspl + 1) * dg
+    vps = np.zeros((len(species_order), 4, nspl))
+    g0 = np.zeros(len(species_order))
+    for s, name in enumerate(species_order):
+        A, sig = params[name]
+        f, d = gaussian_form_factor(A, sig, g)
+        df = (f[1:] - f[:-1]) / dg
+        vps[s, 0] = f[:-1]
+        vps[s, 1] = d[:-1]
+        vps[s, 2] = (3.0 * df - 2.0 * d[:-1] - d[1:]) / dg
+        vps[s, 3] = (d[:-1] + d[1:] - 2.0 * df) / (dg * dg)
+        g0[s] = f[0]
+    return dict(gmin=0.0, dg=dg, vps=vps, vps_g0=g0, order=tuple(species_order))
+
+
 def build(workload="water32", seed=1234, nstates=None, kvec_source=None):
     """Build one seeded system.  kvec_source(ecut_ry, hmat) -> (kx, ky, kz)
     defaults to the library's own enumeration (oa_kvecs)."""
@@ -167,7 +215,9 @@ def build(workload="water32", seed=1234, nstates=None, kvec_source=None):
     kx, ky, kz = src(w["ecut_ry"], hmat)
     psi = initial_coeffs(kx, ky, kz, hmat, nst, seed)
     V, params = gaussian_vloc(pos, spec, hmat, grid, seed)
+    tab = local_pseudo_tables(params, w["ecut_ry"])
+    species = np.array([tab["order"].index(s) for s in spec], np.int32)
     return dict(name=workload if isinstance(workload, str) else "custom", seed=seed,
                 hmat=hmat, grid=grid, ecut_ry=float(w["ecut_ry"]), nstates=nst,
                 natoms=len(spec), nmol=w["nmol"], pos=pos, spec=spec, kx=kx, ky=ky, kz=kz,
-                psi=psi, vks=V, vparams=params)
+                psi=psi, vks=V, vparams=params, species=species, pseudo=tab)

@@ -70,6 +70,11 @@ def lib():
         L.oo_cg_gamma.argtypes = [ctypes.POINTER(CgState), c_double]
         L.oo_cg_gamma.restype = c_double
         L.oo_cg_update.argtypes = [vp, _dp, c_double, c_double, _dp, _dp, _dp]
+        L.oo_vps_eval.argtypes = [ctypes.POINTER(LocalPseudo), c_int, c_double]
+        L.oo_vps_eval.restype = c_double
+        L.oo_vloc_g.argtypes = [vp, ctypes.POINTER(LocalPseudo), _dp]
+        L.oo_pz_lda.argtypes = [ctypes.c_size_t, _dp, _dp, _dp]
+        L.oo_vks.argtypes = [vp, _dp, vp, _dp, ctypes.POINTER(VksEnergies)]
         L.oo_set_threads.argtypes = [c_int]
         L.oo_get_threads.restype = c_int
         _lib = L
@@ -88,13 +93,56 @@ class StepOpts(ctypes.Structure):
                 ("ecut_mass", ctypes.c_double), ("ns_tol", ctypes.c_double),
                 ("ns_max_iter", ctypes.c_int), ("use_lambda", ctypes.c_int),
                 ("use_cg", ctypes.c_int), ("cg_state", ctypes.POINTER(CgState)),
-                ("cg", ctypes.c_void_p)]
+                ("cg", ctypes.c_void_p), ("vks_from_rho", ctypes.c_int),
+                ("vloc", ctypes.c_void_p), ("vks_out", ctypes.c_void_p)]
 
 
 class StepOut(ctypes.Structure):
     _fields_ = [("eke", ctypes.c_double), ("eloc", ctypes.c_double),
                 ("eelec", ctypes.c_double), ("fnorm", ctypes.c_double),
-                ("ns_iters", ctypes.c_int), ("ns_err", ctypes.c_double)]
+                ("ns_iters", ctypes.c_int), ("ns_err", ctypes.c_double),
+                ("ehart", ctypes.c_double), ("exc", ctypes.c_double)]
+
+
+class LocalPseudo(ctypes.Structure):
+    """oo_local_pseudo: atoms, species and the spline tables of the local
+    pseudopotential form factors."""
+    _fields_ = [("natoms", ctypes.c_int), ("nspecies", ctypes.c_int),
+                ("nspline", ctypes.c_int), ("pos", ctypes.c_void_p),
+                ("species", ctypes.c_void_p), ("gmin", ctypes.c_double),
+                ("dg", ctypes.c_double), ("vps", ctypes.c_void_p),
+                ("vps_g0", ctypes.c_void_p)]
+
+
+class VksEnergies(ctypes.Structure):
+    _fields_ = [("ehart", ctypes.c_double), ("exc", ctypes.c_double),
+                ("eext", ctypes.c_double), ("muxc", ctypes.c_double)]
+
+
+def local_pseudo(pos, species, gmin, dg, vps, vps_g0):
+    """LocalPseudo over numpy arrays; returns (struct, arrays kept alive).
+    vps: [nspecies][4][nspline] spline coefficients c0..c3."""
+    keep = [np.ascontiguousarray(pos, np.float64).reshape(-1, 3),
+            np.ascontiguousarray(species, np.int32),
+            np.ascontiguousarray(vps, np.float64),
+            np.ascontiguousarray(vps_g0, np.float64)]
+    lp = LocalPseudo(keep[0].shape[0], keep[2].shape[0], keep[2].shape[2],
+                     _ptr(keep[0]), _ptr(keep[1]), float(gmin), float(dg),
+                     _ptr(keep[2]), _ptr(keep[3]))
+    return lp, keep
+
+
+def pz_lda(rho):
+    """(vxc, sums) of CP_exc_calc on a flat density array."""
+    r = np.ascontiguousarray(rho, np.float64).reshape(-1)
+    v = np.zeros_like(r)
+    sums = np.zeros(3)
+    lib().oo_pz_lda(r.size, r, v, sums)
+    return v.reshape(np.shape(rho)), sums
+
+
+def vps_eval(lp, species, g):
+    return lib().oo_vps_eval(ctypes.byref(lp), int(species), float(g))
 
 
 def _ptr(a):
@@ -245,6 +293,22 @@ class System:
         return lib().oo_eloc(self.h, np.ascontiguousarray(v).reshape(-1),
                              np.ascontiguousarray(rho).reshape(-1))
 
+    def vloc_g(self, lp):
+        """V_loc(G) on the half grid [Nz][Ny][Nx/2+1] (complex)."""
+        out = np.zeros((self.nz, self.ny, self.nx // 2 + 1), np.complex128)
+        lib().oo_vloc_g(self.h, ctypes.byref(lp), out.view(np.float64).reshape(-1))
+        return out
+
+    def vks(self, rho, vloc=None):
+        """(V_KS(r), energies dict) from rho(r) and V_loc(G) (or None)."""
+        r = np.ascontiguousarray(rho, np.float64).reshape(-1)
+        v = np.zeros_like(r)
+        e = VksEnergies()
+        vl = None if vloc is None else np.ascontiguousarray(vloc, np.complex128)
+        lib().oo_vks(self.h, r, _ptr(None if vl is None else vl.view(np.float64)), v,
+                     ctypes.byref(e))
+        return v.reshape(self.nz, self.ny, self.nx), {k: getattr(e, k) for k, _ in e._fields_}
+
     def cg_update(self, cmass, dt, gamma, f, cg, psi):
         """cg = gamma cg + F (|G|^2 <= ecut), psi += dt/m cg, in place."""
         lib().oo_cg_update(self.h, np.ascontiguousarray(cmass, np.float64), float(dt),
@@ -252,11 +316,22 @@ class System:
                            cg.view(np.float64).reshape(-1), psi.view(np.float64).reshape(-1))
 
     def step(self, v, psi, dt=5.0, tau_fs=25.0, ecut_mass=2.0, ns_tol=1e-15,
-             ns_max_iter=100, use_lambda=True, want=(), cg=None):
+             ns_max_iter=100, use_lambda=True, want=(), cg=None, vloc=None):
         """One minimisation step; psi (complex [nst][npts]) updated in place.
         cg: None for steepest descent, or a Minimizer from new_cg() whose
-        direction and state the step updates (conjugate gradient)."""
+        direction and state the step updates (conjugate gradient).
+        vloc: None for a fixed V(r) = v, or the half-grid V_loc(G) of
+        vloc_g(): V_KS is then built from this step's rho (v is ignored and
+        may be None; want "v" returns the V_KS used)."""
         o = StepOpts(dt, tau_fs, ecut_mass, ns_tol, ns_max_iter, int(use_lambda))
+        vk = None
+        if vloc is not None:
+            vloc = np.ascontiguousarray(vloc, np.complex128)
+            vk = np.zeros((self.nz, self.ny, self.nx))
+            o.vks_from_rho = 1
+            o.vloc = vloc.ctypes.data_as(ctypes.c_void_p)
+            o.vks_out = vk.ctypes.data_as(ctypes.c_void_p)
+            v = vk
         if cg is not None:
             o.use_cg = 1
             o.cg_state = ctypes.pointer(cg.state)
@@ -271,11 +346,12 @@ class System:
                            psi.view(np.float64).reshape(-1), _ptr(rho), _ptr(S),
                            _ptr(T), _ptr(None if f is None else f.view(np.float64)),
                            ctypes.byref(out))
-        for k, a in (("rho", rho), ("S", S), ("T", T), ("f", f)):
+        for k, a in (("rho", rho), ("S", S), ("T", T), ("f", f), ("v", vk)):
             if a is not None:
                 extra[k] = a
         res = dict(eke=out.eke, eloc=out.eloc, eelec=out.eelec, fnorm=out.fnorm,
-                   ns_iters=out.ns_iters, ns_err=out.ns_err, rc=rc)
+                   ns_iters=out.ns_iters, ns_err=out.ns_err, ehart=out.ehart, exc=out.exc,
+                   rc=rc)
         res.update(extra)
         return res
 

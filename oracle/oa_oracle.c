@@ -598,6 +598,179 @@ double oo_eloc(const oo_sys *s, const double *v, const double *rho) {
   return acc * s->vol / (double)N;
 }
 
+/* ---------------- V_KS from rho (density chares) ---------------- */
+
+/* cp_hart_ext.C:433-475: row i = clamp(floor((g-gmin)/dg)+1, 1, nspl)
+ * (1-based), h = g - ((i-1) dg + gmin), Horner in h */
+double oo_vps_eval(const oo_local_pseudo *lp, int sp, double g) {
+  int ns = lp->nspline;
+  int i = (int)((g - lp->gmin) / lp->dg + 1.0);
+  if (i > ns) i = ns;
+  if (i < 1) i = 1;
+  double h = g - ((double)(i - 1) * lp->dg + lp->gmin);
+  const double *c = lp->vps + (size_t)sp * 4 * ns;
+  int k = i - 1;
+  return ((c[3 * ns + k] * h + c[2 * ns + k]) * h + c[ns + k]) * h + c[k];
+}
+
+/* signed frequency of grid index i */
+static int sfreq(int i, int n) { return i <= (n - 1) / 2 ? i : i - n; }
+
+void oo_vloc_g(const oo_sys *s, const oo_local_pseudo *lp, double *vloc_) {
+  cplx *vloc = (cplx *)vloc_;
+  int Nx = s->nx, Ny = s->ny, Nz = s->nz, nh = Nx / 2 + 1;
+  double ecut4 = 4.0 * s->ecut_ry, tpi = 2.0 * M_PI;
+  const double *hi = s->hmati;
+#pragma omp parallel for num_threads(oo_get_threads()) schedule(static)
+  for (int z = 0; z < Nz; z++)
+    for (int y = 0; y < Ny; y++)
+      for (int x = 0; x < nh; x++) {
+        int a = x, b = sfreq(y, Ny), c = sfreq(z, Nz);
+        double gx = tpi * (a * hi[0] + b * hi[1] + c * hi[2]);
+        double gy = tpi * (a * hi[3] + b * hi[4] + c * hi[5]);
+        double gz = tpi * (a * hi[6] + b * hi[7] + c * hi[8]);
+        double g2 = gx * gx + gy * gy + gz * gz;
+        cplx v = 0;
+        if (g2 == 0.0) {
+          for (int ia = 0; ia < lp->natoms; ia++) v += lp->vps_g0[lp->species[ia]];
+        } else if (g2 <= ecut4) {
+          double g = sqrt(g2);
+          for (int ia = 0; ia < lp->natoms; ia++) {
+            const double *R = lp->pos + 3 * ia;
+            double arg = gx * R[0] + gy * R[1] + gz * R[2];
+            v += oo_vps_eval(lp, lp->species[ia], g) * (cos(arg) - sin(arg) * I);
+          }
+        }
+        vloc[((size_t)z * Ny + y) * nh + x] = v / s->vol;
+      }
+}
+
+/* cp_pz_lda.C:101-153, per point */
+void oo_pz_lda(size_t n, const double *rho, double *vxc, double sums[3]) {
+  const double gamma = -0.14230, beta1 = 1.05290, beta2 = 0.33340;
+  const double au = 0.03110, bu = -0.0480, cu = 0.00200, du = -0.01160;
+  const double power = 1.0 / 3.0, fpi = 4.0 * M_PI;
+  const double rat1 = (7.0 / 6.0) * beta1, rat2 = (4.0 / 3.0) * beta2;
+  double ex = 0.0, ec = 0.0, vx = 0.0;
+  for (size_t i = 0; i < n; i++) {
+    double r = rho[i];
+    double rs = pow(3.0 / (fpi * r), power);
+    double sq = sqrt(rs);
+    double xf = -pow(3.0 * r / M_PI, power);
+    double cf, mf;
+    if (rs >= 1.0) {
+      double c1 = 1.0 + beta1 * sq + beta2 * rs;
+      double c2 = 1.0 + rat1 * sq + rat2 * rs;
+      cf = gamma / c1;
+      mf = cf * (c2 / c1);
+    } else {
+      double l = log(rs);
+      cf = au * l + bu + cu * rs * l + du * rs;
+      mf = au * l + (bu - power * au) + 2.0 * power * cu * rs * l + power * (2.0 * du - cu) * rs;
+    }
+    ex += xf * r;
+    ec += cf * r;
+    vxc[i] = xf + mf;
+    vx += (xf + mf) * r;
+  }
+  sums[0] = ex; sums[1] = ec; sums[2] = vx;
+}
+
+void oo_vks(const oo_sys *s, const double *rho, const double *vloc_, double *v,
+            oo_vks_energies *e) {
+  const cplx *vloc = (const cplx *)vloc_;
+  int Nx = s->nx, Ny = s->ny, Nz = s->nz, nh = Nx / 2 + 1;
+  size_t N = (size_t)Nx * Ny * Nz, NH = (size_t)Nz * Ny * nh;
+  double sums[3];
+  oo_pz_lda(N, rho, v, sums);  /* v = V_xc */
+  cplx *h = malloc(sizeof(cplx) * NH);
+  int nmax = Nx > Ny ? Nx : Ny; if (Nz > nmax) nmax = Nz;
+  const double sc = s->vol / (double)N;
+  /* rho_G: x (complex FFT of the real row), y, z; FFTW_FORWARD sign */
+#pragma omp parallel num_threads(oo_get_threads())
+  {
+    cplx *row = malloc(sizeof(cplx) * nmax), *w = malloc(sizeof(cplx) * 3 * nmax);
+#pragma omp for schedule(static)
+    for (int zy = 0; zy < Nz * Ny; zy++) {
+      for (int x = 0; x < Nx; x++) row[x] = rho[(size_t)zy * Nx + x] * sc;
+      fft_exec(&s->px, -1, row, w);
+      for (int x = 0; x < nh; x++) h[(size_t)zy * nh + x] = row[x];
+    }
+#pragma omp for schedule(static)
+    for (int zx = 0; zx < Nz * nh; zx++) {
+      int z = zx / nh, x = zx % nh;
+      for (int y = 0; y < Ny; y++) row[y] = h[((size_t)z * Ny + y) * nh + x];
+      fft_exec(&s->py, -1, row, w);
+      for (int y = 0; y < Ny; y++) h[((size_t)z * Ny + y) * nh + x] = row[y];
+    }
+#pragma omp for schedule(static)
+    for (int yx = 0; yx < Ny * nh; yx++) {
+      for (int z = 0; z < Nz; z++) row[z] = h[(size_t)z * Ny * nh + yx];
+      fft_exec(&s->pz, -1, row, w);
+      for (int z = 0; z < Nz; z++) h[(size_t)z * Ny * nh + yx] = row[z];
+    }
+    free(row); free(w);
+  }
+  /* Hartree + local pseudo in G (cp_hart_ext.C:186-330) */
+  double ehart = 0.0, eext = 0.0, ecut4 = 4.0 * s->ecut_ry, fpi = 4.0 * M_PI;
+  for (int z = 0; z < Nz; z++)
+    for (int y = 0; y < Ny; y++)
+      for (int x = 0; x < nh; x++) {
+        size_t i = ((size_t)z * Ny + y) * nh + x;
+        double g2 = full_g2(s->hmati, x, sfreq(y, Ny), sfreq(z, Nz));
+        cplx r = h[i], vl = vloc ? vloc[i] : 0;
+        double w = x == 0 ? 1.0 : 2.0;
+        if (g2 == 0.0) {
+          h[i] = creal(vl);
+          eext += creal(vl) * creal(r);
+        } else if (g2 <= ecut4) {
+          double hf = fpi / (g2 * s->vol);
+          ehart += hf * (creal(r) * creal(r) + cimag(r) * cimag(r)) * w;
+          eext += creal(r * conj(vl)) * w;
+          h[i] = r * hf + vl;
+        } else {
+          h[i] = 0;
+        }
+      }
+  /* back: z, y (FFTW_BACKWARD), Hermitian x c2r (Im of kx = 0 and of the
+   * Nyquist column dropped), added to V_xc */
+#pragma omp parallel num_threads(oo_get_threads())
+  {
+    cplx *row = malloc(sizeof(cplx) * nmax), *w = malloc(sizeof(cplx) * 3 * nmax);
+#pragma omp for schedule(static)
+    for (int yx = 0; yx < Ny * nh; yx++) {
+      for (int z = 0; z < Nz; z++) row[z] = h[(size_t)z * Ny * nh + yx];
+      fft_exec(&s->pz, +1, row, w);
+      for (int z = 0; z < Nz; z++) h[(size_t)z * Ny * nh + yx] = row[z];
+    }
+#pragma omp for schedule(static)
+    for (int zx = 0; zx < Nz * nh; zx++) {
+      int z = zx / nh, x = zx % nh;
+      for (int y = 0; y < Ny; y++) row[y] = h[((size_t)z * Ny + y) * nh + x];
+      fft_exec(&s->py, +1, row, w);
+      for (int y = 0; y < Ny; y++) h[((size_t)z * Ny + y) * nh + x] = row[y];
+    }
+#pragma omp for schedule(static)
+    for (int zy = 0; zy < Nz * Ny; zy++) {
+      const cplx *hr = h + (size_t)zy * nh;
+      row[0] = creal(hr[0]);
+      for (int x = 1; x < nh; x++) row[x] = hr[x];
+      if (Nx % 2 == 0) row[Nx / 2] = creal(hr[Nx / 2]);
+      for (int x = nh; x < Nx; x++) row[x] = conj(hr[Nx - x]);
+      fft_exec(&s->px, +1, row, w);
+      for (int x = 0; x < Nx; x++) v[(size_t)zy * Nx + x] += creal(row[x]);
+    }
+    free(row); free(w);
+  }
+  free(h);
+  if (e) {
+    e->ehart = 0.5 * ehart;
+    e->eext = eext;
+    e->exc = (0.75 * sums[0] + sums[1]) * sc;
+    e->muxc = sums[2] * sc;
+  }
+}
+
 int oo_step(const oo_sys *s, const oo_step_opts *o, const double *v, double *psi,
             double *rho_o, double *S_o, double *T_o, double *f_o, oo_step_out *out) {
   size_t N = (size_t)s->nx * s->ny * s->nz;
@@ -610,9 +783,17 @@ int oo_step(const oo_sys *s, const oo_step_opts *o, const double *v, double *psi
   double *cm = malloc(sizeof(double) * s->npts);
   oo_g2r_all(s, psi, psir);
   oo_density(s, psir, rho);
+  oo_vks_energies ve = {0, 0, 0, 0};
+  double *vk = NULL;
+  if (o->vks_from_rho) {
+    vk = o->vks_out ? o->vks_out : malloc(sizeof(double) * N);
+    oo_vks(s, rho, o->vloc, vk, &ve);
+    v = vk;
+  }
   oo_vpsi_r2g_all(s, psir, v, f);
   double eke = oo_eke_force(s, psi, f);
-  double eloc = oo_eloc(s, v, rho);
+  double eloc = o->vks_from_rho ? ve.eext : oo_eloc(s, v, rho);
+  if (vk && !o->vks_out) free(vk);
   if (o->use_lambda) {
     double *lam = malloc(sizeof(double) * n * n);
     oo_lambda(s, psi, f, lam);
@@ -637,7 +818,8 @@ int oo_step(const oo_sys *s, const oo_step_opts *o, const double *v, double *psi
   memcpy(psi, tmp, sizeof(double) * K * n);
   free(tmp);
   if (out) {
-    out->eke = eke; out->eloc = eloc; out->eelec = eke + eloc;
+    out->eke = eke; out->eloc = eloc; out->eelec = eke + eloc + ve.ehart + ve.exc;
+    out->ehart = ve.ehart; out->exc = ve.exc;
     out->fnorm = sqrt(fn); out->ns_iters = it < 0 ? -it : it; out->ns_err = err;
   }
   free(psir); free(cm);

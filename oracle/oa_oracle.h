@@ -22,6 +22,8 @@
 #ifndef OA_ORACLE_H
 #define OA_ORACLE_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -145,6 +147,47 @@ void oo_rotate(const oo_sys *s, const double *psi, const double *T,
 /* E_loc = (Omega/N) sum_r V rho */
 double oo_eloc(const oo_sys *s, const double *v, const double *rho);
 
+/* ---- V_KS(r) from rho(r): the density chares of the reference ----
+ * Half-grid G arrays are [Nz][Ny][Nx/2+1] complex (kz, ky stored modulo
+ * the grid, kx >= 0); the density sphere is |G|^2 <= 4 ecut_ry
+ * (cp_hart_ext.C:175, ecut4 = 8 ecut_Ha).                              */
+
+/* Local pseudopotential: per-species cubic splines of the form factor on a
+ * uniform |G| grid (PINY vps0..vps3 tables) and the G=0 values (gzvps). */
+typedef struct {
+  int natoms, nspecies, nspline;
+  const double *pos;     /* [natoms][3] Cartesian, bohr            */
+  const int *species;    /* [natoms], 0 .. nspecies-1              */
+  double gmin, dg;       /* spline grid: g_i = gmin + i dg         */
+  const double *vps;     /* [nspecies][4][nspline]: c0, c1, c2, c3 */
+  const double *vps_g0;  /* [nspecies]                             */
+} oo_local_pseudo;
+
+/* Spline value of one species at |G| = g (CP_get_vpsnow,
+ * cp_hart_ext.C:433-475), without the 1/Omega. */
+double oo_vps_eval(const oo_local_pseudo *lp, int species, double g);
+
+/* V_loc(G) = (1/Omega) sum_I v_I(|G|) exp(-i G.R_I) on the density sphere,
+ * v(0) = vps_g0 / Omega at G = 0, zero elsewhere: the conjugated structure
+ * factor sum of CP_hart_eext_calc (cp_hart_ext.C:186-330). */
+void oo_vloc_g(const oo_sys *s, const oo_local_pseudo *lp, double *vloc /* half grid */);
+
+/* PZ-LDA (CP_exc_calc, cp_pz_lda.C:23-170): vxc[i] = V_xc(rho[i]) and the
+ * three sums of section V-VI: sums[0] = sum xfact rho, sums[1] =
+ * sum cfact rho, sums[2] = sum vxc rho. */
+void oo_pz_lda(size_t n, const double *rho, double *vxc, double sums[3]);
+
+typedef struct { double ehart, exc, eext, muxc; } oo_vks_energies;
+
+/* V_KS(r) = V_xc(r) + sum_{|G|^2 <= 4 ecut} [4 pi rho_G/(G^2 Omega) + V_loc(G)]
+ * exp(+i G.r), rho_G = (Omega/N) sum_r rho(r) exp(-i G.r)
+ * (CP_Rho_RealSpacePlane.C:739-864, 1365-1390; cp_hart_ext.C:168-330):
+ * E_H = 1/2 sum_G w 4 pi |rho_G|^2/(G^2 Omega), E_ext = sum_G w Re(rho_G
+ * conj V_loc(G)) (w = 1 on kx = 0, 2 elsewhere), E_xc = (3/4 ex + ec)
+ * Omega/N.  vloc may be NULL (no ions). */
+void oo_vks(const oo_sys *s, const double *rho, const double *vloc, double *v,
+            oo_vks_energies *e);
+
 typedef struct {
   double dt;          /* SD step (a.u.)                        */
   double tau_fs;      /* cp_mass_tau_def (fs), default 25      */
@@ -155,6 +198,9 @@ typedef struct {
   int use_cg;         /* conjugate gradient instead of SD      */
   oo_cg_state *cg_state;  /* CG only: persistent state         */
   double *cg;         /* CG only: [nst][npts] complex direction */
+  int vks_from_rho;   /* V = V_KS(rho) of this step (oo_vks)   */
+  const double *vloc; /* V_KS only: V_loc(G) half grid or NULL */
+  double *vks_out;    /* V_KS only: the step's V(r), or NULL   */
 } oo_step_opts;
 
 typedef struct {
@@ -162,9 +208,11 @@ typedef struct {
   double fnorm;       /* sqrt(sum |F|^2) after projection        */
   int ns_iters;
   double ns_err;
+  double ehart, exc;  /* V_KS only (eloc is then E_ext)          */
 } oo_step_out;
 
-/* One minimisation step on the host (gspace.ci:106-190 order):
+/* One minimisation step on the host (gspace.ci:106-190 order; with
+ * vks_from_rho, V_KS is built from this step's rho before V psi):
  * G->R, rho, V psi, R->G, EKE + kinetic force, [Lambda], SD, red sync,
  * S, NS, Psi.T.  psi is updated in place; rho/S/T/f may be NULL. */
 int oo_step(const oo_sys *s, const oo_step_opts *o, const double *v,

@@ -15,9 +15,41 @@ def small_system(nmol=2, box=(5.0, 5.4, 5.8), ecut=14.0, grid=(24, 27, 30), nsta
     kx, ky, kz = src(ecut, hmat)
     pos, spec = system.place_water(nmol, min(box), seed, min_oo_ang=2.0)
     psi = system.initial_coeffs(kx, ky, kz, hmat, nstates, seed)
-    V, _ = system.gaussian_vloc(pos, spec, hmat, grid, seed)
+    V, params = system.gaussian_vloc(pos, spec, hmat, grid, seed)
+    tab = system.local_pseudo_tables(params, ecut)
+    species = np.array([tab["order"].index(s) for s in spec], np.int32)
     return dict(name="small", seed=seed, hmat=hmat, grid=tuple(grid), ecut_ry=ecut,
-                nstates=nstates, natoms=len(spec), kx=kx, ky=ky, kz=kz, psi=psi, vks=V)
+                nstates=nstates, natoms=len(spec), kx=kx, ky=ky, kz=kz, psi=psi, vks=V,
+                pos=pos, spec=spec, vparams=params, species=species, pseudo=tab)
+
+
+def oracle_pseudo(s):
+    """(oracle LocalPseudo struct, arrays kept alive) of a built system."""
+    import oracle
+    t = s["pseudo"]
+    return oracle.local_pseudo(s["pos"], s["species"], t["gmin"], t["dg"], t["vps"],
+                               t["vps_g0"])
+
+
+def periodic_gaussians(pos, spec, params, hmat, grid, images=2):
+    """sum_I sum_n A exp(-|r - R_I - L n|^2 / (2 sigma^2)) over image shifts
+    n in [-images, images]^3 of an orthorhombic cell: the exact periodic
+    Gaussian potential the local-pseudopotential form factors stand for."""
+    h = np.asarray(hmat, np.float64).reshape(3, 3)
+    L = np.array([h[0, 0], h[1, 1], h[2, 2]])
+    nx, ny, nz = grid
+    ax = [np.arange(n) * (L[d] / n) for d, n in enumerate((nx, ny, nz))]
+    V = np.zeros((nz, ny, nx))
+    sh = np.arange(-images, images + 1)
+    for p, s in zip(pos, spec):
+        A, sig = params[s]
+        c = -1.0 / (2.0 * sig * sig)
+        e = []
+        for d in range(3):
+            dd = ax[d][:, None] - p[d] - L[d] * sh[None, :]
+            e.append(np.exp(c * dd * dd).sum(axis=1))
+        V += A * e[2][:, None, None] * e[1][None, :, None] * e[0][None, None, :]
+    return V
 
 
 def dense_grid(c, kx, ky, kz, grid):

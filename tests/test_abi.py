@@ -1,5 +1,5 @@
 """CPU checks of the C ABI boundary: the library builds, loads and exports
-every symbol include/oa_step.h declares; its host-side k-vector enumeration
+every symbol include/*.h declares; its host-side k-vector enumeration
 (no GPU involved) matches the oracle's."""
 import ctypes
 import os
@@ -15,9 +15,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_symbols():
-    src = open(os.path.join(ROOT, "include", "oa_step.h")).read()
-    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(oa_[a-z0-9_]+)\s*\(", src)))
+    syms = set()
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if h.endswith(".h"):
+            src = open(os.path.join(ROOT, "include", h)).read()
+            src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+            syms |= set(re.findall(r"\b(oa_[a-z0-9_]+)\s*\(", src))
+    return sorted(syms)
 
 
 def test_header_declares_the_binding_table():

@@ -1,0 +1,101 @@
+"""The request loop (include/oa_serve.h): independent minimisations of
+different sizes admitted by HBM budget and run concurrently on one GPU.
+Each request's result must be the same alone and in the mix, bit for bit,
+and the same as driving its context step by step."""
+import numpy as np
+import pytest
+
+import oracle
+from openatom_amd import serve, system
+from openatom_amd.step import CPContext
+from tests.helpers import small_system
+
+pytestmark = pytest.mark.gpu
+
+# conjugate gradient; Newton-Schulz by launches (no spinning kernels while
+# other requests share the device)
+OPTS = dict(dt=20.0, use_cg=1, ns_mode=1)
+F_TOL = 1e-4
+MAX_STEPS = 400
+
+_cache = {}
+
+
+def systems():
+    if not _cache:
+        # each converges to |F| <= 1e-4 within 150..360 CG steps on the oracle
+        _cache["list"] = [
+            system.build(system.serving_workload(4, 50.0), seed=5,
+                         kvec_source=oracle.stored_kvecs),                   # 45^3, 16 states
+            small_system(box=(5.0, 5.2, 5.8), grid=(27, 27, 30), nstates=8, seed=5),
+            system.build("water8", seed=11, kvec_source=oracle.stored_kvecs),  # 48^3, 32 states
+            small_system(box=(6.0, 6.0, 6.0), ecut=20.0, grid=(36, 36, 36), nstates=12, seed=21),
+        ]
+    return _cache["list"]
+
+
+def run(srv, reqs):
+    for tag, s in reqs:
+        srv.submit(s, tag, f_tol=F_TOL, max_steps=MAX_STEPS, want_psi=True, **OPTS)
+    return {r["tag"]: r for r in srv.drain()}
+
+
+def test_alone_equals_mix(gpu):
+    ss = systems()
+    alone = {}
+    for t, s in enumerate(ss):
+        with serve.Server(max_active=1) as srv:
+            alone.update(run(srv, [(t, s)]))
+    with serve.Server(max_active=4) as srv:
+        mix = run(srv, list(enumerate(ss)))
+        st = srv.stats()
+    print("[serve mix]", st, flush=True)
+    assert st["peak_active"] >= 2 and st["finished"] == len(ss)
+    for t in alone:
+        a, m = alone[t], mix[t]
+        print(f"[serve] req {t}: steps {a['steps']} E {a['eelec']:.10f} |F| {a['fnorm']:.2e} "
+              f"alone {a['run_ms']:.1f} ms, in mix {m['run_ms']:.1f} ms", flush=True)
+        assert a["status"] == 0 and m["status"] == 0
+        assert a["fnorm"] <= F_TOL
+        assert a["steps"] == m["steps"] and a["eelec"] == m["eelec"]
+        assert np.array_equal(a["psi"], m["psi"])
+
+
+def test_admission_respects_budget(gpu):
+    ss = systems()
+    need = [serve.ctx_bytes(s, **OPTS) for s in ss]
+    budget = sorted(need)[-1] + sorted(need)[-2]  # the two largest fit, not three
+    with serve.Server(hbm_budget=budget, max_active=4) as srv:
+        res = run(srv, list(enumerate(ss)))
+        st = srv.stats()
+    print("[serve budget]", budget, need, st, flush=True)
+    assert st["peak_hbm_bytes"] <= budget
+    assert st["peak_active"] <= 3
+    assert all(r["status"] == 0 for r in res.values())
+    assert all(r["hbm_bytes"] == need[t] for t, r in res.items())
+
+
+def test_request_equals_direct_steps(gpu):
+    """The server's result is the one of stepping the context directly."""
+    s = systems()[3]
+    with serve.Server(max_active=2) as srv:
+        r = run(srv, [(7, s)])[7]
+    with CPContext(s["grid"], s["hmat"], s["ecut_ry"], s["nstates"], s["kx"], s["ky"], s["kz"],
+                   **OPTS) as c:
+        c.set_psi(s["psi"])
+        c.set_vks(s["vks"])
+        for k in range(MAX_STEPS):
+            o = c.step()
+            if o["fnorm"] <= F_TOL:
+                break
+        psi = c.get_psi()
+    assert r["steps"] == k + 1 and r["eelec"] == o["eelec"]
+    assert np.array_equal(r["psi"], psi)
+
+
+def test_oversized_request_rejected(gpu):
+    s = systems()[2]
+    from openatom_amd import OaError
+    with serve.Server(hbm_budget=1 << 20, max_active=1) as srv:
+        with pytest.raises(OaError):
+            srv.submit(s, 1, **OPTS)
