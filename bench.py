@@ -8,7 +8,9 @@ Newton-Schulz (S/2)^{-1/2}, Psi T, and the ||Psi^H Psi/2 - I|| check.
 
 Workload (default water32): 32 H2O in a 9.86 A box, 128 doubly occupied
 states at Gamma, 70 Ry (32,961 stored plane waves per state), 100^3 grid,
-seeded coefficients and a seeded Gaussian local potential (synthetic data).
+seeded coefficients; V(r) = V_xc + V_H + V_loc (PZ-LDA, Hartree, seeded
+Gaussian local pseudopotential) rebuilt from each step's density on the
+device (synthetic data).
 `value` is the mixed-precision (FP32 FFT chain, FP64 coefficients,
 GEMMs and Newton-Schulz) steps/s with inputs resident in HBM; the plain
 FP64 path is timed in the same run and reported beside it.
@@ -67,16 +69,24 @@ def plane_bytes(stage, geo, csz, nst, launch):
     return 2 * nst * lines + launch["pcp"] * n * 8
 
 
-def run_precision(sysd, precision, steps, warmup, device=0, use_graph=0):
+def run_precision(sysd, precision, steps, warmup, device=0, use_graph=0, vks=True):
+    """vks: V(r) = V_xc + V_H + V_loc rebuilt from each step's rho on the
+    device (the density chares' work inside the step); else the seeded
+    local potential as a fixed input."""
     from openatom_amd.step import CPContext
     # The reference step has no ||Psi^H Psi - I|| evaluation (Ortho checks the
     # T iterates); it is measured once after the timed steps instead.
     c = CPContext(sysd["grid"], sysd["hmat"], sysd["ecut_ry"], sysd["nstates"], sysd["kx"],
                   sysd["ky"], sysd["kz"], precision=precision, device=device, dt=5.0,
                   ns_tol=1e-15 if precision == "fp64" else 1e-14, ns_max_iter=30,
-                  want_resid=0, use_graph=use_graph)
+                  want_resid=0, use_graph=use_graph, vks_from_rho=int(vks))
     c.set_psi(sysd["psi"])
-    c.set_vks(sysd["vks"])
+    if vks:
+        t = sysd["pseudo"]
+        c.set_local_pseudo(sysd["pos"], sysd["species"], t["gmin"], t["dg"], t["vps"],
+                           t["vps_g0"])
+    else:
+        c.set_vks(sysd["vks"])
     first = c.step()
     for _ in range(max(0, warmup - 1)):
         c.step(fetch=False)
@@ -196,6 +206,8 @@ def serving_run(systems, arrive, precision, f_tol, max_steps, budget, max_active
 
 
 def cpu_baseline(sysd, nsteps):
+    """The C oracle's FP64 steps (V_KS rebuilt from rho each step, as on
+    the GPU) on the host cores; V_loc(G) is set up before the clock."""
     import numpy as np
     import oracle
     ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
@@ -203,28 +215,36 @@ def cpu_baseline(sysd, nsteps):
     oracle.set_threads(nthr)
     o = oracle.System(sysd["grid"], sysd["hmat"], sysd["ecut_ry"], sysd["nstates"], sysd["kx"],
                       sysd["ky"], sysd["kz"])
+    t = sysd["pseudo"]
+    lp, keep = oracle.local_pseudo(sysd["pos"], sysd["species"], t["gmin"], t["dg"], t["vps"],
+                                   t["vps_g0"])
+    vloc = o.vloc_g(lp)
     psi = np.array(sysd["psi"], copy=True)
     t0 = time.perf_counter()
     outs = []
     for _ in range(nsteps):
-        outs.append(o.step(sysd["vks"], psi, dt=5.0, ns_tol=1e-15, ns_max_iter=30))
+        outs.append(o.step(None, psi, dt=5.0, ns_tol=1e-15, ns_max_iter=30, vloc=vloc))
     dt = time.perf_counter() - t0
     return dict(value=nsteps / dt, unit="steps/s", cores=nthr, kind="port",
-                sample=f"{nsteps} full FP64 CP steps of the same workload on the C oracle "
-                       f"(OpenMP, {nthr} threads of {ncores} visible cores)",
+                sample=f"{nsteps} full FP64 CP steps (V_KS from rho included) of the same "
+                       f"workload on the C oracle (OpenMP, {nthr} threads of {ncores} "
+                       f"visible cores)",
                 seconds=dt, eelec=[x["eelec"] for x in outs], psi=psi)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="water32")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="eager launches only")
+    ap.add_argument("--fixed-v", action="store_true",
+                    help="time the step with the seeded local potential as a fixed V(r) "
+                         "instead of V_KS rebuilt from rho")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="before the runs, copy a buffer of this many MiB three times "
                          "(a plain copy kernel that scales the HBM byte counters)")
@@ -262,12 +282,22 @@ def main():
     # HIP-event timings for the roofline) and, unless --no-graph, a run that
     # replays the captured step graph, reported beside it
     runs = {}
+    vks = not args.fixed_v
     for prec in ("fp64", "mixed"):
         for g in ((0, 1) if not args.no_graph else (0,)):
             log(f"[bench] {prec} run ({'graph' if g else 'eager'})")
-            r = run_precision(sysd, prec, args.steps, args.warmup, use_graph=g)
+            r = run_precision(sysd, prec, args.steps, args.warmup, use_graph=g, vks=vks)
             log(f"[bench] {prec} {'graph' if g else 'eager'} {r['ms']:.3f} ms/step")
             runs[(prec, g)] = r
+    # the same step with V(r) a fixed input (no density-chare work), beside it
+    fixed = None
+    if vks and not args.no_graph:
+        fixed = {}
+        for prec in ("fp64", "mixed"):
+            r = run_precision(sysd, prec, args.steps, args.warmup, vks=False)
+            log(f"[bench] {prec} fixed V(r) {r['ms']:.3f} ms/step")
+            fixed[prec] = {"steps_per_s": round(r["steps_per_s"], 3),
+                           "ms_per_step": round(r["ms"], 4)}
     r64, rmx = runs[("fp64", 0)], runs[("mixed", 0)]
     r64v, rmxv = r64, rmx
     graph = {p: round(runs[(p, 1)]["steps_per_s"], 3) for p in ("fp64", "mixed")
@@ -334,19 +364,26 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c64 FFT + f64 (coefficients, GEMMs, Newton-Schulz, energies)",
-        "data": "synthetic (seeded water box, Gaussian coefficients, Gaussian local potential)",
+        "data": ("synthetic (seeded water box, Gaussian coefficients; V(r) = PZ-LDA V_xc + "
+                 "Hartree + seeded-Gaussian local pseudopotential rebuilt from each step's rho)"
+                 if vks else
+                 "synthetic (seeded water box, Gaussian coefficients, fixed Gaussian local "
+                 "potential)"),
         "config": {"workload": f"{args.workload}: {sysd['nmol']} H2O, {nst} states at Gamma, "
                                f"{sysd['ecut_ry']:.0f} Ry, {sysd['grid'][0]}^3 grid, "
                                f"{r64['geo']['npts']} stored plane waves/state",
                    "global_batch": nst, "seq_len": r64["geo"]["npts"], "parallelism": "none (1 GPU)"},
         "step_launch": "eager launches on one stream",
         "graph_replay_steps_per_s": graph,
+        "fixed_v_steps": fixed,
         "fp64": {"steps_per_s": round(r64v["steps_per_s"], 3), "ms_per_step": round(r64v["ms"], 4),
                  "ns_iters": r64["last"]["ns_iters"], "resid_max": r64["last"]["resid_max"],
                  "roofline": rl64, "timings_ms": r64["timings"], "launch": r64["launch"]},
         "speedup_mixed_vs_fp64": round(rmxv["steps_per_s"] / r64v["steps_per_s"], 3),
         "quality": {"dE_per_atom_mixed_vs_fp64_after_steps": de_atom,
                     "dE_per_atom_first_step": de_first,
+                    "energies_fp64_last": {k: r64["last"][k] for k in
+                                           ("eelec", "eke", "eloc", "ehart", "exc")},
                     "resid_max_mixed": rmx["last"]["resid_max"],
                     "ns_iters_mixed": rmx["last"]["ns_iters"],
                     "steps_compared": args.warmup + args.steps},

@@ -243,6 +243,8 @@ class CPContext:
         context's stream); names follow the FFT mode in use."""
         names = {2: self.TIMING_NAMES_FUSED, 3: self.TIMING_NAMES_PLANE}.get(
             self.config["engine"], self.TIMING_NAMES)
+        if self.opts.vks_from_rho:  # stage 2 also builds V_KS from rho
+            names = names[:2] + (names[2] + "+vks",) + names[3:]
         ms = np.zeros(len(names))
         self._chk(self._L.oa_timings(self.h, ptr(ms), len(ms)))
         out = dict(zip(names, ms.tolist()))
