@@ -9,7 +9,8 @@
 // One LDS exchange per transform.  The thread -> (transform, j) map of each
 // pass is chosen per kernel so that the global loads and stores of that pass
 // are contiguous across lanes.  Radix-R DFTs up to 16 are unrolled in
-// registers: 2, 3, 4, 5 directly, composites as P x Q Cooley-Tukey with
+// registers: 2, 3, 4, 5 directly, composites of coprime factors by the
+// prime-factor map (no twiddles), the others as P x Q Cooley-Tukey with
 // compile-time twiddles.
 #pragma once
 
@@ -177,12 +178,48 @@ struct DftPQ {
   }
 };
 
-template <typename T, int S> struct Dft<T, 6, S> : DftPQ<T, 2, 3, S> {};
+// R = P*Q with gcd(P, Q) = 1 (Good-Thomas prime-factor map, no twiddles):
+// n = (Q n1 + P n2) mod R, k = (Q a k1 + P b k2) mod R with a = Q^-1 mod P,
+// b = P^-1 mod Q, so that W_R^{nk} = W_P^{n1 k1} W_Q^{n2 k2}.
+constexpr int ct_inv_mod(int x, int m) {
+  for (int i = 1; i < m; ++i)
+    if ((x * i) % m == 1) return i;
+  return 1;
+}
+
+template <typename T, int P, int Q, int S>
+struct DftPFA {
+  static __device__ __forceinline__ void run(cpx<T>* v) {
+    constexpr int R = P * Q;
+    constexpr int A = ct_inv_mod(Q % P, P), B = ct_inv_mod(P % Q, Q);
+    cpx<T> y[Q][P];
+#pragma unroll
+    for (int n2 = 0; n2 < Q; ++n2) {
+      cpx<T> a[P];
+#pragma unroll
+      for (int n1 = 0; n1 < P; ++n1) a[n1] = v[(Q * n1 + P * n2) % R];
+      Dft<T, P, S>::run(a);
+#pragma unroll
+      for (int k1 = 0; k1 < P; ++k1) y[n2][k1] = a[k1];
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) {
+      cpx<T> b[Q];
+#pragma unroll
+      for (int n2 = 0; n2 < Q; ++n2) b[n2] = y[n2][k1];
+      Dft<T, Q, S>::run(b);
+#pragma unroll
+      for (int k2 = 0; k2 < Q; ++k2) v[(Q * A * k1 + P * B * k2) % R] = b[k2];
+    }
+  }
+};
+
+template <typename T, int S> struct Dft<T, 6, S> : DftPFA<T, 2, 3, S> {};
 template <typename T, int S> struct Dft<T, 8, S> : DftPQ<T, 2, 4, S> {};
 template <typename T, int S> struct Dft<T, 9, S> : DftPQ<T, 3, 3, S> {};
-template <typename T, int S> struct Dft<T, 10, S> : DftPQ<T, 2, 5, S> {};
-template <typename T, int S> struct Dft<T, 12, S> : DftPQ<T, 4, 3, S> {};
-template <typename T, int S> struct Dft<T, 15, S> : DftPQ<T, 3, 5, S> {};
+template <typename T, int S> struct Dft<T, 10, S> : DftPFA<T, 2, 5, S> {};
+template <typename T, int S> struct Dft<T, 12, S> : DftPFA<T, 4, 3, S> {};
+template <typename T, int S> struct Dft<T, 15, S> : DftPFA<T, 3, 5, S> {};
 template <typename T, int S> struct Dft<T, 16, S> : DftPQ<T, 4, 4, S> {};
 
 // pass-2 twiddle W_N^{m} from the table exp(-2 pi i j / N), conjugated for S = +1

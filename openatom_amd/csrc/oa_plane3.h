@@ -47,20 +47,37 @@ struct PS {
   static constexpr int ROUNDS = (NPAIR + NTRX - 1) / NTRX;
   static constexpr int CY = NY | 1;
   static constexpr int EX = NTRX * NX > NTRY * NY ? NTRX * NX : NTRY * NY;  // E slots
+  // pass-2 twiddles W_N^m, m = t j <= (X1 - 1)(X2 - 1), in LDS (planes are
+  // square, so one table serves the x and the y transforms)
+  static constexpr int NTW = (X1 - 1) * (X2 - 1) + 1;
+  static_assert(NX == NY, "the fused plane passes need square x-y planes");
 };
 
-// LDS bytes of either kernel: P (nplane_x columns) + E
+// pass-2 twiddle W_N^m from the LDS table, conjugated for S = +1
+template <typename T, int S>
+__device__ __forceinline__ cpx<T> twl(const cpx<T>* tw, int m) {
+  const cpx<T> w = tw[m];
+  return {w.x, S < 0 ? w.y : -w.y};
+}
+
+// copy the pass-2 twiddles into LDS (callers synchronise before use)
+template <typename T, int NTW, int NT>
+__device__ __forceinline__ void load_twl(cpx<T>* tw, const double2* __restrict__ g) {
+  for (int i = threadIdx.x; i < NTW; i += NT) tw[i] = {(T)g[i].x, (T)g[i].y};
+}
+
+// LDS bytes of either kernel: P (nplane_x columns) + E + twiddles
 template <typename T, int X1, int X2, int Y1, int Y2, int NT>
 size_t plane3_smem(int npx) {
   using S = PS<X1, X2, Y1, Y2, NT>;
-  return sizeof(cpx<T>) * ((size_t)npx * S::CY + S::EX);
+  return sizeof(cpx<T>) * ((size_t)npx * S::CY + S::EX + S::NTW);
 }
 
 // y-FFT (FFTW_FORWARD) of the populated columns of one z-plane into P.
 // zs: zbuf plane [kx][NY]; only ky in [kylo, kyhi] of each column is read.
 template <typename T, int Y1, int Y2, int NT>
-__device__ __forceinline__ void y_fwd(const cpx<T>* __restrict__ zs, cpx<T>* P, const FftCtx& c,
-                                      int npx) {
+__device__ __forceinline__ void y_fwd(const cpx<T>* __restrict__ zs, cpx<T>* P, const cpx<T>* TW,
+                                      const FftCtx& c, int npx) {
   constexpr int NY = Y1 * Y2, GY = Y1 > Y2 ? Y1 : Y2, NTRY = NT / GY, CY = NY | 1;
   const int tid = threadIdx.x;
   const int tr = tid / GY, j = tid - tr * GY;
@@ -90,7 +107,7 @@ __device__ __forceinline__ void y_fwd(const cpx<T>* __restrict__ zs, cpx<T>* P, 
 #pragma unroll
       for (int t = 0; t < Y2; t++) u[t] = P[kx * CY + j + t * Y1];
 #pragma unroll
-      for (int t = 1; t < Y2; t++) u[t] = cm(u[t], twiddle<T, -1>(c.twy, t * j));
+      for (int t = 1; t < Y2; t++) u[t] = cm(u[t], twl<T, -1>(TW, t * j));
       Dft<T, Y2, -1>::run(u);
 #pragma unroll
       for (int t = 0; t < Y2; t++) P[kx * CY + j + t * Y1] = u[t];
@@ -102,8 +119,8 @@ __device__ __forceinline__ void y_fwd(const cpx<T>* __restrict__ zs, cpx<T>* P, 
 // y-FFT (FFTW_BACKWARD) of the nplane_x columns of P into the zout plane,
 // keeping ky in [kylo, kyhi]; E is the exchange buffer.
 template <typename T, int Y1, int Y2, int NT>
-__device__ __forceinline__ void y_bwd(const cpx<T>* P, cpx<T>* E, cpx<T>* __restrict__ zo,
-                                      const FftCtx& c, int npx) {
+__device__ __forceinline__ void y_bwd(const cpx<T>* P, cpx<T>* E, const cpx<T>* TW,
+                                      cpx<T>* __restrict__ zo, const FftCtx& c, int npx) {
   constexpr int NY = Y1 * Y2, GY = Y1 > Y2 ? Y1 : Y2, NTRY = NT / GY, CY = NY | 1;
   const int tid = threadIdx.x;
   const int tr = tid / GY, j = tid - tr * GY;
@@ -124,7 +141,7 @@ __device__ __forceinline__ void y_bwd(const cpx<T>* P, cpx<T>* E, cpx<T>* __rest
 #pragma unroll
       for (int t = 0; t < Y2; t++) u[t] = E[tr * NY + j + t * Y1];
 #pragma unroll
-      for (int t = 1; t < Y2; t++) u[t] = cm(u[t], twiddle<T, +1>(c.twy, t * j));
+      for (int t = 1; t < Y2; t++) u[t] = cm(u[t], twl<T, +1>(TW, t * j));
       Dft<T, Y2, +1>::run(u);
       const int lo = c.kylo[kx], hi = c.kyhi[kx];
       cpx<T>* dst = zo + (size_t)kx * NY;
@@ -144,9 +161,9 @@ __device__ __forceinline__ void y_bwd(const cpx<T>* P, cpx<T>* E, cpx<T>* __rest
 // On return thread (tr, j < X1) holds psi(r) at x = j + t*X1 in u[t].
 // Ends after the pass-2 reads of E (callers sync before reusing E).
 template <typename T, int X1, int X2, int NT>
-__device__ __forceinline__ void x_c2r(const cpx<T>* P, cpx<T>* E, const FftCtx& c, int npx,
-                                      int ny, int q, int tr, int j, bool act,
-                                      cpx<T> (&u)[X2]) {
+__device__ __forceinline__ void x_c2r(const cpx<T>* P, cpx<T>* E, const cpx<T>* TW,
+                                      const FftCtx& c, int npx, int ny, int q, int tr, int j,
+                                      bool act, cpx<T> (&u)[X2]) {
   constexpr int NX = X1 * X2;
   const int CY = ny | 1;
   const int y1 = 2 * q, y2 = y1 + 1;
@@ -182,7 +199,7 @@ __device__ __forceinline__ void x_c2r(const cpx<T>* P, cpx<T>* E, const FftCtx& 
 #pragma unroll
     for (int t = 0; t < X2; t++) u[t] = E[tr * NX + j + t * X1];
 #pragma unroll
-    for (int t = 1; t < X2; t++) u[t] = cm(u[t], twiddle<T, -1>(c.twx, t * j));
+    for (int t = 1; t < X2; t++) u[t] = cm(u[t], twl<T, -1>(TW, t * j));
     Dft<T, X2, -1>::run(u);
   }
 }
@@ -198,6 +215,7 @@ __global__ __launch_bounds__(NT) void k3_g2r(FftCtx c) {
   const int npx = c.g.nplane_x;
   cpx<T>* P = reinterpret_cast<cpx<T>*>(smem_raw);
   cpx<T>* E = P + (size_t)npx * S::CY;
+  cpx<T>* TW = E + S::EX;
   const int z = blockIdx.x, rp = blockIdx.y, chunk = blockIdx.z;
   const int r0 = rp * RPW;
   int st0, st1;
@@ -212,6 +230,8 @@ __global__ __launch_bounds__(NT) void k3_g2r(FftCtx c) {
   const int tr = tid / GX, j = tid - tr * GX;
   const size_t plane = (size_t)npx * NY;
   const size_t NP = (size_t)NX * NY;
+  load_twl<T, S::NTW, NT>(TW, c.twx);
+  __syncthreads();
   double acc[RPW][2 * X2];
 #pragma unroll
   for (int a = 0; a < RPW; a++)
@@ -219,7 +239,7 @@ __global__ __launch_bounds__(NT) void k3_g2r(FftCtx c) {
     for (int b = 0; b < 2 * X2; b++) acc[a][b] = 0.0;
   for (int s = st0; s < st1; s++) {
     const cpx<T>* zs = (const cpx<T>*)c.zbuf + ((size_t)s * c.g.nz + z) * plane;
-    y_fwd<T, Y1, Y2, NT>(zs, P, c, npx);
+    y_fwd<T, Y1, Y2, NT>(zs, P, TW, c, npx);
     const double oc = c.occ[s];
 #pragma unroll
     for (int rr = 0; rr < RPW; rr++) {
@@ -228,7 +248,7 @@ __global__ __launch_bounds__(NT) void k3_g2r(FftCtx c) {
         const int q = r * NTRX + tr;
         const bool act = tr < NTRX && q < S::NPAIR;
         cpx<T> u[X2];
-        x_c2r<T, X1, X2, NT>(P, E, c, npx, NY, q, tr, j, act, u);
+        x_c2r<T, X1, X2, NT>(P, E, TW, c, npx, NY, q, tr, j, act, u);
         if (act && j < X1) {
           const int y1 = 2 * q, y2 = y1 + 1;
           if (PSIR) {
@@ -269,8 +289,9 @@ __global__ __launch_bounds__(NT) void k3_g2r(FftCtx c) {
 
 // R->G plane pass.  Grid (nz, state chunks).  VREG: V(r) of the thread's
 // pass-2 positions for all rounds is held in registers across the chunk.
+// FP32: at most 168 VGPRs, three waves per SIMD (measured faster than two)
 template <typename T, int X1, int X2, int Y1, int Y2, int NT, bool VREG>
-__global__ __launch_bounds__(NT) void k3_r2g(FftCtx c) {
+__global__ __launch_bounds__(NT, sizeof(T) == 4 ? 3 : 1) void k3_r2g(FftCtx c) {
   using S = PS<X1, X2, Y1, Y2, NT>;
   constexpr int NX = S::NX, NY = S::NY, GX = S::GX, NTRX = S::NTRX, R = S::ROUNDS;
   constexpr int CY = S::CY;
@@ -278,6 +299,7 @@ __global__ __launch_bounds__(NT) void k3_r2g(FftCtx c) {
   const int npx = c.g.nplane_x;
   cpx<T>* P = reinterpret_cast<cpx<T>*>(smem_raw);
   cpx<T>* E = P + (size_t)npx * CY;
+  cpx<T>* TW = E + S::EX;
   const int z = blockIdx.x, chunk = blockIdx.y;
   const int st0 = chunk * c.cs_r, st1 = min(c.g.nst, st0 + c.cs_r);
   const int tid = threadIdx.x;
@@ -285,6 +307,8 @@ __global__ __launch_bounds__(NT) void k3_r2g(FftCtx c) {
   const size_t plane = (size_t)npx * NY;
   const size_t NP = (size_t)NX * NY;
   const double* vz = c.vks + (size_t)z * NP;
+  load_twl<T, S::NTW, NT>(TW, c.twx);
+  __syncthreads();
   T vr[VREG ? R : 1][VREG ? 2 * X2 : 1];
   if (VREG) {
 #pragma unroll
@@ -301,14 +325,14 @@ __global__ __launch_bounds__(NT) void k3_r2g(FftCtx c) {
   }
   for (int s = st0; s < st1; s++) {
     const cpx<T>* zs = (const cpx<T>*)c.zbuf + ((size_t)s * c.g.nz + z) * plane;
-    y_fwd<T, Y1, Y2, NT>(zs, P, c, npx);
+    y_fwd<T, Y1, Y2, NT>(zs, P, TW, c, npx);
     const T oc = (T)c.occ[s];
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int q = r * NTRX + tr;
       const bool act = tr < NTRX && q < S::NPAIR;
       cpx<T> u[X2];
-      x_c2r<T, X1, X2, NT>(P, E, c, npx, NY, q, tr, j, act, u);
+      x_c2r<T, X1, X2, NT>(P, E, TW, c, npx, NY, q, tr, j, act, u);
       __syncthreads();  // E is rewritten below
       if (act && j < X1) {
         const int y1 = 2 * q, y2 = y1 + 1;
@@ -338,7 +362,7 @@ __global__ __launch_bounds__(NT) void k3_r2g(FftCtx c) {
 #pragma unroll
         for (int t = 0; t < X1; t++) a[t] = E[tr * NX + j + t * X2];
 #pragma unroll
-        for (int t = 1; t < X1; t++) a[t] = cm(a[t], twiddle<T, -1>(c.twx, t * j));
+        for (int t = 1; t < X1; t++) a[t] = cm(a[t], twl<T, -1>(TW, t * j));
         Dft<T, X1, -1>::run(a);
 #pragma unroll
         for (int t = 0; t < X1; t++) E[tr * NX + j + t * X2] = a[t];
@@ -357,7 +381,7 @@ __global__ __launch_bounds__(NT) void k3_r2g(FftCtx c) {
       __syncthreads();
     }
     cpx<T>* zo = (cpx<T>*)c.zout + ((size_t)s * c.g.nz + z) * plane;
-    y_bwd<T, Y1, Y2, NT>(P, E, zo, c, npx);
+    y_bwd<T, Y1, Y2, NT>(P, E, TW, zo, c, npx);
   }
 }
 

@@ -39,15 +39,20 @@ __device__ __forceinline__ double gsq(const double* hi, int a, int b, int c) {
 }
 
 // Perdew-Zunger 1981 LDA per point: exchange factor xf, correlation energy
-// factor cf and potential factor mf (cp_pz_lda.C:48-54, 114-138)
+// factor cf and potential factor mf (cp_pz_lda.C:48-54, 114-138).  One cube
+// root serves both powers: rs = (3/(4 pi))^(1/3) / rho^(1/3) and
+// xf = -(3/pi)^(1/3) rho^(1/3).
 __device__ __forceinline__ void pz_lda(double r, double& xf, double& cf, double& mf) {
   const double gamma = -0.14230, beta1 = 1.05290, beta2 = 0.33340;
   const double au = 0.03110, bu = -0.0480, cu = 0.00200, du = -0.01160;
-  const double power = 1.0 / 3.0, fpi = 4.0 * 3.14159265358979323846;
+  const double power = 1.0 / 3.0;
   const double rat1 = (7.0 / 6.0) * beta1, rat2 = (4.0 / 3.0) * beta2;
-  const double rs = pow(3.0 / (fpi * r), power);
+  const double c_rs = 0.62035049089940001667;  // (3/(4 pi))^(1/3)
+  const double c_x = 0.98474502184269654118;   // (3/pi)^(1/3)
+  const double cr = cbrt(r);
+  const double rs = c_rs / cr;
   const double sq = sqrt(rs);
-  xf = -pow(3.0 * r / 3.14159265358979323846, power);
+  xf = -c_x * cr;
   if (rs >= 1.0) {
     const double c1 = 1.0 + beta1 * sq + beta2 * rs;
     const double c2 = 1.0 + rat1 * sq + rat2 * rs;
