@@ -346,12 +346,20 @@ __global__ __launch_bounds__(1024) void k_gram_reduce(const double* __restrict__
       j = t;
     }
     const double* p = part + (size_t)i * n + j;
-    int sp = y;
-    for (; sp + 16 < nslice; sp += 32) {
-      s0 += p[(size_t)sp * nn];
-      s1 += p[(size_t)(sp + 16) * nn];
+    // thread y sums slices y, y+16, y+32, ...: up to 16 loads in flight
+    for (int base = 0; base < nslice; base += 256) {
+      double v[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const int sp = base + y + 16 * u;
+        v[u] = sp < nslice ? p[(size_t)sp * nn] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 16; u += 2) {
+        s0 += v[u];
+        s1 += v[u + 1];
+      }
     }
-    if (sp < nslice) s0 += p[(size_t)sp * nn];
   }
   red[y][x] = s0 + s1;
   __syncthreads();

@@ -289,9 +289,11 @@ __global__ __launch_bounds__(NT) void k3_g2r(FftCtx c) {
 
 // R->G plane pass.  Grid (nz, state chunks).  VREG: V(r) of the thread's
 // pass-2 positions for all rounds is held in registers across the chunk.
-// FP32: at most 168 VGPRs, three waves per SIMD (measured faster than two)
+// FP32 up to N = 100: at most 168 VGPRs, three waves per SIMD (measured
+// faster than two; larger planes would spill)
 template <typename T, int X1, int X2, int Y1, int Y2, int NT, bool VREG>
-__global__ __launch_bounds__(NT, sizeof(T) == 4 ? 3 : 1) void k3_r2g(FftCtx c) {
+__global__ __launch_bounds__(NT, sizeof(T) == 4 ? (X1 * X2 <= 100 ? 3 : 2) : 1) void k3_r2g(
+    FftCtx c) {
   using S = PS<X1, X2, Y1, Y2, NT>;
   constexpr int NX = S::NX, NY = S::NY, GX = S::GX, NTRX = S::NTRX, R = S::ROUNDS;
   constexpr int CY = S::CY;
@@ -392,10 +394,20 @@ enum PlaneKind { G2R = 0, R2G = 1, G2R_PSIR = 2 };
 constexpr int kPlaneThreads = 256;
 
 // x rounds per k3_g2r workgroup: the rho accumulators are 2*X2 doubles per
-// round and thread; FP64 keeps one round
+// round and thread; FP64 keeps one round, FP32 two.  When the LDS of one
+// workgroup exceeds half a CU's (one workgroup per CU, four waves), the
+// register file has room for every round and one workgroup covers the whole
+// plane ("wide"), so the y-FFT of a plane is not repeated per row part.
 template <typename T>
 constexpr int row_rounds() {
   return sizeof(T) == 8 ? 1 : 2;
+}
+
+constexpr size_t kCuLds = 160 * 1024;
+
+template <typename T, int X1, int X2, int Y1, int Y2, int NT>
+bool g2r_wide(int npx) {
+  return 2 * plane3_smem<T, X1, X2, Y1, Y2, NT>(npx) > kCuLds;
 }
 
 // Launch facts of one size for the host's chunk choice: row parts of the
@@ -406,8 +418,10 @@ hipError_t plane3_info(int npx, int* rowparts, int* occ_g2r, int* occ_r2g) {
   using S = PS<X1, X2, Y1, Y2, NT>;
   constexpr bool VREG = sizeof(T) == 4 && S::ROUNDS * 2 * X2 <= 48;
   const size_t smem = plane3_smem<T, X1, X2, Y1, Y2, NT>(npx);
-  *rowparts = (S::ROUNDS + row_rounds<T>() - 1) / row_rounds<T>();
-  const void* g = (const void*)k3_g2r<T, X1, X2, Y1, Y2, NT, row_rounds<T>(), false>;
+  const bool wide = g2r_wide<T, X1, X2, Y1, Y2, NT>(npx);
+  *rowparts = wide ? 1 : (S::ROUNDS + row_rounds<T>() - 1) / row_rounds<T>();
+  const void* g = wide ? (const void*)k3_g2r<T, X1, X2, Y1, Y2, NT, S::ROUNDS, false>
+                       : (const void*)k3_g2r<T, X1, X2, Y1, Y2, NT, row_rounds<T>(), false>;
   const void* r = (const void*)k3_r2g<T, X1, X2, Y1, Y2, NT, VREG>;
   hipError_t e;
   if (smem > 65536) {
@@ -434,19 +448,30 @@ hipError_t launch_plane3(int kind, const FftCtx& c, hipStream_t st) {
   if (c.g.nx != S::NX || c.g.ny != S::NY) return hipErrorInvalidValue;
   // raise the dynamic LDS limit of an instantiation once per larger request
   static size_t granted[3] = {65536, 65536, 65536};
-  auto grant = [&](const void* fn) -> hipError_t {
-    if (kind < 0 || kind > 2 || smem <= granted[kind]) return hipSuccess;
+  static size_t granted_wide = 65536;
+  auto grant_fn = [&](const void* fn, size_t& have) -> hipError_t {
+    if (smem <= have) return hipSuccess;
     const hipError_t e =
         hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-    if (e == hipSuccess) granted[kind] = smem;
+    if (e == hipSuccess) have = smem;
     return e;
+  };
+  auto grant = [&](const void* fn) -> hipError_t {
+    if (kind < 0 || kind > 2) return hipSuccess;
+    return grant_fn(fn, granted[kind]);
   };
   hipError_t e = hipSuccess;
   switch (kind) {
     case G2R: {
-      auto fn = k3_g2r<T, X1, X2, Y1, Y2, NT, kRowRounds, false>;
-      if ((e = grant((const void*)fn)) != hipSuccess) return e;
-      hipLaunchKernelGGL(fn, dim3(c.g.nz, RPn, c.nchunk), dim3(NT), smem, st, c);
+      if (g2r_wide<T, X1, X2, Y1, Y2, NT>(c.g.nplane_x)) {
+        auto fn = k3_g2r<T, X1, X2, Y1, Y2, NT, S::ROUNDS, false>;
+        if ((e = grant_fn((const void*)fn, granted_wide)) != hipSuccess) return e;
+        hipLaunchKernelGGL(fn, dim3(c.g.nz, 1, c.nchunk), dim3(NT), smem, st, c);
+      } else {
+        auto fn = k3_g2r<T, X1, X2, Y1, Y2, NT, kRowRounds, false>;
+        if ((e = grant((const void*)fn)) != hipSuccess) return e;
+        hipLaunchKernelGGL(fn, dim3(c.g.nz, RPn, c.nchunk), dim3(NT), smem, st, c);
+      }
       break;
     }
     case G2R_PSIR: {
