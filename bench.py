@@ -69,7 +69,7 @@ def plane_bytes(stage, geo, csz, nst, launch):
     return 2 * nst * lines + launch["pcp"] * n * 8
 
 
-def run_precision(sysd, precision, steps, warmup, device=0, use_graph=0, vks=True):
+def run_precision(sysd, precision, steps, warmup, device=0, use_graph=0, vks=True, autotune=1):
     """vks: V(r) = V_xc + V_H + V_loc rebuilt from each step's rho on the
     device (the density chares' work inside the step); else the seeded
     local potential as a fixed input."""
@@ -79,7 +79,8 @@ def run_precision(sysd, precision, steps, warmup, device=0, use_graph=0, vks=Tru
     c = CPContext(sysd["grid"], sysd["hmat"], sysd["ecut_ry"], sysd["nstates"], sysd["kx"],
                   sysd["ky"], sysd["kz"], precision=precision, device=device, dt=5.0,
                   ns_tol=1e-15 if precision == "fp64" else 1e-14, ns_max_iter=30,
-                  want_resid=0, use_graph=use_graph, vks_from_rho=int(vks))
+                  want_resid=0, use_graph=use_graph, vks_from_rho=int(vks),
+                  autotune=int(autotune))
     c.set_psi(sysd["psi"])
     if vks:
         t = sysd["pseudo"]
@@ -103,9 +104,11 @@ def run_precision(sysd, precision, steps, warmup, device=0, use_graph=0, vks=Tru
     geo = dict(nx=c.nx, ny=c.ny, nz=c.nz, nlines=c.nlines, nplane_x=c.nplane_x, npts=c.npts)
     psi = c.get_psi()
     cfg = dict(c.config)
+    tuning = c.tuning_log()
     c.close()
     return dict(precision=precision, ms=(t1 - t0) * 1e3 / steps, steps_per_s=steps / (t1 - t0),
-                first=first, last=last, timings=tim, geo=geo, psi=psi, launch=cfg)
+                first=first, last=last, timings=tim, geo=geo, psi=psi, launch=cfg,
+                tuning=[t for t in tuning.split("\n") if t])
 
 
 def roofline(res, nst, traffic=None):
@@ -242,6 +245,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="eager launches only")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="heuristic launch shapes instead of the autotuner's measured choice")
     ap.add_argument("--fixed-v", action="store_true",
                     help="time the step with the seeded local potential as a fixed V(r) "
                          "instead of V_KS rebuilt from rho")
@@ -286,7 +291,8 @@ def main():
     for prec in ("fp64", "mixed"):
         for g in ((0, 1) if not args.no_graph else (0,)):
             log(f"[bench] {prec} run ({'graph' if g else 'eager'})")
-            r = run_precision(sysd, prec, args.steps, args.warmup, use_graph=g, vks=vks)
+            r = run_precision(sysd, prec, args.steps, args.warmup, use_graph=g, vks=vks,
+                              autotune=not args.no_autotune)
             log(f"[bench] {prec} {'graph' if g else 'eager'} {r['ms']:.3f} ms/step")
             runs[(prec, g)] = r
     # the same step with V(r) a fixed input (no density-chare work), beside it
@@ -294,7 +300,8 @@ def main():
     if vks and not args.no_graph:
         fixed = {}
         for prec in ("fp64", "mixed"):
-            r = run_precision(sysd, prec, args.steps, args.warmup, vks=False)
+            r = run_precision(sysd, prec, args.steps, args.warmup, vks=False,
+                              autotune=not args.no_autotune)
             log(f"[bench] {prec} fixed V(r) {r['ms']:.3f} ms/step")
             fixed[prec] = {"steps_per_s": round(r["steps_per_s"], 3),
                            "ms_per_step": round(r["ms"], 4)}
@@ -378,7 +385,8 @@ def main():
         "fixed_v_steps": fixed,
         "fp64": {"steps_per_s": round(r64v["steps_per_s"], 3), "ms_per_step": round(r64v["ms"], 4),
                  "ns_iters": r64["last"]["ns_iters"], "resid_max": r64["last"]["resid_max"],
-                 "roofline": rl64, "timings_ms": r64["timings"], "launch": r64["launch"]},
+                 "roofline": rl64, "timings_ms": r64["timings"], "launch": r64["launch"],
+                 "autotune": r64["tuning"]},
         "speedup_mixed_vs_fp64": round(rmxv["steps_per_s"] / r64v["steps_per_s"], 3),
         "quality": {"dE_per_atom_mixed_vs_fp64_after_steps": de_atom,
                     "dE_per_atom_first_step": de_first,
@@ -389,6 +397,7 @@ def main():
                     "steps_compared": args.warmup + args.steps},
         "timings_ms": rmx["timings"],
         "launch": rmx["launch"],
+        "autotune": rmx["tuning"],
         "roofline": rl,
         "cpu_baseline": cpu,
         "serving": serving,

@@ -88,6 +88,11 @@ typedef struct {
                           (PZ-LDA exchange-correlation, Hartree, local pseudopotential of
                           oa_local_pseudo_set) between rho and V psi, as the density chares
                           do; 0 (default): V(r) is the input of oa_vks_set */
+  int autotune;        /* 1: oa_create times candidate launch shapes on the device and keeps
+                          the fastest: state chunks of the two plane passes, K slices of
+                          the S and Lambda products, Newton-Schulz as one cooperative launch
+                          or two launches per iteration.  Chunking changes the order of the
+                          rho sums (rounding only).  0 (default): fixed heuristics */
 } oa_options;
 
 typedef struct {
@@ -228,6 +233,12 @@ oa_status oa_step(oa_ctx *ctx, oa_step_out *out);
  * CP_exc_calc (cp_pz_lda.C:23-170); its energies land in the next
  * oa_step_result (ehart, exc, eloc = E_ext).  oa_vks_get copies V(r) out. */
 oa_status oa_local_pseudo_set(oa_ctx *ctx, const oa_local_pseudo *lp);
+
+/* What the autotuner measured and chose (one line per knob; empty when
+ * opt.autotune was 0).  No counterpart in the reference, whose launch
+ * shapes are fixed by its chare decomposition; the string lives as long as
+ * the context. */
+const char *oa_tuning_log(const oa_ctx *ctx);
 oa_status oa_vks_build(oa_ctx *ctx);
 oa_status oa_vks_get(oa_ctx *ctx, double *v);
 /* ||Psi^H Psi / 2 - I|| of the current coefficients (max-abs and rms over

@@ -24,7 +24,7 @@ EXPORTS = (
     "oa_fft_g2r", "oa_psir_get", "oa_density_get", "oa_vpsi_r2g", "oa_force_get",
     "oa_force_set", "oa_lambda", "oa_sd_update", "oa_overlap", "oa_ortho_ns",
     "oa_tmat_set", "oa_rotate", "oa_step", "oa_residual", "oa_sync", "oa_step_result", "oa_timings",
-    "oa_set_timing", "oa_local_pseudo_set", "oa_vks_build", "oa_vks_get",
+    "oa_set_timing", "oa_local_pseudo_set", "oa_vks_build", "oa_vks_get", "oa_tuning_log",
     # request loop (oa_serve.h)
     "oa_ctx_bytes", "oa_server_create", "oa_server_submit", "oa_server_poll", "oa_server_drain",
     "oa_server_stats", "oa_server_destroy",
@@ -45,7 +45,8 @@ class OaOptions(ctypes.Structure):
                 ("use_lambda", ctypes.c_int), ("use_cg", ctypes.c_int),
                 ("want_resid", ctypes.c_int),
                 ("fft_mode", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("ns_mode", ctypes.c_int), ("vks_from_rho", ctypes.c_int)]
+                ("ns_mode", ctypes.c_int), ("vks_from_rho", ctypes.c_int),
+                ("autotune", ctypes.c_int)]
 
 
 class OaStepOut(ctypes.Structure):
@@ -94,6 +95,8 @@ def load():
                             ctypes.POINTER(vp)]
     L.oa_last_error.argtypes = [vp]
     L.oa_last_error.restype = ctypes.c_char_p
+    L.oa_tuning_log.argtypes = [vp]
+    L.oa_tuning_log.restype = ctypes.c_char_p
     for name, args in (("oa_destroy", [vp]), ("oa_layout_info", [vp, vp]), ("oa_get_config", [vp, vp]),
                        ("oa_get_kvecs", [vp, vp, vp, vp]), ("oa_psi_set", [vp, vp]),
                        ("oa_psi_get", [vp, vp]), ("oa_vks_set", [vp, vp]),

@@ -41,13 +41,6 @@ __global__ __launch_bounds__(kThreads) void k2_zfwd(FftCtx c) {
   const int s = blockIdx.y, l0 = blockIdx.x * NTR;
   const int nl = min(NTR, c.g.nlines - l0);
   const int tid = threadIdx.x;
-  // pass-2 twiddles of this thread, loaded before pass 1 hides their latency
-  cpx<T> w2[R2];
-  {
-    const int j = tid / NTR;
-#pragma unroll
-    for (int t = 1; t < R2; t++) w2[t] = j < R1 ? twiddle<T, -1>(c.twz, t * j) : cpx<T>{};
-  }
   {  // pass 1: lanes along z (contiguous sphere points)
     const int tr = tid / G, j = tid - tr * G;
     if (tr < nl && j < R2) {
@@ -77,7 +70,7 @@ __global__ __launch_bounds__(kThreads) void k2_zfwd(FftCtx c) {
 #pragma unroll
       for (int t = 0; t < R2; t++) u[t] = ex[tr * N + j + t * R1];
 #pragma unroll
-      for (int t = 1; t < R2; t++) u[t] = cm(u[t], w2[t]);
+      for (int t = 1; t < R2; t++) u[t] = cm(u[t], twiddle<T, -1>(c.twz, t * j));
       Dft<T, R2, -1>::run(u);
       const size_t plane = (size_t)c.g.nplane_x * c.g.ny;
       cpx<T>* zs = (cpx<T>*)c.zbuf + (size_t)s * c.g.nz * plane + c.lines[l0 + tr].w;
@@ -97,15 +90,6 @@ __global__ __launch_bounds__(kThreads) void k2_zbwd(FftCtx c) {
   const int nl = min(NTR, c.g.nlines - l0);
   const int tid = threadIdx.x;
   const size_t plane = (size_t)c.g.nplane_x * c.g.ny;
-  // pass-2 twiddles and line of this thread, loaded before pass 1 hides
-  // their latency
-  const int tr2 = tid / G, j2 = tid - tr2 * G;
-  const bool act2 = tr2 < nl && j2 < R1;
-  cpx<T> w2[R2];
-  int4 L2 = make_int4(0, 0, 0, 0);
-  if (act2) L2 = c.lines[l0 + tr2];
-#pragma unroll
-  for (int t = 1; t < R2; t++) w2[t] = act2 ? twiddle<T, +1>(c.twz, t * j2) : cpx<T>{};
   {  // pass 1: lanes along lines
     const int tr = tid % NTR, j = tid / NTR;
     if (tr < nl && j < R2) {
@@ -121,15 +105,15 @@ __global__ __launch_bounds__(kThreads) void k2_zbwd(FftCtx c) {
   __syncthreads();
   double eke = 0.0;
   {  // pass 2: lanes along z, pack to the sphere
-    const int tr = tr2, j = j2;
-    if (act2) {
+    const int tr = tid / G, j = tid - tr * G;
+    if (tr < nl && j < R1) {
       cpx<T> u[R2];
 #pragma unroll
       for (int t = 0; t < R2; t++) u[t] = ex[tr * N + j + t * R1];
 #pragma unroll
-      for (int t = 1; t < R2; t++) u[t] = cm(u[t], w2[t]);
+      for (int t = 1; t < R2; t++) u[t] = cm(u[t], twiddle<T, +1>(c.twz, t * j));
       Dft<T, R2, +1>::run(u);
-      const int4 L = L2;
+      const int4 L = c.lines[l0 + tr];
       const double w = (L.w < c.g.ny) ? 1.0 : 2.0;  // kx = 0 lines have dst < ny
       const double oc = c.occ[s];
       const double2* ps = c.psi + (size_t)s * c.g.ld + L.x;

@@ -182,6 +182,10 @@ struct oa_ctx {
   // V_KS from rho (opt.vks_from_rho): the density-sphere box and its buffers
   oa::VksArgs va{};
   double2* vloc = nullptr;
+  // autotuner (opt.autotune): candidate launch shapes (buffers are sized for
+  // the largest) and what was measured
+  std::vector<int> tune_nchunk, tune_nchunk_r, tune_nslice_sym, tune_nslice_full;
+  std::string tune_log;
 };
 
 namespace {
@@ -265,6 +269,18 @@ void pick_chunks(int units, int slots, int nst, int max_chunks, int& cs, int& nc
       nchunk = ncr;
     }
   }
+}
+
+// Autotuner candidates around a heuristic chunk count h: every count in
+// [h/2, 2h] (at most max_chunks) that splits nst into equal chunks.
+std::vector<int> chunk_candidates(int nst, int h, int max_chunks) {
+  std::vector<int> out;
+  for (int nc = std::max(1, h / 2); nc <= std::min({nst, max_chunks, 2 * h}); nc++) {
+    const int s = (nst + nc - 1) / nc;
+    if ((nst + s - 1) / s == nc) out.push_back(nc);
+  }
+  if (out.empty()) out.push_back(h);
+  return out;
 }
 
 // Validate the stored layout and build the device tables.
@@ -389,6 +405,10 @@ oa_status build_layout(oa_ctx* c) {
     pick_chunks(c->nz * rowparts, std::max(1, occ_g) * ncu, c->nst, 16, f.cs, f.nchunk);
     pick_chunks(c->nz, std::max(1, occ_r) * ncu, c->nst, c->nst, f.cs_r, f.nchunk_r);
     c->nzblk = (c->nlines + oa::fft2_ntr(c->nz) - 1) / oa::fft2_ntr(c->nz);
+    if (c->opt.autotune) {
+      c->tune_nchunk = chunk_candidates(c->nst, f.nchunk, 16);
+      c->tune_nchunk_r = chunk_candidates(c->nst, f.nchunk_r, c->nst);
+    }
   }
   if (c->engine == kEngFused) {
     const int npairs = (c->ny + 1) / 2;
@@ -534,7 +554,11 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
   const size_t zsz = (size_t)c->nst * c->nz * c->nplane_x * c->ny * csz;
   TRY(dalloc(c, &c->zbuf, zsz));
   if (c->engine != kEngSep)
-    TRY(dalloc(c, &c->fc.rho_part, (size_t)c->fc.nchunk * N * sizeof(double)));
+    TRY(dalloc(c, &c->fc.rho_part,
+               (size_t)std::max(c->fc.nchunk, c->tune_nchunk.empty()
+                                                   ? 0
+                                                   : c->tune_nchunk.back()) *
+                   N * sizeof(double)));
   if (c->engine == kEngPlane) {
     // psi(r) stays on chip; one state's worth for oa_psir_get
     TRY(dalloc(c, &c->zout, zsz));
@@ -563,8 +587,21 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
     ncu = 256;
   c->nslice_sym = oa::gram_slices(n, K, 2 * c->nkx0, 1, ncu);
   c->nslice_full = oa::gram_slices(n, K, 2 * c->nkx0, 0, ncu);
-  TRY(dalloc(c, &c->gpart,
-             (size_t)std::max(c->nslice_sym, c->nslice_full) * nn * sizeof(double)));
+  int max_slice = std::max(c->nslice_sym, c->nslice_full);
+  if (c->opt.autotune) {
+    // K slices: half, the heuristic and twice it (at least 2, at most K/64)
+    auto cands = [&](int h) {
+      std::vector<int> v;
+      for (int x : {h / 2, h, 2 * h})
+        if (x >= 2 && x <= std::max(2, K / 64) && (v.empty() || v.back() != x)) v.push_back(x);
+      if (v.empty()) v.push_back(h);
+      return v;
+    };
+    c->tune_nslice_sym = cands(c->nslice_sym);
+    c->tune_nslice_full = cands(c->nslice_full);
+    max_slice = std::max({max_slice, c->tune_nslice_sym.back(), c->tune_nslice_full.back()});
+  }
+  TRY(dalloc(c, &c->gpart, (size_t)max_slice * nn * sizeof(double)));
   TRY(dalloc(c, &c->nsX0, nn * sizeof(double)));
   TRY(dalloc(c, &c->nsX1, nn * sizeof(double)));
   TRY(dalloc(c, &c->nsT0, nn * sizeof(double)));
@@ -884,6 +921,108 @@ oa_status fetch_result(oa_ctx* c, oa_step_out* out) {
   return OA_OK;
 }
 
+// Mean device ms of fn() over three calls after one warm-up call, on the
+// context's stream; < 0 if a call fails.
+template <typename F>
+double time_stage(oa_ctx* c, hipEvent_t e0, hipEvent_t e1, F&& fn) {
+  if (fn() != OA_OK) return -1.0;
+  if (hipEventRecord(e0, c->stream) != hipSuccess) return -1.0;
+  for (int r = 0; r < 3; r++)
+    if (fn() != OA_OK) return -1.0;
+  if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+    return -1.0;
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.0;
+  return ms / 3.0;
+}
+
+// Try each candidate value of one knob (set(v) applies it), keep the
+// fastest, log the times.  The data in the buffers is whatever they hold
+// (zeros at creation): the kernels' time does not depend on it.
+template <typename Set, typename Run>
+void tune_knob(oa_ctx* c, hipEvent_t e0, hipEvent_t e1, const char* name,
+               const std::vector<int>& cand, int current, Set&& set, Run&& run) {
+  if (cand.size() < 2) return;
+  int best = current;
+  double tbest = 1e300;
+  std::string line = std::string(name) + ":";
+  for (int v : cand) {
+    set(v);
+    const double t = time_stage(c, e0, e1, run);
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), " %d=%.4fms", v, t);
+    line += buf;
+    if (t >= 0.0 && t < tbest) {
+      tbest = t;
+      best = v;
+    }
+  }
+  set(best);
+  line += " -> " + std::to_string(best) + (best == current ? " (heuristic)\n" : "\n");
+  c->tune_log += line;
+}
+
+// The measured choice of launch shapes (opt.autotune).
+oa_status autotune(oa_ctx* c) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(c, OA_EHIP, "hipEventCreate failed");
+  }
+  FftCtx& f = c->fc;
+  if (c->engine == kEngPlane) {
+    HIPCHK(oa::fft2_launch(oa::FFT_ZFWD, c->nz, c->fp32, fctx(c), c->stream));
+    tune_knob(
+        c, e0, e1, "G->R state chunks", c->tune_nchunk, f.nchunk,
+        [&](int v) {
+          f.cs = (c->nst + v - 1) / v;
+          f.nchunk = (c->nst + f.cs - 1) / f.cs;
+        },
+        [&]() -> oa_status {
+          HIPCHK(oa::plane3_launch(0, c->nx, c->fp32, fctx(c), c->stream));
+          HIPCHK(oa::rho_reduce(fctx(c), c->dpart, c->ndpart, c->stream));
+          return OA_OK;
+        });
+    tune_knob(
+        c, e0, e1, "R->G state chunks", c->tune_nchunk_r, f.nchunk_r,
+        [&](int v) {
+          f.cs_r = (c->nst + v - 1) / v;
+          f.nchunk_r = (c->nst + f.cs_r - 1) / f.cs_r;
+        },
+        [&]() -> oa_status {
+          HIPCHK(oa::plane3_launch(1, c->nx, c->fp32, fctx(c), c->stream));
+          return OA_OK;
+        });
+  }
+  tune_knob(
+      c, e0, e1, "S K slices", c->tune_nslice_sym, c->nslice_sym,
+      [&](int v) { c->nslice_sym = v; },
+      [&]() { return st_gram(c, c->psi, c->psi, true, 1.0 / std::sqrt(2.0), 1.0, c->S); });
+  tune_knob(
+      c, e0, e1, "Lambda K slices", c->tune_nslice_full, c->nslice_full,
+      [&](int v) { c->nslice_full = v; },
+      [&]() { return st_gram(c, c->psi, c->force, false, 0.5, 1.0, c->lam); });
+  if (c->ns_coop) {
+    // S = 2I converges in one iteration: the cost of one iteration either way
+    std::vector<double> two((size_t)c->nst * c->nst, 0.0);
+    for (int i = 0; i < c->nst; i++) two[(size_t)i * c->nst + i] = 2.0;
+    HIPCHK(hipMemcpy(c->S, two.data(), two.size() * sizeof(double), hipMemcpyHostToDevice));
+    tune_knob(
+        c, e0, e1, "Newton-Schulz cooperative (1) or launches (0)", {1, 0}, 1,
+        [&](int v) { c->ns_coop = v != 0; },
+        [&]() { return st_ns(c, c->opt.ns_tol, c->opt.ns_max_iter); });
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // leave no state from the trial runs behind
+  HIPCHK(oa::fill_zero(c->res, sizeof(StepDev), c->stream));
+  c->have_S = c->have_T = false;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return OA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -902,6 +1041,7 @@ void oa_default_options(oa_options* o) {
   o->use_graph = 0;
   o->ns_mode = 0;
   o->vks_from_rho = 0;
+  o->autotune = 0;
 }
 
 int oa_kvecs(double ecut_ry, const double hmat[9], int* kx, int* ky, int* kz) {
@@ -981,6 +1121,10 @@ static oa_status create_impl(int device, const oa_system* sys, const oa_options*
     c->err = "FFT workgroup needs " + std::to_string(need) + " bytes of LDS";
     return bail(OA_EINVAL);
   }
+  if (c->opt.autotune && !dry) {
+    s = autotune(c);
+    if (s != OA_OK) return bail(s);
+  }
   *out = c;
   return OA_OK;
 }
@@ -1028,6 +1172,8 @@ oa_status oa_destroy(oa_ctx* c) {
 }
 
 const char* oa_last_error(const oa_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+const char* oa_tuning_log(const oa_ctx* c) { return c ? c->tune_log.c_str() : ""; }
 
 oa_status oa_layout_info(const oa_ctx* c, int info[4]) {
   if (!c || !info) return OA_EINVAL;

@@ -140,6 +140,10 @@ class CPContext:
                                 ptr(keep[2]), ptr(keep[3]))
         self._chk(self._L.oa_local_pseudo_set(self.h, ctypes.byref(lp)))
 
+    def tuning_log(self):
+        """What the autotuner measured and chose (empty unless autotune=1)."""
+        return self._L.oa_tuning_log(self.h).decode()
+
     def vks_build(self):
         """V_KS from the context's current rho (after fft_g2r)."""
         self._chk(self._L.oa_vks_build(self.h))

@@ -293,3 +293,30 @@ def test_bad_layout_rejected(gpu):
     from openatom_amd import OaError
     with pytest.raises(OaError):
         CPContext(s["grid"], s["hmat"], s["ecut_ry"], s["nstates"], kx, ky, kz)
+
+
+@pytest.mark.parametrize("name,precision", [("water8", "fp64"), ("square27", "fp64"),
+                                            ("water8", "mixed")])
+def test_autotuned_context_matches_oracle(gpu, name, precision):
+    """opt.autotune times candidate chunkings, K slices and Newton-Schulz
+    modes at creation; whatever it picks, three steps still match the
+    oracle (FP64 at 1e-10; mixed at the FP32 FFT bar) and the log names
+    every knob it measured."""
+    s = get_system(name)
+    o = make_oracle(s)
+    psio = s["psi"].copy()
+    with make_ctx(s, precision=precision, autotune=1) as c:
+        log = c.tuning_log()
+        print(f"[{name} {precision} autotune]\n{log}", flush=True)
+        assert "K slices" in log
+        c.set_psi(s["psi"])
+        c.set_vks(s["vks"])
+        for k in range(3):
+            r = c.step()
+            ro = o.step(s["vks"], psio, dt=5.0, ns_tol=1e-15, ns_max_iter=30)
+        e_psi = rel(c.get_psi(), psio)
+    report(f"[{name} {precision} autotune]", dE=abs(r["eelec"] - ro["eelec"]), psi=e_psi)
+    if precision == "fp64":
+        assert abs(r["eelec"] - ro["eelec"]) < TOL * abs(ro["eelec"]) and e_psi < TOL
+    else:
+        assert abs(r["eelec"] - ro["eelec"]) / s["natoms"] < 1e-6 and e_psi < 1e-5
