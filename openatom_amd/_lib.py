@@ -25,6 +25,8 @@ EXPORTS = (
     "oa_force_set", "oa_lambda", "oa_sd_update", "oa_overlap", "oa_ortho_ns",
     "oa_tmat_set", "oa_rotate", "oa_step", "oa_residual", "oa_sync", "oa_step_result", "oa_timings",
     "oa_set_timing", "oa_local_pseudo_set", "oa_vks_build", "oa_vks_get", "oa_tuning_log",
+    # state files (oa_state.h)
+    "oa_state_read", "oa_state_write", "oa_states_read", "oa_states_write",
     # request loop (oa_serve.h)
     "oa_ctx_bytes", "oa_server_create", "oa_server_submit", "oa_server_poll", "oa_server_drain",
     "oa_server_stats", "oa_server_destroy",
@@ -111,7 +113,11 @@ def load():
                        ("oa_step_result", [vp, vp]), ("oa_timings", [vp, vp, ci]),
                        ("oa_set_timing", [vp, ci]),
                        ("oa_local_pseudo_set", [vp, ctypes.POINTER(OaLocalPseudo)]),
-                       ("oa_vks_build", [vp]), ("oa_vks_get", [vp, vp])):
+                       ("oa_vks_build", [vp]), ("oa_vks_get", [vp, vp]),
+                       ("oa_state_read", [ctypes.c_char_p, ci, ci, vp, vp, vp, vp, vp]),
+                       ("oa_state_write", [ctypes.c_char_p, ci, ci, vp, vp, vp, vp, vp]),
+                       ("oa_states_read", [vp, ctypes.c_char_p, ci, ci, ci, ci, ci]),
+                       ("oa_states_write", [vp, ctypes.c_char_p, ci, ci, ci, ci, ci, ci])):
         f = getattr(L, name)
         f.argtypes = args
         f.restype = ci

@@ -11,6 +11,7 @@
 #include <utility>
 #include <vector>
 
+#include "oa_ctx_info.h"
 #include "oa_internal.h"
 #include "oa_kernels.h"
 #include "oa_step.h"
@@ -1150,6 +1151,12 @@ size_t oa_ctx_device_bytes(const oa_ctx* c) { return c ? c->bytes : 0; }
 }  // extern "C"
 
 hipStream_t oa_ctx_stream(oa_ctx* c) { return c ? c->stream : nullptr; }
+int oa_ctx_nstates(const oa_ctx* c) { return c ? c->nst : 0; }
+void oa_ctx_grid(const oa_ctx* c, int grid[3]) {
+  grid[0] = c ? c->nx : 0;
+  grid[1] = c ? c->ny : 0;
+  grid[2] = c ? c->nz : 0;
+}
 
 extern "C" {
 

@@ -140,6 +140,18 @@ class CPContext:
                                 ptr(keep[2]), ptr(keep[3]))
         self._chk(self._L.oa_local_pseudo_set(self.h, ctypes.byref(lp)))
 
+    def read_states(self, directory, fmt=1, spin=0, kpt=0, bead=0, temper=0):
+        """Load every state from the reference's STATES layout
+        (directory/Spin.s_Kpt.k_Bead.b_Temper.t/state<i>.out)."""
+        self._chk(self._L.oa_states_read(self.h, str(directory).encode(), int(fmt), spin, kpt,
+                                         bead, temper))
+
+    def write_states(self, directory, fmt=1, spin=0, kpt=0, bead=0, temper=0, iteration=0):
+        """Write every state (and TimeStamp) in the reference's STATES layout;
+        the Spin/Kpt/Bead/Temper directory must exist."""
+        self._chk(self._L.oa_states_write(self.h, str(directory).encode(), int(fmt), spin, kpt,
+                                          bead, temper, int(iteration)))
+
     def tuning_log(self):
         """What the autotuner measured and chose (empty unless autotune=1)."""
         return self._L.oa_tuning_log(self.h).decode()
