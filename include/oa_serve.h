@@ -42,6 +42,10 @@ typedef struct {
   double *psi_out;      /* optional [nstates][npts] final coefficients; must stay valid
                            until the request's result is returned */
   long long tag;        /* caller's id, returned with the result */
+  double fp64_below;    /* mixed precision only: continue in FP64, from the current
+                           coefficients with a fresh CG direction, once |F| <= fp64_below or
+                           |F| has not improved for 50 steps (the FP32 FFT chain sets a
+                           floor on |F|); 0 disables the switch */
 } oa_request;
 
 typedef struct {
@@ -53,6 +57,7 @@ typedef struct {
   double run_ms;        /* admitted -> done */
   double total_ms;      /* submit -> done */
   size_t hbm_bytes;     /* device memory the request held */
+  int fp64_steps;       /* steps run in FP64 after a mixed-precision switch */
 } oa_result;
 
 /* HBM one context of this system needs (what admission charges) */

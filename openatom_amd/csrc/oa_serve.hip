@@ -10,7 +10,9 @@
 //   3. retires finished steps (event complete): reads the step's energies and
 //      |F|, and finishes the request at |F| <= f_tol (tol_cp_min,
 //      CP_State_GSpacePlane.C:158-191), at max_steps, or on an error,
-//      releasing its memory so that queued requests can be admitted.
+//      releasing its memory so that queued requests can be admitted; a
+//      mixed-precision request that reaches fp64_below or stalls continues
+//      in an FP64 context.
 // Steps of different requests overlap on the device; no request waits for
 // another one to finish.
 #include <hip/hip_runtime.h>
@@ -54,7 +56,14 @@ struct Active {
   int steps = 0;
   clk::time_point t_admit;
   oa_step_out last{};
+  // mixed precision: best |F| so far and when, and the FP64 continuation
+  double best_f = 1e300;
+  int best_step = 0;
+  bool in_fp64 = false;
+  int fp64_steps = 0;
 };
+
+constexpr int kStallSteps = 50;  // steps without a new best |F| that count as a stall
 
 }  // namespace
 
@@ -89,6 +98,7 @@ void finish(oa_server* s, Active* a, oa_status st) {
   res.run_ms = ms_since(a->t_admit, now);
   res.total_ms = ms_since(a->job.t_submit, now);
   res.hbm_bytes = a->job.bytes;
+  res.fp64_steps = a->fp64_steps;
   if (a->ev) (void)hipEventDestroy(a->ev);
   if (a->ctx) oa_destroy(a->ctx);
   std::lock_guard<std::mutex> lk(s->mu);
@@ -99,13 +109,18 @@ void finish(oa_server* s, Active* a, oa_status st) {
   s->cv_done.notify_all();
 }
 
+oa_system job_system(Job& j) {
+  oa_system sys = j.r.sys;
+  sys.kx = j.kx.empty() ? nullptr : j.kx.data();
+  sys.ky = j.ky.empty() ? nullptr : j.ky.data();
+  sys.kz = j.kz.empty() ? nullptr : j.kz.data();
+  sys.occ = j.occ.empty() ? nullptr : j.occ.data();
+  return sys;
+}
+
 // create and load one admitted request's context; false: it finished with an error
 bool start(oa_server* s, Active* a) {
-  oa_system sys = a->job.r.sys;
-  sys.kx = a->job.kx.empty() ? nullptr : a->job.kx.data();
-  sys.ky = a->job.ky.empty() ? nullptr : a->job.ky.data();
-  sys.kz = a->job.kz.empty() ? nullptr : a->job.kz.data();
-  sys.occ = a->job.occ.empty() ? nullptr : a->job.occ.data();
+  oa_system sys = job_system(a->job);
   oa_status st = oa_create(s->device, &sys, &a->job.r.opt, &a->ctx);
   if (st == OA_OK) st = oa_psi_set(a->ctx, a->job.psi0.data());
   if (st == OA_OK) st = oa_vks_set(a->ctx, a->job.vks.data());
@@ -116,6 +131,23 @@ bool start(oa_server* s, Active* a) {
     return false;
   }
   return true;
+}
+
+// Continue a mixed-precision request in FP64: its coefficients go through
+// the host copy of psi0 into a new FP64 context (the mixed one is released
+// first, so the request never holds both; admission charged the larger).
+oa_status to_fp64(oa_server* s, Active* a) {
+  oa_status st = oa_psi_get(a->ctx, a->job.psi0.data());
+  if (st != OA_OK) return st;
+  oa_destroy(a->ctx);
+  a->ctx = nullptr;
+  a->job.r.opt.precision = OA_FP64;
+  oa_system sys = job_system(a->job);
+  st = oa_create(s->device, &sys, &a->job.r.opt, &a->ctx);
+  if (st == OA_OK) st = oa_psi_set(a->ctx, a->job.psi0.data());
+  if (st == OA_OK) st = oa_vks_set(a->ctx, a->job.vks.data());
+  a->in_fp64 = true;
+  return st;
 }
 
 void scheduler(oa_server* s) {
@@ -180,6 +212,7 @@ void scheduler(oa_server* s) {
           s->steps_run++;
         }
         st = oa_step_result(a->ctx, &a->last);
+        if (a->in_fp64) a->fp64_steps++;
         if (st != OA_OK) {
           end = true;
         } else if (a->last.fnorm <= a->job.r.f_tol) {
@@ -187,6 +220,16 @@ void scheduler(oa_server* s) {
         } else if (a->steps >= a->job.r.max_steps) {
           st = OA_ENOCONV;
           end = true;
+        } else if (!a->in_fp64 && a->job.r.opt.precision == OA_MIXED &&
+                   a->job.r.fp64_below > 0.0) {
+          if (a->last.fnorm < a->best_f) {
+            a->best_f = a->last.fnorm;
+            a->best_step = a->steps;
+          }
+          if (a->last.fnorm <= a->job.r.fp64_below || a->steps - a->best_step >= kStallSteps) {
+            st = to_fp64(s, a);
+            if (st != OA_OK) end = true;
+          }
         }
       }
       if (end) {
@@ -228,6 +271,11 @@ oa_status oa_server_submit(oa_server* s, const oa_request* req) {
   const oa_system& sys = req->sys;
   (void)hipSetDevice(s->device);
   j.bytes = oa_ctx_bytes(&sys, &req->opt);
+  if (req->opt.precision == OA_MIXED && req->fp64_below > 0.0) {
+    oa_options o64 = req->opt;
+    o64.precision = OA_FP64;
+    j.bytes = std::max(j.bytes, oa_ctx_bytes(&sys, &o64));
+  }
   if (j.bytes == 0 || j.bytes > s->budget) return OA_EINVAL;
   int npts = sys.npts;
   if (npts > 0) {

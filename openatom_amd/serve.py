@@ -13,14 +13,15 @@ class OaRequest(ctypes.Structure):
     _fields_ = [("sys", OaSystem), ("opt", OaOptions), ("psi0", ctypes.c_void_p),
                 ("vks", ctypes.c_void_p), ("f_tol", ctypes.c_double),
                 ("max_steps", ctypes.c_int), ("psi_out", ctypes.c_void_p),
-                ("tag", ctypes.c_longlong)]
+                ("tag", ctypes.c_longlong), ("fp64_below", ctypes.c_double)]
 
 
 class OaResult(ctypes.Structure):
     _fields_ = [("tag", ctypes.c_longlong), ("status", ctypes.c_int), ("steps", ctypes.c_int),
                 ("eelec", ctypes.c_double), ("fnorm", ctypes.c_double),
                 ("queue_ms", ctypes.c_double), ("run_ms", ctypes.c_double),
-                ("total_ms", ctypes.c_double), ("hbm_bytes", ctypes.c_size_t)]
+                ("total_ms", ctypes.c_double), ("hbm_bytes", ctypes.c_size_t),
+                ("fp64_steps", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -99,7 +100,9 @@ class Server:
         self._pending = {}  # tag -> arrays kept alive until the result returns
 
     def submit(self, system, tag, f_tol=1e-5, max_steps=1000, precision="fp64", want_psi=False,
-               **opts):
+               fp64_below=None, **opts):
+        """fp64_below: mixed precision continues in FP64 once |F| reaches
+        it or stalls (default 30 f_tol for mixed; 0 never switches)."""
         keep = []
         req = OaRequest()
         req.sys = _system_struct(system, keep)
@@ -114,6 +117,9 @@ class Server:
         out = np.zeros_like(psi0) if want_psi else None
         req.psi_out = ptr(out)
         req.tag = int(tag)
+        if fp64_below is None:
+            fp64_below = 30.0 * f_tol if precision == "mixed" else 0.0
+        req.fp64_below = float(fp64_below)
         st = self._L.oa_server_submit(self.h, ctypes.byref(req))
         if st != _lib.OA_OK:
             raise OaError(st, "oa_server_submit failed")

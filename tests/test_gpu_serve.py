@@ -99,3 +99,23 @@ def test_oversized_request_rejected(gpu):
     with serve.Server(hbm_budget=1 << 20, max_active=1) as srv:
         with pytest.raises(OaError):
             srv.submit(s, 1, **OPTS)
+
+
+def test_mixed_request_finishes_in_fp64(gpu):
+    """A mixed-precision request whose tolerance lies below the FP32 FFT
+    floor continues in FP64 (fp64_below) and reaches the FP64 solve's
+    minimum; admission charged the larger of the two footprints."""
+    s = systems()[2]  # water8
+    f_tol = 1e-6
+    with serve.Server(max_active=2) as srv:
+        srv.submit(s, 1, f_tol=f_tol, max_steps=800, **OPTS)
+        srv.submit(s, 2, f_tol=f_tol, max_steps=800, precision="mixed", **OPTS)
+        res = {r["tag"]: r for r in srv.drain()}
+    r64, rmx = res[1], res[2]
+    print(f"[serve mixed->fp64] fp64: {r64['steps']} steps E {r64['eelec']:.10f}; mixed: "
+          f"{rmx['steps']} steps ({rmx['fp64_steps']} in FP64) E {rmx['eelec']:.10f}", flush=True)
+    assert r64["status"] == 0 and rmx["status"] == 0
+    assert rmx["fnorm"] <= f_tol and rmx["fp64_steps"] > 0
+    assert abs(rmx["eelec"] - r64["eelec"]) / s["natoms"] < 1e-8
+    need = max(serve.ctx_bytes(s, **OPTS), serve.ctx_bytes(s, precision="mixed", **OPTS))
+    assert rmx["hbm_bytes"] == need
