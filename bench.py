@@ -32,7 +32,8 @@ HBM_PEAK_GBS = 8000.0
 FP64_MFMA_TFLOPS = 78.6
 
 STAGES = ("k_zfwd", "k_y_fwd", "k_xc2r", "k_xr2c", "k_y_bwd", "k_zbwd",
-          "k_plane_fwd", "k_plane_bwd", "k_plane_g2r", "k_plane_r2g")
+          "k_plane_fwd", "k_plane_bwd", "k_plane_g2r", "k_plane_r2g", "k_plane_r2g_rho")
+PLANE3 = ("k_plane_g2r", "k_plane_r2g", "k_plane_r2g_rho")
 
 
 def log(*a):
@@ -52,7 +53,7 @@ def stage_bytes(stage, geo, csz, rsz):
         return bz
     if stage in ("k_y_fwd", "k_y_bwd"):
         return by
-    if stage in ("k_plane_fwd", "k_plane_bwd", "k_plane_g2r", "k_plane_r2g"):
+    if stage in ("k_plane_fwd", "k_plane_bwd") + PLANE3:
         return by + bx
     return bx
 
@@ -61,11 +62,14 @@ def plane_bytes(stage, geo, csz, nst, launch):
     """HBM bytes a two-factor fused plane pass must move (psi(r) stays on
     chip): G->R reads the populated z-lines of every state (n_lines Nz c)
     and writes one rho partial (N doubles) per state chunk; R->G reads and
-    writes the z-lines and reads V (N doubles) once per chunk."""
+    writes the z-lines and reads V (N doubles) once per chunk, and with rho
+    fused into it also writes one rho partial per chunk."""
     n = geo["nx"] * geo["ny"] * geo["nz"]
     lines = geo["nlines"] * geo["nz"] * csz
     if stage == "k_plane_g2r":
         return nst * lines + launch["nchunk"] * n * 8
+    if stage == "k_plane_r2g_rho":
+        return 2 * nst * lines + 2 * launch["pcp"] * n * 8
     return 2 * nst * lines + launch["pcp"] * n * 8
 
 
@@ -121,7 +125,7 @@ def roofline(res, nst, traffic=None):
     cands = {k: tim[k] for k in STAGES + ("overlap", "rotate") if k in tim}
     dom = max(cands, key=cands.get)
     ms = cands[dom]
-    if dom in ("k_plane_g2r", "k_plane_r2g"):
+    if dom in PLANE3:
         # fused pass: its own compulsory bytes; the 3-pass model's bytes for the
         # same work are reported beside it
         b = plane_bytes(dom, geo, csz, nst, res["launch"])

@@ -53,12 +53,13 @@ int fft2_ntr(int n);  // transforms per workgroup for length n, 0 if not compile
 
 // fused y+x plane passes on the two-factor engine (oa_plane3.h), square
 // planes nx == ny == n: kind 0 G->R + rho partials, 1 R->G (V psi, x and
-// y back to zout), 2 G->R writing psi(r) of c.psir_state
+// y back to zout), 2 G->R writing psi(r) of c.psir_state, 3 R->G that also
+// leaves the rho partials of its c.nchunk_r chunks (V independent of rho)
 hipError_t plane3_launch(int kind, int n, bool fp32, const FftCtx& c, hipStream_t st);
 size_t plane3_smem(int n, int nplane_x, bool fp32);  // 0 if n is not compiled
 // row parts of the G->R grid and resident workgroups per CU of both passes
 hipError_t plane3_info(int n, int nplane_x, bool fp32, int* rowparts, int* occ_g2r,
-                       int* occ_r2g);
+                       int* occ_r2g, int* rho_ok);
 
 // fused y+x plane passes (one z-plane, a chunk of states per workgroup)
 hipError_t plane_fwd(const FftCtx& c, bool fp32, hipStream_t st);

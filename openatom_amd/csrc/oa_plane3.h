@@ -290,10 +290,12 @@ __global__ __launch_bounds__(NT) void k3_g2r(FftCtx c) {
 // R->G plane pass.  Grid (nz, state chunks).  VREG: V(r) of the thread's
 // pass-2 positions for all rounds is held in registers across the chunk.
 // FP32 up to N = 100: at most 168 VGPRs, three waves per SIMD (measured
-// faster than two; larger planes would spill)
-template <typename T, int X1, int X2, int Y1, int Y2, int NT, bool VREG>
-__global__ __launch_bounds__(NT, sizeof(T) == 4 ? (X1 * X2 <= 100 ? 3 : 2) : 1) void k3_r2g(
-    FftCtx c) {
+// faster than two; larger planes would spill).  RHO: also accumulate rho
+// partials of the chunk from the recomputed psi(r), for a V(r) that does
+// not depend on this step's rho (then no separate G->R pass is needed).
+template <typename T, int X1, int X2, int Y1, int Y2, int NT, bool VREG, bool RHO>
+__global__ __launch_bounds__(NT, (sizeof(T) == 4 && !RHO) ? (X1 * X2 <= 100 ? 3 : 2) : 1) void
+k3_r2g(FftCtx c) {
   using S = PS<X1, X2, Y1, Y2, NT>;
   constexpr int NX = S::NX, NY = S::NY, GX = S::GX, NTRX = S::NTRX, R = S::ROUNDS;
   constexpr int CY = S::CY;
@@ -312,6 +314,13 @@ __global__ __launch_bounds__(NT, sizeof(T) == 4 ? (X1 * X2 <= 100 ? 3 : 2) : 1) 
   load_twl<T, S::NTW, NT>(TW, c.twx);
   __syncthreads();
   T vr[VREG ? R : 1][VREG ? 2 * X2 : 1];
+  double acc[RHO ? R : 1][RHO ? 2 * X2 : 1];
+  if (RHO) {
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+      for (int b = 0; b < 2 * X2; b++) acc[r][b] = 0.0;
+  }
   if (VREG) {
 #pragma unroll
     for (int r = 0; r < R; r++) {
@@ -338,6 +347,14 @@ __global__ __launch_bounds__(NT, sizeof(T) == 4 ? (X1 * X2 <= 100 ? 3 : 2) : 1) 
       __syncthreads();  // E is rewritten below
       if (act && j < X1) {
         const int y1 = 2 * q, y2 = y1 + 1;
+        if (RHO) {
+          const double od = c.occ[s];
+#pragma unroll
+          for (int t = 0; t < X2; t++) {
+            acc[r][2 * t] += od * (double)u[t].x * (double)u[t].x;
+            acc[r][2 * t + 1] += od * (double)u[t].y * (double)u[t].y;
+          }
+        }
         cpx<T> w[X2];
 #pragma unroll
         for (int t = 0; t < X2; t++) {
@@ -385,11 +402,33 @@ __global__ __launch_bounds__(NT, sizeof(T) == 4 ? (X1 * X2 <= 100 ? 3 : 2) : 1) 
     cpx<T>* zo = (cpx<T>*)c.zout + ((size_t)s * c.g.nz + z) * plane;
     y_bwd<T, Y1, Y2, NT>(P, E, TW, zo, c, npx);
   }
+  if (!RHO) return;
+  // rho partial of this chunk (same layout and per-element order as k3_g2r)
+  double* rp_out = c.rho_part + (size_t)chunk * NP * c.g.nz + (size_t)z * NP;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int q = r * NTRX + tr;
+    if (tr < NTRX && q < S::NPAIR && j < X1) {
+      const int y1 = 2 * q, y2 = y1 + 1;
+#pragma unroll
+      for (int t = 0; t < X2; t++) {
+        rp_out[(size_t)y1 * NX + j + t * X1] = acc[r][2 * t];
+        if (y2 < NY) rp_out[(size_t)y2 * NX + j + t * X1] = acc[r][2 * t + 1];
+      }
+    }
+  }
+}
+
+// rho accumulation fits the R->G pass's registers: at most two rounds of
+// radix-10 pair rows
+template <int X1, int X2, int NT>
+constexpr bool r2g_rho_ok() {
+  return PS<X1, X2, X2, X1, NT>::ROUNDS * X2 <= 20;
 }
 
 // ------------------------------------------------------------------ launch
 
-enum PlaneKind { G2R = 0, R2G = 1, G2R_PSIR = 2 };
+enum PlaneKind { G2R = 0, R2G = 1, G2R_PSIR = 2, R2G_RHO = 3 };
 
 constexpr int kPlaneThreads = 256;
 
@@ -413,7 +452,7 @@ bool g2r_wide(int npx) {
 // Launch facts of one size for the host's chunk choice: row parts of the
 // G->R grid and resident workgroups per CU of both kernels.
 template <typename T, int X1, int X2, int Y1, int Y2>
-hipError_t plane3_info(int npx, int* rowparts, int* occ_g2r, int* occ_r2g) {
+hipError_t plane3_info(int npx, int* rowparts, int* occ_g2r, int* occ_r2g, int* rho_ok) {
   constexpr int NT = kPlaneThreads;
   using S = PS<X1, X2, Y1, Y2, NT>;
   constexpr bool VREG = sizeof(T) == 4 && S::ROUNDS * 2 * X2 <= 48;
@@ -422,7 +461,8 @@ hipError_t plane3_info(int npx, int* rowparts, int* occ_g2r, int* occ_r2g) {
   *rowparts = wide ? 1 : (S::ROUNDS + row_rounds<T>() - 1) / row_rounds<T>();
   const void* g = wide ? (const void*)k3_g2r<T, X1, X2, Y1, Y2, NT, S::ROUNDS, false>
                        : (const void*)k3_g2r<T, X1, X2, Y1, Y2, NT, row_rounds<T>(), false>;
-  const void* r = (const void*)k3_r2g<T, X1, X2, Y1, Y2, NT, VREG>;
+  const void* r = (const void*)k3_r2g<T, X1, X2, Y1, Y2, NT, VREG, false>;
+  *rho_ok = r2g_rho_ok<X1, X2, NT>() ? 1 : 0;
   hipError_t e;
   if (smem > 65536) {
     if ((e = hipFuncSetAttribute(g, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem)) !=
@@ -447,7 +487,7 @@ hipError_t launch_plane3(int kind, const FftCtx& c, hipStream_t st) {
   const size_t smem = plane3_smem<T, X1, X2, Y1, Y2, NT>(c.g.nplane_x);
   if (c.g.nx != S::NX || c.g.ny != S::NY) return hipErrorInvalidValue;
   // raise the dynamic LDS limit of an instantiation once per larger request
-  static size_t granted[3] = {65536, 65536, 65536};
+  static size_t granted[4] = {65536, 65536, 65536, 65536};
   static size_t granted_wide = 65536;
   auto grant_fn = [&](const void* fn, size_t& have) -> hipError_t {
     if (smem <= have) return hipSuccess;
@@ -457,7 +497,7 @@ hipError_t launch_plane3(int kind, const FftCtx& c, hipStream_t st) {
     return e;
   };
   auto grant = [&](const void* fn) -> hipError_t {
-    if (kind < 0 || kind > 2) return hipSuccess;
+    if (kind < 0 || kind > 3) return hipSuccess;
     return grant_fn(fn, granted[kind]);
   };
   hipError_t e = hipSuccess;
@@ -481,7 +521,14 @@ hipError_t launch_plane3(int kind, const FftCtx& c, hipStream_t st) {
       break;
     }
     case R2G: {
-      auto fn = k3_r2g<T, X1, X2, Y1, Y2, NT, VREG>;
+      auto fn = k3_r2g<T, X1, X2, Y1, Y2, NT, VREG, false>;
+      if ((e = grant((const void*)fn)) != hipSuccess) return e;
+      hipLaunchKernelGGL(fn, dim3(c.g.nz, c.nchunk_r), dim3(NT), smem, st, c);
+      break;
+    }
+    case R2G_RHO: {
+      if (!r2g_rho_ok<X1, X2, NT>()) return hipErrorInvalidValue;
+      auto fn = k3_r2g<T, X1, X2, Y1, Y2, NT, VREG, r2g_rho_ok<X1, X2, NT>()>;
       if ((e = grant((const void*)fn)) != hipSuccess) return e;
       hipLaunchKernelGGL(fn, dim3(c.g.nz, c.nchunk_r), dim3(NT), smem, st, c);
       break;

@@ -167,6 +167,9 @@ struct oa_ctx {
   bool have_S = false, have_T = false, have_v = false;
   bool ns_coop = false;  // Newton-Schulz as one cooperative launch
   int engine = 0;     // kEngFft2, kEngSep, kEngFused or kEngPlane
+  // plane engine with V fixed during the step: rho partials come from the
+  // R->G pass (psi(r) is recomputed there anyway), no G->R plane pass
+  bool fuse_rho = false;
   // graph mode: the two coefficient buffers, and per parity (which buffer
   // holds psi) the chain before Newton-Schulz and the one after it
   double2* psi_buf[2] = {nullptr, nullptr};
@@ -194,6 +197,7 @@ constexpr int kEngFft2 = 0;   // two-factor register engine, one kernel per pass
 constexpr int kEngSep = 1;    // Stockham LDS kernels, one kernel per pass
 constexpr int kEngFused = 2;  // Stockham LDS, y and x passes fused per plane
 constexpr int kEngPlane = 3;  // two-factor engine, y and x fused per plane, psi(r) recomputed
+constexpr int kMaxRhoChunks = 16;  // rho partials (N doubles each) of one plane pass
 }  // namespace
 
 namespace {
@@ -397,18 +401,24 @@ oa_status build_layout(oa_ctx* c) {
   if (c->engine == kEngPlane) {
     // State chunks of both plane passes from the resident-workgroup count
     // (every G->R chunk also writes one rho partial of N doubles)
-    int rowparts = 1, occ_g = 1, occ_r = 1, ncu = 0;
-    if (oa::plane3_info(c->nx, c->nplane_x, c->fp32, &rowparts, &occ_g, &occ_r) != hipSuccess)
+    int rowparts = 1, occ_g = 1, occ_r = 1, rho_ok = 0, ncu = 0;
+    if (oa::plane3_info(c->nx, c->nplane_x, c->fp32, &rowparts, &occ_g, &occ_r, &rho_ok) !=
+        hipSuccess)
       return fail(c, OA_EHIP, "plane pass occupancy query failed");
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) !=
             hipSuccess || ncu <= 0)
       ncu = 256;
-    pick_chunks(c->nz * rowparts, std::max(1, occ_g) * ncu, c->nst, 16, f.cs, f.nchunk);
-    pick_chunks(c->nz, std::max(1, occ_r) * ncu, c->nst, c->nst, f.cs_r, f.nchunk_r);
+    // fused rho: every R->G chunk then writes one rho partial too, so its
+    // chunk count is bounded like the G->R one's
+    c->fuse_rho = rho_ok && !c->opt.vks_from_rho;
+    const int max_r = c->fuse_rho ? std::min(c->nst, kMaxRhoChunks) : c->nst;
+    pick_chunks(c->nz * rowparts, std::max(1, occ_g) * ncu, c->nst, kMaxRhoChunks, f.cs,
+                f.nchunk);
+    pick_chunks(c->nz, std::max(1, occ_r) * ncu, c->nst, max_r, f.cs_r, f.nchunk_r);
     c->nzblk = (c->nlines + oa::fft2_ntr(c->nz) - 1) / oa::fft2_ntr(c->nz);
     if (c->opt.autotune) {
-      c->tune_nchunk = chunk_candidates(c->nst, f.nchunk, 16);
-      c->tune_nchunk_r = chunk_candidates(c->nst, f.nchunk_r, c->nst);
+      c->tune_nchunk = chunk_candidates(c->nst, f.nchunk, kMaxRhoChunks);
+      c->tune_nchunk_r = chunk_candidates(c->nst, f.nchunk_r, max_r);
     }
   }
   if (c->engine == kEngFused) {
@@ -554,12 +564,12 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
   TRY(dalloc(c, &c->force, gsz));
   const size_t zsz = (size_t)c->nst * c->nz * c->nplane_x * c->ny * csz;
   TRY(dalloc(c, &c->zbuf, zsz));
-  if (c->engine != kEngSep)
-    TRY(dalloc(c, &c->fc.rho_part,
-               (size_t)std::max(c->fc.nchunk, c->tune_nchunk.empty()
-                                                   ? 0
-                                                   : c->tune_nchunk.back()) *
-                   N * sizeof(double)));
+  if (c->engine != kEngSep) {
+    int np = std::max(c->fc.nchunk, c->tune_nchunk.empty() ? 0 : c->tune_nchunk.back());
+    if (c->fuse_rho)
+      np = std::max({np, c->fc.nchunk_r, c->tune_nchunk_r.empty() ? 0 : c->tune_nchunk_r.back()});
+    TRY(dalloc(c, &c->fc.rho_part, (size_t)np * N * sizeof(double)));
+  }
   if (c->engine == kEngPlane) {
     // psi(r) stays on chip; one state's worth for oa_psir_get
     TRY(dalloc(c, &c->zout, zsz));
@@ -730,6 +740,36 @@ oa_status st_g2r(oa_ctx* c, int ev0 = -1) {
   return OA_OK;
 }
 
+// The EKE and E_loc reductions at the end of the R->G transform.
+oa_status st_energies(oa_ctx* c) {
+  const size_t N = (size_t)c->nx * c->ny * c->nz;
+  HIPCHK(oa::reduce_to(c->eke_part, c->nst * c->nzblk, 1.0, &c->res->eke, c->stream));
+  if (!c->opt.vks_from_rho)  // else E_ext came with V_KS
+    HIPCHK(oa::reduce_to(c->dpart, c->ndpart, c->vol / (double)N, &c->res->eloc, c->stream));
+  return OA_OK;
+}
+
+// Plane engine, fused rho (V fixed during the step): z forward, the R->G
+// plane pass with the rho partials of its chunks, the rho reduction, z
+// back.  Marks ev0+0..5 (1 and 2 empty), then the energy reductions.
+oa_status st_g2r2g_rho(oa_ctx* c, int ev0 = -1) {
+  auto mk = [&](int k) { mark(c, ev0 < 0 ? -1 : ev0 + k); };
+  HIPCHK(oa::fft2_launch(oa::FFT_ZFWD, c->nz, c->fp32, fctx(c), c->stream));
+  mk(0);
+  mk(1);
+  mk(2);
+  FftCtx f = fctx(c);
+  HIPCHK(oa::plane3_launch(3, c->nx, c->fp32, f, c->stream));
+  mk(3);
+  f.nchunk = f.nchunk_r;  // the partials are the R->G chunks'
+  HIPCHK(oa::rho_reduce(f, c->dpart, c->ndpart, c->stream));
+  mk(4);
+  f.zbuf = c->zout;
+  HIPCHK(oa::fft2_launch(oa::FFT_ZBWD, c->nz, c->fp32, f, c->stream));
+  mk(5);
+  return st_energies(c);
+}
+
 // Two-factor / separate: x r2c, y, z.  Fused: plane backward, z.
 // Marks ev0+3..5, then the EKE and E_loc reductions.
 oa_status st_r2g(oa_ctx* c, int ev0 = -1) {
@@ -763,10 +803,7 @@ oa_status st_r2g(oa_ctx* c, int ev0 = -1) {
     }
     HIPCHK(oa::dot_parts(c->vks, c->rho, N, c->dpart, c->ndpart, c->stream));
   }
-  HIPCHK(oa::reduce_to(c->eke_part, c->nst * c->nzblk, 1.0, &c->res->eke, c->stream));
-  if (!c->opt.vks_from_rho)  // else E_ext came with V_KS
-    HIPCHK(oa::reduce_to(c->dpart, c->ndpart, c->vol / (double)N, &c->res->eloc, c->stream));
-  return OA_OK;
+  return st_energies(c);
 }
 
 oa_status st_gram(oa_ctx* c, const double2* A, const double2* B, bool sym, double wlo, double whi,
@@ -938,12 +975,13 @@ double time_stage(oa_ctx* c, hipEvent_t e0, hipEvent_t e1, F&& fn) {
 }
 
 // Try each candidate value of one knob (set(v) applies it), keep the
-// fastest, log the times.  The data in the buffers is whatever they hold
-// (zeros at creation): the kernels' time does not depend on it.
+// fastest, log the times; returns the best time (ms), or -1 when there was
+// no choice to make.  The data in the buffers is whatever they hold (zeros
+// at creation): the kernels' time does not depend on it.
 template <typename Set, typename Run>
-void tune_knob(oa_ctx* c, hipEvent_t e0, hipEvent_t e1, const char* name,
-               const std::vector<int>& cand, int current, Set&& set, Run&& run) {
-  if (cand.size() < 2) return;
+double tune_knob(oa_ctx* c, hipEvent_t e0, hipEvent_t e1, const char* name,
+                 const std::vector<int>& cand, int current, Set&& set, Run&& run) {
+  if (cand.size() < 2) return -1.0;
   int best = current;
   double tbest = 1e300;
   std::string line = std::string(name) + ":";
@@ -961,6 +999,7 @@ void tune_knob(oa_ctx* c, hipEvent_t e0, hipEvent_t e1, const char* name,
   set(best);
   line += " -> " + std::to_string(best) + (best == current ? " (heuristic)\n" : "\n");
   c->tune_log += line;
+  return tbest;
 }
 
 // The measured choice of launch shapes (opt.autotune).
@@ -986,15 +1025,45 @@ oa_status autotune(oa_ctx* c) {
           return OA_OK;
         });
     tune_knob(
-        c, e0, e1, "R->G state chunks", c->tune_nchunk_r, f.nchunk_r,
+        c, e0, e1, c->fuse_rho ? "R->G (+rho) state chunks" : "R->G state chunks",
+        c->tune_nchunk_r, f.nchunk_r,
         [&](int v) {
           f.cs_r = (c->nst + v - 1) / v;
           f.nchunk_r = (c->nst + f.cs_r - 1) / f.cs_r;
         },
         [&]() -> oa_status {
-          HIPCHK(oa::plane3_launch(1, c->nx, c->fp32, fctx(c), c->stream));
+          HIPCHK(oa::plane3_launch(c->fuse_rho ? 3 : 1, c->nx, c->fp32, fctx(c), c->stream));
+          if (c->fuse_rho) {
+            FftCtx fr = fctx(c);
+            fr.nchunk = fr.nchunk_r;
+            HIPCHK(oa::rho_reduce(fr, c->dpart, c->ndpart, c->stream));
+          }
           return OA_OK;
         });
+    if (c->fuse_rho) {
+      // the fused pass against the separate G->R and R->G passes, each at
+      // its tuned chunking (the separate R->G pass at the fused one's)
+      const double t_fused = time_stage(c, e0, e1, [&]() -> oa_status {
+        FftCtx fr = fctx(c);
+        HIPCHK(oa::plane3_launch(3, c->nx, c->fp32, fr, c->stream));
+        fr.nchunk = fr.nchunk_r;
+        return oa::rho_reduce(fr, c->dpart, c->ndpart, c->stream) == hipSuccess ? OA_OK
+                                                                             : OA_EHIP;
+      });
+      const double t_sep = time_stage(c, e0, e1, [&]() -> oa_status {
+        HIPCHK(oa::plane3_launch(0, c->nx, c->fp32, fctx(c), c->stream));
+        HIPCHK(oa::rho_reduce(fctx(c), c->dpart, c->ndpart, c->stream));
+        HIPCHK(oa::plane3_launch(1, c->nx, c->fp32, fctx(c), c->stream));
+        return OA_OK;
+      });
+      char buf[128];
+      std::snprintf(buf, sizeof(buf),
+                    "rho in the R->G pass (1) or a G->R pass (0): 1=%.4fms 0=%.4fms -> %d "
+                    "(heuristic 1)\n",
+                    t_fused, t_sep, (t_fused >= 0.0 && t_sep >= 0.0 && t_sep < t_fused) ? 0 : 1);
+      c->tune_log += buf;
+      if (t_fused >= 0.0 && t_sep >= 0.0 && t_sep < t_fused) c->fuse_rho = false;
+    }
   }
   tune_knob(
       c, e0, e1, "S K slices", c->tune_nslice_sym, c->nslice_sym,
@@ -1196,7 +1265,7 @@ oa_status oa_get_config(const oa_ctx* c, int cfg[8]) {
   const FftCtx& f = c->fc;
   const bool fused = c->engine == kEngFused, plane = c->engine == kEngPlane;
   const bool fft2 = c->engine == kEngFft2 || plane;
-  cfg[0] = c->engine;
+  cfg[0] = c->fuse_rho ? 4 : c->engine;
   cfg[1] = fft2 ? oa::fft2_ntr(c->nz) : f.lpb;
   cfg[2] = fft2 ? oa::fft2_ntr(c->ny) : f.kc;
   cfg[3] = fft2 ? oa::fft2_ntr(c->nx) : f.npair;
@@ -1444,8 +1513,12 @@ oa_status enqueue_pre(oa_ctx* c, bool marks) {
   auto mk = [&](int i) {
     if (marks) mark(c, i);
   };
-  TRY(st_g2r(c, marks ? 1 : -1));  // marks 1..3
-  TRY(st_r2g(c, marks ? 1 : -1));  // marks 4..6, then the EKE / E_loc reductions
+  if (c->fuse_rho) {
+    TRY(st_g2r2g_rho(c, marks ? 1 : -1));  // marks 1..6, then the EKE / E_loc reductions
+  } else {
+    TRY(st_g2r(c, marks ? 1 : -1));  // marks 1..3
+    TRY(st_r2g(c, marks ? 1 : -1));  // marks 4..6, then the EKE / E_loc reductions
+  }
   if (c->opt.use_cg) {
     TRY(st_lambda_cg(c));
     mk(7);

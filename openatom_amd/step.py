@@ -29,7 +29,8 @@ class CPContext:
     # engine codes reported by oa_get_config (fft_mode option: 0 auto, 1-4 force
     # Stockham per pass, Stockham fused, two-factor per pass, two-factor fused)
     ENGINES = {0: "two-factor register FFT, one kernel per pass", 1: "Stockham per pass",
-               2: "Stockham fused planes", 3: "two-factor fused y+x planes"}
+               2: "Stockham fused planes", 3: "two-factor fused y+x planes",
+               4: "two-factor fused y+x planes, rho from the R->G pass (fixed V)"}
 
     def __init__(self, grid, hmat, ecut_ry, nstates, kx=None, ky=None, kz=None,
                  occ=None, precision="fp64", device=0, **opts):
@@ -250,6 +251,9 @@ class CPContext:
     TIMING_NAMES_PLANE = ("k_zfwd", "k_plane_g2r", "k_rho_reduce", "k_plane_r2g", "none",
                           "k_zbwd", "reduce_lambda", "sd_sync", "overlap", "ns", "rotate",
                           "resid", "step")
+    TIMING_NAMES_PLANE_RHO = ("k_zfwd", "none", "none", "k_plane_r2g_rho", "k_rho_reduce",
+                              "k_zbwd", "reduce_lambda", "sd_sync", "overlap", "ns", "rotate",
+                              "resid", "step")
     TIMING_NAMES_FUSED = ("k_zfwd", "k_plane_fwd", "k_rho_reduce", "k_plane_bwd", "none",
                           "k_zbwd", "reduce_lambda", "sd_sync", "overlap", "ns", "rotate",
                           "resid", "step")
@@ -257,8 +261,8 @@ class CPContext:
     def timings(self):
         """Per-stage device ms averaged over the timed steps (HIP events on the
         context's stream); names follow the FFT mode in use."""
-        names = {2: self.TIMING_NAMES_FUSED, 3: self.TIMING_NAMES_PLANE}.get(
-            self.config["engine"], self.TIMING_NAMES)
+        names = {2: self.TIMING_NAMES_FUSED, 3: self.TIMING_NAMES_PLANE,
+                 4: self.TIMING_NAMES_PLANE_RHO}.get(self.config["engine"], self.TIMING_NAMES)
         if self.opts.vks_from_rho:  # stage 2 also builds V_KS from rho
             names = names[:2] + (names[2] + "+vks",) + names[3:]
         ms = np.zeros(len(names))
