@@ -109,6 +109,9 @@ typedef struct {
   double ehart;        /* vks_from_rho: Hartree energy (cp_hart_ext.C:207, 385), else 0 */
   double exc;          /* vks_from_rho: PZ-LDA exchange-correlation energy
                           (cp_pz_lda.C:143-167), else 0 */
+  double enl;          /* non-local KB energy when oa_nonlocal_set was called
+                          (CP_enl_atm_forc_calc, cp_nl_energy_forc.C:196-237),
+                          else 0; included in eelec */
 } oa_step_out;
 
 /* Local pseudopotential of fixed ions for a vks_from_rho context: per-species
@@ -125,6 +128,28 @@ typedef struct {
   const double *vps;     /* [nspecies][4][nspline]: rows c0, c1, c2, c3 */
   const double *vps_g0;  /* [nspecies] */
 } oa_local_pseudo;
+
+/* Non-local Kleinman-Bylander pseudopotential of fixed ions, l = 0 channel,
+ * by the reference's G-space method (its default, cp_nonloc_ees_opt off;
+ * that method supports l = 0 only, control_vps_params.C:1776-1792).  The
+ * radial projectors P_t(|G|) use the spline layout of oa_local_pseudo
+ * (get_rad_proj, cp_struct_fact.C:268-300; proj_g0 is the G = 0 value,
+ * gzvps0), vnorm_t the KB normalisation (vnorm_0).  With
+ * SF_I(G) = P_t(|G|) exp(-i G.R_I) / sqrt(4 pi) and w = 1 on kx = 0, 2
+ * elsewhere, per state s: Z_sI = sum_stored w Re(psi_s conj SF_I),
+ * E_NL = sum_s occ_s sum_I (vnorm_t/Omega) Z_sI^2,
+ * F_s -= w sum_I (2 vnorm_t/Omega) Z_sI SF_I and, for the ions,
+ * f_I -= sum_s (2 vnorm_t/Omega) Z_sI dZ_sI/dR_I; as in the reference, the
+ * occupation weights the energy but not the two forces. */
+typedef struct {
+  int natoms, nspecies, nspline;
+  const double *pos;     /* [natoms][3] Cartesian positions of the KB atoms, bohr */
+  const int *species;    /* [natoms] in 0 .. nspecies-1 */
+  double gmin, dg;
+  const double *proj;    /* [nspecies][4][nspline]: rows c0, c1, c2, c3 */
+  const double *proj_g0; /* [nspecies] */
+  const double *vnorm;   /* [nspecies] */
+} oa_nonlocal;
 
 /* default options: FP64, ns_tol 1e-15, ns_max_iter 30, dt 5, tau 25 fs,
  * ecut_mass 2, Lambda on, residual on */
@@ -233,6 +258,20 @@ oa_status oa_step(oa_ctx *ctx, oa_step_out *out);
  * CP_exc_calc (cp_pz_lda.C:23-170); its energies land in the next
  * oa_step_result (ehart, exc, eloc = E_ext).  oa_vks_get copies V(r) out. */
 oa_status oa_local_pseudo_set(oa_ctx *ctx, const oa_local_pseudo *lp);
+
+/* Set (or replace) the non-local atoms of a context: builds the projector
+ * table once on the device (CP_calc_Struct_Fact, cp_struct_fact.C:38-260,
+ * which the reference's StructFactCache holds per plane); every following
+ * oa_step adds E_NL to eelec and the non-local force to F before Lambda,
+ * where the reference's ParticlePlane chares join doneNLForces
+ * (CP_State_ParticlePlane.C:500-790: computeZ, reduceZ, getForces).
+ * oa_nonlocal_get evaluates, for the current coefficients, Z ([nstates]
+ * [natoms], may be NULL), E_NL (may be NULL) and the ion forces
+ * ([natoms][3], may be NULL); oa_nonlocal_force adds the non-local force to
+ * the context's F (after oa_vpsi_r2g, before oa_lambda). */
+oa_status oa_nonlocal_set(oa_ctx *ctx, const oa_nonlocal *nl);
+oa_status oa_nonlocal_get(oa_ctx *ctx, double *z, double *enl, double *fatoms);
+oa_status oa_nonlocal_force(oa_ctx *ctx);
 
 /* What the autotuner measured and chose (one line per knob; empty when
  * opt.autotune was 0).  No counterpart in the reference, whose launch

@@ -25,6 +25,7 @@ EXPORTS = (
     "oa_force_set", "oa_lambda", "oa_sd_update", "oa_overlap", "oa_ortho_ns",
     "oa_tmat_set", "oa_rotate", "oa_step", "oa_residual", "oa_sync", "oa_step_result", "oa_timings",
     "oa_set_timing", "oa_local_pseudo_set", "oa_vks_build", "oa_vks_get", "oa_tuning_log",
+    "oa_nonlocal_set", "oa_nonlocal_get", "oa_nonlocal_force",
     # state files (oa_state.h)
     "oa_state_read", "oa_state_write", "oa_states_read", "oa_states_write",
     # request loop (oa_serve.h)
@@ -56,7 +57,8 @@ class OaStepOut(ctypes.Structure):
                 ("eelec", ctypes.c_double), ("fnorm", ctypes.c_double),
                 ("ns_iters", ctypes.c_int), ("ns_err", ctypes.c_double),
                 ("resid_max", ctypes.c_double), ("resid_rms", ctypes.c_double),
-                ("ms", ctypes.c_double), ("ehart", ctypes.c_double), ("exc", ctypes.c_double)]
+                ("ms", ctypes.c_double), ("ehart", ctypes.c_double), ("exc", ctypes.c_double),
+                ("enl", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -67,6 +69,13 @@ class OaLocalPseudo(ctypes.Structure):
                 ("pos", ctypes.c_void_p), ("species", ctypes.c_void_p),
                 ("gmin", ctypes.c_double), ("dg", ctypes.c_double), ("vps", ctypes.c_void_p),
                 ("vps_g0", ctypes.c_void_p)]
+
+
+class OaNonlocal(ctypes.Structure):
+    _fields_ = [("natoms", ctypes.c_int), ("nspecies", ctypes.c_int), ("nspline", ctypes.c_int),
+                ("pos", ctypes.c_void_p), ("species", ctypes.c_void_p),
+                ("gmin", ctypes.c_double), ("dg", ctypes.c_double), ("proj", ctypes.c_void_p),
+                ("proj_g0", ctypes.c_void_p), ("vnorm", ctypes.c_void_p)]
 
 
 class OaError(RuntimeError):

@@ -226,7 +226,7 @@ __device__ __forceinline__ void gram_full_wave(const GramArgs& a, const double* 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, l16 = lane & 15, g = lane >> 4;
   const int wr = wave >> 1, wc = wave & 1;
   const int rbase = bi * kGB + wr * 64, cbase = bj * kGB + wc * 64;
-  if (rbase >= a.n || cbase >= a.n) return;
+  if (rbase >= a.n || cbase >= a.nb) return;
   int ra[4], rb[4];
   bool va[4], vb[4];
 #pragma unroll
@@ -234,7 +234,7 @@ __device__ __forceinline__ void gram_full_wave(const GramArgs& a, const double* 
     ra[x] = rbase + 16 * x + l16;
     rb[x] = cbase + 16 * x + l16;
     va[x] = ra[x] < a.n;
-    vb[x] = rb[x] < a.n;
+    vb[x] = rb[x] < a.nb;
   }
   v4d acc[4][4];
 #pragma unroll
@@ -262,7 +262,7 @@ __device__ __forceinline__ void gram_full_wave(const GramArgs& a, const double* 
 #pragma unroll
             for (int y = 0; y < 4; y++) acc[x][y] = mfma16(d.f[x][e], d.f[4 + y][e], acc[x][y]);
       });
-  double* part = a.part + (size_t)sp * a.n * a.n;
+  double* part = a.part + (size_t)sp * a.n * a.nb;
 #pragma unroll
   for (int x = 0; x < 4; x++)
 #pragma unroll
@@ -270,13 +270,13 @@ __device__ __forceinline__ void gram_full_wave(const GramArgs& a, const double* 
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const int r = rbase + 16 * x + g + 4 * v, col = cbase + 16 * y + l16;
-        if (r < a.n && col < a.n) part[(size_t)r * a.n + col] = s.w * acc[x][y][v];
+        if (r < a.n && col < a.nb) part[(size_t)r * a.nb + col] = s.w * acc[x][y][v];
       }
 }
 
-// grid (output blocks, K slices); nbe = blocks per edge, nlo = slices of the
-// kx=0 region
-__global__ __launch_bounds__(256) void k_gram(GramArgs a, int nbe, int nlo) {
+// grid (output blocks, K slices); nbe = row blocks (blocks per edge when
+// symmetric), nbe_b = column blocks, nlo = slices of the kx=0 region
+__global__ __launch_bounds__(256) void k_gram(GramArgs a, int nbe, int nbe_b, int nlo) {
   int bi, bj, b = blockIdx.x;
   if (a.sym) {
     bi = 0;
@@ -286,8 +286,8 @@ __global__ __launch_bounds__(256) void k_gram(GramArgs a, int nbe, int nlo) {
     }
     bj = bi + b;
   } else {
-    bi = b / nbe;
-    bj = b - bi * nbe;
+    bi = b / nbe_b;
+    bj = b - bi * nbe_b;
   }
   const int sp = blockIdx.y;
   const GSlice s = gram_slice(a, sp, nlo);
@@ -320,32 +320,34 @@ int gram_slices(int n, int K, int k_lo, int sym, int target) {
 }
 
 hipError_t gram_f64(const GramArgs& a, hipStream_t st) {
-  if (a.nslice < 1 || (a.k_lo > 0 && a.k_lo < a.K && a.nslice < 2) || (a.lda & 3))
+  if (a.nslice < 1 || (a.k_lo > 0 && a.k_lo < a.K && a.nslice < 2) || (a.lda & 3) ||
+      a.n < 1 || a.nb < 1 || (a.sym && a.nb != a.n))
     return hipErrorInvalidValue;
-  const int nbe = (a.n + kGB - 1) / kGB;
-  const int blocks = a.sym ? nbe * (nbe + 1) / 2 : nbe * nbe;
-  hipLaunchKernelGGL(k_gram, dim3(blocks, a.nslice), dim3(256), 0, st, a, nbe, gram_nlo(a));
+  const int nbe = (a.n + kGB - 1) / kGB, nbe_b = (a.nb + kGB - 1) / kGB;
+  const int blocks = a.sym ? nbe * (nbe + 1) / 2 : nbe * nbe_b;
+  hipLaunchKernelGGL(k_gram, dim3(blocks, a.nslice), dim3(256), 0, st, a, nbe, nbe_b,
+                     gram_nlo(a));
   return hipGetLastError();
 }
 
 // C[e] = alpha * sum_sp part[sp][e'] with e' = e or its mirror; 64 outputs
 // per workgroup, 16 groups of slices summed through LDS in a fixed order
 __global__ __launch_bounds__(1024) void k_gram_reduce(const double* __restrict__ part,
-                                                      int nslice, int n, int sym, double alpha,
-                                                      double* __restrict__ C) {
+                                                      int nslice, int n, int nb, int sym,
+                                                      double alpha, double* __restrict__ C) {
   __shared__ double red[16][64];
   const int x = threadIdx.x & 63, y = threadIdx.x >> 6;
-  const size_t nn = (size_t)n * n;
+  const size_t nn = (size_t)n * nb;
   const size_t e = (size_t)blockIdx.x * 64 + x;
   double s0 = 0.0, s1 = 0.0;
   if (e < nn) {
-    int i = (int)(e / n), j = (int)(e - (size_t)i * n);
-    if (sym && (i >> 4) > (j >> 4)) {
+    int i = (int)(e / nb), j = (int)(e - (size_t)i * nb);
+    if (sym && (i >> 4) > (j >> 4)) {  // sym: nb == n
       const int t = i;
       i = j;
       j = t;
     }
-    const double* p = part + (size_t)i * n + j;
+    const double* p = part + (size_t)i * nb + j;
     // thread y sums slices y, y+16, y+32, ...: up to 16 loads in flight
     for (int base = 0; base < nslice; base += 256) {
       double v[16];
@@ -371,16 +373,17 @@ __global__ __launch_bounds__(1024) void k_gram_reduce(const double* __restrict__
   }
 }
 
-hipError_t gram_reduce(const double* part, int nslice, int n, int sym, double alpha, double* C,
-                       hipStream_t st) {
-  const size_t nn = (size_t)n * n;
+hipError_t gram_reduce(const double* part, int nslice, int n, int nb, int sym, double alpha,
+                       double* C, hipStream_t st) {
+  if (sym && nb != n) return hipErrorInvalidValue;
+  const size_t nn = (size_t)n * nb;
   hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)((nn + 63) / 64)), dim3(1024), 0, st, part,
-                     nslice, n, sym, alpha, C);
+                     nslice, n, nb, sym, alpha, C);
   return hipGetLastError();
 }
 
 // ================================================================== rotate
-// Y[j][k] = beta Y[j][k] + s(k) sum_i M[i][j] X[i][k] as D = M^T X: the
+// Y[j][k] = beta Y[j][k] + s(k) sum_{i < ni} M[i][j] X[i][k] as D = M^T X: the
 // output tile rows are j and its columns k, so the stores run along k.
 // Workgroup: 64 columns k x 128 rows j; wave w owns j tiles 2w, 2w+1 of the
 // block.  The i loop runs in chunks of 16 with the next chunk's operands
@@ -416,7 +419,7 @@ __global__ __launch_bounds__(256) void k_rotate(RotArgs a) {
 #pragma unroll
       for (int s = 0; s < 4; s++) {
         const int i = i0 + 4 * s + g;
-        const bool vi = i < a.n;
+        const bool vi = i < a.ni;
 #pragma unroll
         for (int x = 0; x < 2; x++) am[s][x] = (vi && vj[x]) ? a.M[(size_t)i * a.n + jc[x]] : 0.0;
 #pragma unroll
@@ -434,12 +437,12 @@ __global__ __launch_bounds__(256) void k_rotate(RotArgs a) {
     };
     int i0 = 0;
     load(0, am0, bx0);
-    while (i0 < a.n) {
-      if (i0 + 16 < a.n) load(i0 + 16, am1, bx1);
+    while (i0 < a.ni) {
+      if (i0 + 16 < a.ni) load(i0 + 16, am1, bx1);
       mma(am0, bx0);
       i0 += 16;
-      if (i0 >= a.n) break;
-      if (i0 + 16 < a.n) load(i0 + 16, am0, bx0);
+      if (i0 >= a.ni) break;
+      if (i0 + 16 < a.ni) load(i0 + 16, am0, bx0);
       mma(am1, bx1);
       i0 += 16;
     }
@@ -477,8 +480,8 @@ __global__ __launch_bounds__(256) void k_rotate(RotArgs a) {
   }
 }
 
-// n <= 128: persistent workgroups of 8 waves (two per SIMD).  M is staged
-// once per workgroup in LDS (all 128 x 128, zero beyond n, row stride 129);
+// n, ni <= 128: persistent workgroups of 8 waves (two per SIMD).  M is staged
+// once per workgroup in LDS (all 128 x 128, zero beyond ni x n, row stride 129);
 // workgroup b walks the 64-column k-blocks b, b + gridDim.x, ...  Wave w
 // owns the output rows j in [32 (w & 3), +32) and the k tiles 2 (w >> 2),
 // 2 (w >> 2) + 1 of a block.  The work items are (k-block, chunk of 16 i);
@@ -487,7 +490,6 @@ __global__ __launch_bounds__(256) void k_rotate(RotArgs a) {
 // epilogue's operands (Y when beta != 0, psi and dt/m for the SD update)
 // are loaded when a block starts, behind its MFMAs.
 constexpr int kRotRing = 4;
-constexpr int kRotCh = 8;                      // i chunks per k-block (n <= 128)
 constexpr int kRotLd = 129;                    // LDS row stride of M
 constexpr int kRotThreads = 512;
 constexpr size_t kRotSmem = (size_t)128 * kRotLd * sizeof(double);
@@ -507,7 +509,7 @@ __global__ __launch_bounds__(kRotThreads) void k_rotate_p(RotArgs a) {
 #pragma unroll
     for (int u = 0; u < PER; u++) {
       const int e = tid + kRotThreads * u, i = e >> 7, j = e & 127;
-      mv[u] = (i < a.n && j < a.n) ? a.M[(size_t)i * a.n + j] : 0.0;
+      mv[u] = (i < a.ni && j < a.n) ? a.M[(size_t)i * a.n + j] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < PER; u++) {
@@ -521,14 +523,15 @@ __global__ __launch_bounds__(kRotThreads) void k_rotate_p(RotArgs a) {
   const int nkb = (a.K + kRK - 1) / kRK;
   const int nmine =
       (int)blockIdx.x < nkb ? (nkb - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x : 0;
-  const int nitems = nmine * kRotCh;
-  auto kblock = [&](int q) { return (int)blockIdx.x + (q / kRotCh) * (int)gridDim.x; };
+  const int nch = (a.ni + 15) >> 4;  // i chunks of 16 per k-block
+  const int nitems = nmine * nch;
+  auto kblock = [&](int q) { return (int)blockIdx.x + (q / nch) * (int)gridDim.x; };
   auto load = [&](int q, XFrag& d) __attribute__((always_inline)) {
-    const int kb = kblock(q) * kRK, ic = q % kRotCh;
+    const int kb = kblock(q) * kRK, ic = q % nch;
 #pragma unroll
     for (int s = 0; s < 4; s++) {
       const int i = ic * 16 + 4 * s + g;
-      const bool vi = i < a.n;
+      const bool vi = i < a.ni;
 #pragma unroll
       for (int y = 0; y < 2; y++) {
         const int k = kb + 16 * (kt0 + y) + l16;
@@ -592,7 +595,7 @@ __global__ __launch_bounds__(kRotThreads) void k_rotate_p(RotArgs a) {
       constexpr int d = decltype(dc)::value;
       const int q = q0 + d;
       if (q < nitems) {
-        const int ic = q % kRotCh;
+        const int ic = q % nch;
         if (wact) {
 #pragma unroll
           for (int s = 0; s < 4; s++) {
@@ -606,7 +609,7 @@ __global__ __launch_bounds__(kRotThreads) void k_rotate_p(RotArgs a) {
           }
         }
         if (q + kRotRing < nitems) load(q + kRotRing, ring[d]);
-        if (ic == kRotCh - 1) {
+        if (ic == nch - 1) {
           epilogue(kblock(q) * kRK);
           if (q + 1 < nitems) preload(kblock(q + 1) * kRK);
         }
@@ -642,10 +645,13 @@ int rotate_nwg(int n, int K) {
   return n <= 128 ? rotate_pgrid(K) : ((K + kRK - 1) / kRK) * ((n + 127) / 128);
 }
 
-hipError_t rotate_f64(const RotArgs& a, hipStream_t st) {
-  if ((a.sd_psi || a.fpart) && a.n > 128) return hipErrorInvalidValue;
-  if (a.sd_psi && (a.sd_psi != a.X || !a.dtm || !a.fpart)) return hipErrorInvalidValue;
-  if (a.n <= 128) {
+hipError_t rotate_f64(const RotArgs& args, hipStream_t st) {
+  RotArgs a = args;
+  if (a.ni <= 0) a.ni = a.n;
+  if ((a.sd_psi || a.fpart) && (a.n > 128 || a.ni > 128)) return hipErrorInvalidValue;
+  if (a.sd_psi && (a.sd_psi != a.X || !a.dtm || !a.fpart || a.ni != a.n))
+    return hipErrorInvalidValue;
+  if (a.n <= 128 && a.ni <= 128) {
     static bool granted = false;
     if (!granted) {
       const hipError_t e = hipFuncSetAttribute(

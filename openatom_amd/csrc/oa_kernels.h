@@ -84,7 +84,29 @@ struct StepDev {
   int pad;
   double resid_max, resid_rms;
   double ehart, exc, muxc;  // V_KS from rho: Hartree, exchange-correlation, sum V_xc rho dV
+  double enl;               // non-local KB energy
 };
+
+// ---- non-local KB pseudopotential, l = 0 (oa_nonlocal.hip) ----
+struct NlDev {
+  int natoms, nspecies, nspline, npts, ldr;  // ldr: reals per table row (>= 2 npts)
+  double gmin, dg, y00;                      // spline grid; y00 = 1/sqrt(4 pi)
+  double hmati[9];
+  const int *kx, *ky, *kz;  // [npts] stored k-vectors
+  const double* pos;        // [natoms][3]
+  const int* species;       // [natoms]
+  const double* proj;       // [nspecies][4][nspline]
+  const double* proj_g0;    // [nspecies]
+};
+// sf rows q * natoms + I: q = 0 the conjugated structure factor times the
+// projector, q = 1..3 its derivative rows (d/dR_x, y, z); [4 natoms][ldr]
+hipError_t nl_sf_build(const NlDev& d, double* sf, hipStream_t st);
+// Mt[I][s] = 2 vn[I] Z[s][I]; *enl = sum_s occ[s] sum_I vn[I] Z[s][I]^2
+hipError_t nl_energy(const double* Z, int ldz, int n, int natm, const double* vn,
+                     const double* occ, double* Mt, double* enl, hipStream_t st);
+// f[I][q] = -sum_s 2 vn[I] Z[s][I] dZ_q[s][I], Z: [n][4 natm]
+hipError_t nl_atom_forces(const double* Z, int n, int natm, const double* vn, double* f,
+                          hipStream_t st);
 
 // ---- V_KS from rho (oa_vks.hip) ----
 // Dense real 3D transform of rho restricted to the density sphere's box:
@@ -127,6 +149,7 @@ hipError_t vloc_build(const VksArgs& a, const PseudoDev& p, double2* vloc, hipSt
 // for k < k_lo else w_hi.  Every 128 x 128 output block is split into nslice
 // K slices (slices never straddle k_lo).  sym: B == A, only the 16 x 16
 // tiles with i/16 <= j/16 are computed.
+// C[i][j] = sum_k w(k) A[i][k] B[j][k], i < n, j < nb (nb == n when sym)
 struct GramArgs {
   const double* A;
   const double* B;
@@ -134,17 +157,19 @@ struct GramArgs {
   double w_lo, w_hi;
   int sym;
   int nslice;
-  double* part;  // [nslice][n][n]
+  double* part;  // [nslice][n][nb]
+  int nb;        // rows of B
 };
 // K slices per output block for about `target` workgroups in all
 int gram_slices(int n, int K, int k_lo, int sym, int target);
 hipError_t gram_f64(const GramArgs& a, hipStream_t st);
 // C = alpha * sum_sp part[sp]; mirrored from the computed tiles if sym
-hipError_t gram_reduce(const double* part, int nslice, int n, int sym, double alpha,
+hipError_t gram_reduce(const double* part, int nslice, int n, int nb, int sym, double alpha,
                        double* C, hipStream_t st);
 
-// Y[j][k] = beta * Y[j][k] + s(k) * sum_i M[i][j] X[i][k]; s(k) = s_lo for
-// k < k_lo else s_hi.  X, Y: [n][ldx] (Y may not alias X).
+// Y[j][k] = beta * Y[j][k] + s(k) * sum_{i < ni} M[i][j] X[i][k]; s(k) = s_lo
+// for k < k_lo else s_hi.  X: [ni][ldx], Y: [n][ldx] (Y may not alias X),
+// M: [ni][n]; ni = 0 means ni = n.
 // With sd_psi set (n <= 128 only) the epilogue also applies the steepest-
 // descent update to the rows of X: sd_psi[j][k] += dtm[k/2] * Ynew[j][k]
 // (sd_psi == X), and writes per-workgroup partials of sum Ynew^2 to fpart.
@@ -157,6 +182,7 @@ struct RotArgs {
   double* sd_psi;
   const double* dtm;
   double* fpart;
+  int ni;
 };
 hipError_t rotate_f64(const RotArgs& a, hipStream_t st);
 int rotate_nwg(int n, int K);  // workgroups of a rotate launch (fpart entries)

@@ -190,6 +190,15 @@ struct oa_ctx {
   // the largest) and what was measured
   std::vector<int> tune_nchunk, tune_nchunk_r, tune_nslice_sym, tune_nslice_full;
   std::string tune_log;
+  // non-local KB pseudopotential (oa_nonlocal_set): projector table, the
+  // projections Z, their scaled transpose and the gram partials
+  int nl_natm = 0, nl_cap = 0, nl_nslice = 0;  // atoms set, atoms allocated, K slices
+  double* nl_sf = nullptr;    // [4 nl_cap][2 ld]: rows of oa::nl_sf_build
+  double* nl_z = nullptr;     // [nst][4 nl_cap]
+  double* nl_mt = nullptr;    // [nl_cap][nst]
+  double* nl_vn = nullptr;    // [nl_cap] vnorm / Omega per atom
+  double* nl_part = nullptr;  // [nl_nslice][nst][4 nl_cap]
+  double* nl_fatm = nullptr;  // [nl_cap][3]
 };
 
 namespace {
@@ -820,8 +829,9 @@ oa_status st_gram(oa_ctx* c, const double2* A, const double2* B, bool sym, doubl
   g.sym = sym ? 1 : 0;
   g.nslice = sym ? c->nslice_sym : c->nslice_full;
   g.part = c->gpart;
+  g.nb = c->nst;
   HIPCHK(oa::gram_f64(g, c->stream));
-  HIPCHK(oa::gram_reduce(c->gpart, g.nslice, c->nst, g.sym, 1.0, C, c->stream));
+  HIPCHK(oa::gram_reduce(c->gpart, g.nslice, c->nst, c->nst, g.sym, 1.0, C, c->stream));
   return OA_OK;
 }
 
@@ -891,6 +901,53 @@ oa_status st_lambda_cg(oa_ctx* c) {
   return OA_OK;
 }
 
+// Non-local KB projections of psi: Z = Psi SF^T over the stored points with
+// w = 1 on kx = 0 and 2 elsewhere (CP_enl_matrix_calc, cp_nlmat.C:86-118);
+// derivs: also the three derivative row groups (nb = 4 natoms).  Leaves
+// Z in nl_z ([nst][nb]).
+oa_status st_nl_z(oa_ctx* c, bool derivs) {
+  oa::GramArgs g;
+  g.A = reinterpret_cast<const double*>(c->psi);
+  g.B = c->nl_sf;
+  g.n = c->nst;
+  g.nb = derivs ? 4 * c->nl_natm : c->nl_natm;
+  g.lda = 2 * c->ld;
+  g.K = 2 * c->npts;
+  g.k_lo = 2 * c->nkx0;
+  g.w_lo = 1.0;
+  g.w_hi = 2.0;
+  g.sym = 0;
+  g.nslice = c->nl_nslice;
+  g.part = c->nl_part;
+  HIPCHK(oa::gram_f64(g, c->stream));
+  HIPCHK(oa::gram_reduce(c->nl_part, g.nslice, c->nst, g.nb, 0, 1.0, c->nl_z, c->stream));
+  return OA_OK;
+}
+
+// The step's non-local part: Z, E_NL and Z' = 2 vnorm/Omega Z
+// (cp_nl_energy_forc.C:196-216), then F -= w Z' SF
+// (CP_enl_force_calc, cp_nl_energy_forc.C:61-85), no occupation weight on
+// the force, as in the reference.
+oa_status st_nonlocal(oa_ctx* c) {
+  TRY(st_nl_z(c, false));
+  HIPCHK(oa::nl_energy(c->nl_z, c->nl_natm, c->nst, c->nl_natm, c->nl_vn, c->occ, c->nl_mt,
+                       &c->res->enl, c->stream));
+  oa::RotArgs r{};
+  r.X = c->nl_sf;
+  r.M = c->nl_mt;
+  r.Y = reinterpret_cast<double*>(c->force);
+  r.n = c->nst;
+  r.ni = c->nl_natm;
+  r.ldx = 2 * c->ld;
+  r.K = 2 * c->npts;
+  r.k_lo = 2 * c->nkx0;
+  r.s_lo = -1.0;
+  r.s_hi = -2.0;
+  r.beta = 1.0;
+  HIPCHK(oa::rotate_f64(r, c->stream));
+  return OA_OK;
+}
+
 oa_status st_sd(oa_ctx* c) {
   HIPCHK(oa::sd_update(c->psi, c->force, c->dtm, c->nst, c->npts, c->ld, c->fpart, c->nfpart,
                        c->stream));
@@ -934,6 +991,15 @@ oa_status st_rot_psi(oa_ctx* c) {
   return OA_OK;
 }
 
+// Forget the captured step graphs (the next graph step captures anew).
+void drop_graphs(oa_ctx* c) {
+  for (int p = 0; p < 2; p++) {
+    if (c->gpre[p]) (void)hipGraphExecDestroy(c->gpre[p]);
+    if (c->gpost[p]) (void)hipGraphExecDestroy(c->gpost[p]);
+    c->gpre[p] = c->gpost[p] = nullptr;
+  }
+}
+
 oa_status fetch_result(oa_ctx* c, oa_step_out* out) {
   HIPCHK(hipMemcpyAsync(c->res_host, c->res, sizeof(StepDev), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -943,7 +1009,8 @@ oa_status fetch_result(oa_ctx* c, oa_step_out* out) {
     out->eloc = r.eloc;
     out->ehart = c->opt.vks_from_rho ? r.ehart : 0.0;
     out->exc = c->opt.vks_from_rho ? r.exc : 0.0;
-    out->eelec = r.eke + r.eloc + out->ehart + out->exc;
+    out->enl = c->nl_natm > 0 ? r.enl : 0.0;
+    out->eelec = r.eke + r.eloc + out->ehart + out->exc + out->enl;
     out->fnorm = std::sqrt(r.fnorm2);
     out->ns_iters = r.ns_iter < 0 ? -r.ns_iter : r.ns_iter;
     out->ns_err = r.ns_err;
@@ -1233,10 +1300,7 @@ oa_status oa_destroy(oa_ctx* c) {
   if (!c) return OA_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (int p = 0; p < 2; p++) {
-    if (c->gpre[p]) (void)hipGraphExecDestroy(c->gpre[p]);
-    if (c->gpost[p]) (void)hipGraphExecDestroy(c->gpost[p]);
-  }
+  drop_graphs(c);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->res_host) (void)hipHostFree(c->res_host);
   for (auto& e : c->ev)
@@ -1390,6 +1454,117 @@ oa_status oa_vks_get(oa_ctx* c, double* v) {
   return OA_OK;
 }
 
+oa_status oa_nonlocal_set(oa_ctx* c, const oa_nonlocal* nl) {
+  if (!c || !nl) return OA_EINVAL;
+  if (nl->natoms < 1 || nl->nspecies < 1 || nl->nspline < 1 || !(nl->dg > 0.0) || !nl->pos ||
+      !nl->species || !nl->proj || !nl->proj_g0 || !nl->vnorm)
+    return fail(c, OA_EINVAL, "bad non-local pseudopotential description");
+  for (int i = 0; i < nl->natoms; i++)
+    if (nl->species[i] < 0 || nl->species[i] >= nl->nspecies)
+      return fail(c, OA_EINVAL, "non-local atom species out of range");
+  // the wavefunction sphere's largest |G| must lie on the spline grid
+  if (nl->gmin + (nl->nspline - 1) * nl->dg < std::sqrt(c->ecut_ry))
+    return fail(c, OA_EINVAL, "projector table ends below the wavefunction cutoff");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const int na = nl->natoms;
+  if (na > c->nl_cap) {  // buffers for this many atoms (kept until destroy)
+    const int ldr = 2 * c->ld;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) !=
+            hipSuccess || ncu <= 0)
+      ncu = 256;
+    const int ns = oa::gram_slices(std::max(c->nst, 4 * na), 2 * c->npts, 2 * c->nkx0, 0, ncu);
+    TRY(dalloc(c, &c->nl_sf, (size_t)4 * na * ldr * sizeof(double)));
+    TRY(dalloc(c, &c->nl_z, (size_t)c->nst * 4 * na * sizeof(double)));
+    TRY(dalloc(c, &c->nl_mt, (size_t)na * c->nst * sizeof(double)));
+    TRY(dalloc(c, &c->nl_vn, (size_t)na * sizeof(double)));
+    TRY(dalloc(c, &c->nl_part, (size_t)ns * c->nst * 4 * na * sizeof(double)));
+    TRY(dalloc(c, &c->nl_fatm, (size_t)na * 3 * sizeof(double)));
+    c->nl_cap = na;
+    c->nl_nslice = ns;
+  }
+  // the table of this atom set, rows [4 na][2 ld] (padding points zero)
+  const size_t ldr = 2 * (size_t)c->ld;
+  HIPCHK(oa::fill_zero(c->nl_sf, (size_t)4 * na * ldr * sizeof(double), c->stream));
+  const size_t bk = sizeof(int) * c->npts, bpos = sizeof(double) * 3 * na;
+  const size_t bsp = sizeof(int) * na;
+  const size_t bproj = sizeof(double) * 4 * (size_t)nl->nspecies * nl->nspline;
+  const size_t bg0 = sizeof(double) * nl->nspecies;
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t o_ky = up(bk), o_kz = o_ky + up(bk), o_pos = o_kz + up(bk);
+  const size_t o_sp = o_pos + up(bpos), o_proj = o_sp + up(bsp), o_g0 = o_proj + up(bproj);
+  char* tmp = nullptr;
+  HIPCHK(hipMalloc((void**)&tmp, o_g0 + bg0));
+  oa::NlDev d{};
+  d.natoms = na;
+  d.nspecies = nl->nspecies;
+  d.nspline = nl->nspline;
+  d.npts = c->npts;
+  d.ldr = (int)ldr;
+  d.gmin = nl->gmin;
+  d.dg = nl->dg;
+  d.y00 = 1.0 / std::sqrt(4.0 * M_PI);
+  std::copy(c->hmati, c->hmati + 9, d.hmati);
+  d.kx = reinterpret_cast<const int*>(tmp);
+  d.ky = reinterpret_cast<const int*>(tmp + o_ky);
+  d.kz = reinterpret_cast<const int*>(tmp + o_kz);
+  d.pos = reinterpret_cast<const double*>(tmp + o_pos);
+  d.species = reinterpret_cast<const int*>(tmp + o_sp);
+  d.proj = reinterpret_cast<const double*>(tmp + o_proj);
+  d.proj_g0 = reinterpret_cast<const double*>(tmp + o_g0);
+  std::vector<double> vn(na);
+  for (int i = 0; i < na; i++) vn[i] = nl->vnorm[nl->species[i]] / c->vol;
+  hipError_t e = hipMemcpy(tmp, c->kx.data(), bk, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(tmp + o_ky, c->ky.data(), bk, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(tmp + o_kz, c->kz.data(), bk, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(tmp + o_pos, nl->pos, bpos, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(tmp + o_sp, nl->species, bsp, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(tmp + o_proj, nl->proj, bproj, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(tmp + o_g0, nl->proj_g0, bg0, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(c->nl_vn, vn.data(), sizeof(double) * na, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = oa::nl_sf_build(d, c->nl_sf, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(tmp);
+  HIPCHK(e);
+  c->nl_natm = na;
+  drop_graphs(c);  // captured steps lack (or hold the old) non-local stage
+  return OA_OK;
+}
+
+oa_status oa_nonlocal_get(oa_ctx* c, double* z, double* enl, double* fatoms) {
+  if (!c) return OA_EINVAL;
+  if (c->nl_natm < 1) return fail(c, OA_ESTATE, "no non-local pseudopotential set");
+  HIPCHK(hipSetDevice(c->device));
+  const int na = c->nl_natm, nb = fatoms ? 4 * na : na;
+  TRY(st_nl_z(c, fatoms != nullptr));
+  HIPCHK(oa::nl_energy(c->nl_z, nb, c->nst, na, c->nl_vn, c->occ, c->nl_mt, &c->res->enl,
+                       c->stream));
+  if (fatoms) HIPCHK(oa::nl_atom_forces(c->nl_z, c->nst, na, c->nl_vn, c->nl_fatm, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (z) {
+    std::vector<double> h((size_t)c->nst * nb);
+    HIPCHK(hipMemcpy(h.data(), c->nl_z, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int s = 0; s < c->nst; s++)
+      std::copy(h.begin() + (size_t)s * nb, h.begin() + (size_t)s * nb + na, z + (size_t)s * na);
+  }
+  if (fatoms)
+    HIPCHK(hipMemcpy(fatoms, c->nl_fatm, sizeof(double) * 3 * na, hipMemcpyDeviceToHost));
+  TRY(fetch_result(c, nullptr));
+  if (enl) *enl = c->res_host->enl;
+  return OA_OK;
+}
+
+oa_status oa_nonlocal_force(oa_ctx* c) {
+  if (!c) return OA_EINVAL;
+  if (c->nl_natm < 1) return fail(c, OA_ESTATE, "no non-local pseudopotential set");
+  HIPCHK(hipSetDevice(c->device));
+  TRY(st_nonlocal(c));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return OA_OK;
+}
+
 oa_status oa_fft_g2r(oa_ctx* c) {
   if (!c) return OA_EINVAL;
   HIPCHK(hipSetDevice(c->device));
@@ -1519,6 +1694,7 @@ oa_status enqueue_pre(oa_ctx* c, bool marks) {
     TRY(st_g2r(c, marks ? 1 : -1));  // marks 1..3
     TRY(st_r2g(c, marks ? 1 : -1));  // marks 4..6, then the EKE / E_loc reductions
   }
+  if (c->nl_natm > 0) TRY(st_nonlocal(c));  // timed with the Lambda stage
   if (c->opt.use_cg) {
     TRY(st_lambda_cg(c));
     mk(7);

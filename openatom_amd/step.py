@@ -141,6 +141,35 @@ class CPContext:
                                 ptr(keep[2]), ptr(keep[3]))
         self._chk(self._L.oa_local_pseudo_set(self.h, ctypes.byref(lp)))
 
+    def set_nonlocal(self, pos, species, gmin, dg, proj, proj_g0, vnorm):
+        """Non-local KB atoms (l = 0 channel): positions (bohr), species
+        indices, per-species projector splines proj[nspecies][4][nspline],
+        their G = 0 values and normalisations.  Every later step adds E_NL
+        and the non-local force."""
+        keep = [np.ascontiguousarray(pos, np.float64).reshape(-1, 3),
+                np.ascontiguousarray(species, np.int32),
+                np.ascontiguousarray(proj, np.float64),
+                np.ascontiguousarray(proj_g0, np.float64),
+                np.ascontiguousarray(vnorm, np.float64)]
+        nl = _lib.OaNonlocal(keep[0].shape[0], keep[2].shape[0], keep[2].shape[2],
+                             ptr(keep[0]), ptr(keep[1]), float(gmin), float(dg),
+                             ptr(keep[2]), ptr(keep[3]), ptr(keep[4]))
+        self._chk(self._L.oa_nonlocal_set(self.h, ctypes.byref(nl)))
+        self._nl_natoms = keep[0].shape[0]
+
+    def nonlocal_get(self, atom_forces=False):
+        """(Z [nstates][natoms], E_NL[, ion forces [natoms][3]]) of the
+        current coefficients."""
+        z = np.zeros((self.nst, self._nl_natoms))
+        e = ctypes.c_double(0.0)
+        fa = np.zeros((self._nl_natoms, 3)) if atom_forces else None
+        self._chk(self._L.oa_nonlocal_get(self.h, ptr(z), ctypes.byref(e), ptr(fa)))
+        return (z, e.value, fa) if atom_forces else (z, e.value)
+
+    def nonlocal_force(self):
+        """F += the non-local force of the current coefficients."""
+        self._chk(self._L.oa_nonlocal_force(self.h))
+
     def read_states(self, directory, fmt=1, spin=0, kpt=0, bead=0, temper=0):
         """Load every state from the reference's STATES layout
         (directory/Spin.s_Kpt.k_Bead.b_Temper.t/state<i>.out)."""
@@ -265,6 +294,8 @@ class CPContext:
                  4: self.TIMING_NAMES_PLANE_RHO}.get(self.config["engine"], self.TIMING_NAMES)
         if self.opts.vks_from_rho:  # stage 2 also builds V_KS from rho
             names = names[:2] + (names[2] + "+vks",) + names[3:]
+        if getattr(self, "_nl_natoms", 0):  # the non-local part runs before Lambda
+            names = tuple("nonlocal+" + n if n == "reduce_lambda" else n for n in names)
         ms = np.zeros(len(names))
         self._chk(self._L.oa_timings(self.h, ptr(ms), len(ms)))
         out = dict(zip(names, ms.tolist()))

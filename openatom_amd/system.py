@@ -180,26 +180,75 @@ def gaussian_form_factor(A, sigma, g):
     return f, -sigma * sigma * g * f
 
 
+def _spline_grid(ecut_ry, dg):
+    """g_i = i dg up to the density cutoff sqrt(4 ecut_ry), with margin."""
+    gmax = np.sqrt(4.0 * ecut_ry) + 2 * dg
+    nspl = int(np.ceil(gmax / dg)) + 2
+    return np.arange(nspl + 1) * dg, nspl
+
+
+def _hermite_rows(f, d, dg):
+    """Rows c0..c3 of the cubic Hermite interpolant of values f and slopes d
+    on a uniform grid, evaluated as ((c3 h + c2) h + c1) h + c0."""
+    df = (f[1:] - f[:-1]) / dg
+    return np.stack([f[:-1], d[:-1], (3.0 * df - 2.0 * d[:-1] - d[1:]) / dg,
+                     (d[:-1] + d[1:] - 2.0 * df) / (dg * dg)])
+
+
 def local_pseudo_tables(params, ecut_ry, dg=0.002, species_order=("O", "H")):
     """Spline tables of the Gaussian local potential in the layout of PINY's
     vps0..vps3 (cubic Hermite on g_i = i dg up to the density cutoff
     sqrt(4 ecut_ry)): returns dict(gmin, dg, vps[nspecies][4][nspline],
     vps_g0[nspecies], order)."""
-    gmax = np.sqrt(4.0 * ecut_ry) + 2 * dg
-    nspl = int(np.ceil(gmax / dg)) + 2
-    g = np.arange(nspl + 1) * dg
+    g, nspl = _spline_grid(ecut_ry, dg)
     vps = np.zeros((len(species_order), 4, nspl))
     g0 = np.zeros(len(species_order))
     for s, name in enumerate(species_order):
         A, sig = params[name]
         f, d = gaussian_form_factor(A, sig, g)
-        df = (f[1:] - f[:-1]) / dg
-        vps[s, 0] = f[:-1]
-        vps[s, 1] = d[:-1]
-        vps[s, 2] = (3.0 * df - 2.0 * d[:-1] - d[1:]) / dg
-        vps[s, 3] = (d[:-1] + d[1:] - 2.0 * df) / (dg * dg)
+        vps[s] = _hermite_rows(f, d, dg)
         g0[s] = f[0]
     return dict(gmin=0.0, dg=dg, vps=vps, vps_g0=g0, order=tuple(species_order))
+
+
+def kb_params(seed):
+    """Seeded l = 0 Kleinman-Bylander channel for oxygen (the KB species of
+    the water inputs; hydrogen is local only): a Gaussian projector
+    beta(r) = B exp(-r^2 / (2 s^2)) and its normalisation vnorm.  Positive
+    vnorm makes the channel repulsive, as the s channel of an oxygen
+    pseudopotential with a p local channel is.  Returns {"O": (B, s, vnorm)}."""
+    rng = np.random.Generator(np.random.PCG64(seed + 7919))
+    s = float(rng.uniform(0.50, 0.60))
+    return {"O": (float(rng.uniform(0.9, 1.1)) / (2.0 * np.pi * s * s) ** 1.5, s,
+                  float(rng.uniform(15.0, 25.0)))}
+
+
+def nonlocal_tables(params, ecut_ry, dg=0.002, species_order=("O",)):
+    """Spline tables of the KB projectors in the layout of PINY's l = 0
+    vps rows (as local_pseudo_tables): dict(gmin, dg, proj[nspecies][4]
+    [nspline], proj_g0[nspecies], vnorm[nspecies], order)."""
+    g, nspl = _spline_grid(ecut_ry, dg)
+    proj = np.zeros((len(species_order), 4, nspl))
+    g0 = np.zeros(len(species_order))
+    vn = np.zeros(len(species_order))
+    for s, name in enumerate(species_order):
+        B, sig, vnorm = params[name]
+        f, d = gaussian_form_factor(B, sig, g)
+        proj[s] = _hermite_rows(f, d, dg)
+        g0[s] = f[0]
+        vn[s] = vnorm
+    return dict(gmin=0.0, dg=dg, proj=proj, proj_g0=g0, vnorm=vn, order=tuple(species_order))
+
+
+def kb_setup(pos, spec, ecut_ry, seed):
+    """(KB parameters, dict of nonlocal_tables plus the non-local atoms:
+    indices, positions and species indices into the table order)."""
+    kb = kb_params(seed)
+    nlt = nonlocal_tables(kb, ecut_ry)
+    nl_atoms = [i for i, s in enumerate(spec) if s in nlt["order"]]
+    nlt.update(atoms=np.array(nl_atoms, np.int32), pos=np.asarray(pos)[nl_atoms],
+               species=np.array([nlt["order"].index(spec[i]) for i in nl_atoms], np.int32))
+    return kb, nlt
 
 
 def build(workload="water32", seed=1234, nstates=None, kvec_source=None):
@@ -217,7 +266,9 @@ def build(workload="water32", seed=1234, nstates=None, kvec_source=None):
     V, params = gaussian_vloc(pos, spec, hmat, grid, seed)
     tab = local_pseudo_tables(params, w["ecut_ry"])
     species = np.array([tab["order"].index(s) for s in spec], np.int32)
+    kb, nlt = kb_setup(pos, spec, w["ecut_ry"], seed)
     return dict(name=workload if isinstance(workload, str) else "custom", seed=seed,
                 hmat=hmat, grid=grid, ecut_ry=float(w["ecut_ry"]), nstates=nst,
                 natoms=len(spec), nmol=w["nmol"], pos=pos, spec=spec, kx=kx, ky=ky, kz=kz,
-                psi=psi, vks=V, vparams=params, species=species, pseudo=tab)
+                psi=psi, vks=V, vparams=params, species=species, pseudo=tab,
+                kbparams=kb, kb=nlt)

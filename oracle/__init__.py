@@ -75,6 +75,11 @@ def lib():
         L.oo_vloc_g.argtypes = [vp, ctypes.POINTER(LocalPseudo), _dp]
         L.oo_pz_lda.argtypes = [ctypes.c_size_t, _dp, _dp, _dp]
         L.oo_vks.argtypes = [vp, _dp, vp, _dp, ctypes.POINTER(VksEnergies)]
+        L.oo_nl_proj.argtypes = [ctypes.POINTER(NonLocal), c_int, c_double]
+        L.oo_nl_proj.restype = c_double
+        L.oo_nl_zmat.argtypes = [vp, ctypes.POINTER(NonLocal), _dp, _dp, vp]
+        L.oo_nl_apply.argtypes = [vp, ctypes.POINTER(NonLocal), _dp, vp, vp]
+        L.oo_nl_apply.restype = c_double
         L.oo_set_threads.argtypes = [c_int]
         L.oo_get_threads.restype = c_int
         _lib = L
@@ -94,14 +99,16 @@ class StepOpts(ctypes.Structure):
                 ("ns_max_iter", ctypes.c_int), ("use_lambda", ctypes.c_int),
                 ("use_cg", ctypes.c_int), ("cg_state", ctypes.POINTER(CgState)),
                 ("cg", ctypes.c_void_p), ("vks_from_rho", ctypes.c_int),
-                ("vloc", ctypes.c_void_p), ("vks_out", ctypes.c_void_p)]
+                ("vloc", ctypes.c_void_p), ("vks_out", ctypes.c_void_p),
+                ("nl", ctypes.c_void_p)]
 
 
 class StepOut(ctypes.Structure):
     _fields_ = [("eke", ctypes.c_double), ("eloc", ctypes.c_double),
                 ("eelec", ctypes.c_double), ("fnorm", ctypes.c_double),
                 ("ns_iters", ctypes.c_int), ("ns_err", ctypes.c_double),
-                ("ehart", ctypes.c_double), ("exc", ctypes.c_double)]
+                ("ehart", ctypes.c_double), ("exc", ctypes.c_double),
+                ("enl", ctypes.c_double)]
 
 
 class LocalPseudo(ctypes.Structure):
@@ -112,6 +119,16 @@ class LocalPseudo(ctypes.Structure):
                 ("species", ctypes.c_void_p), ("gmin", ctypes.c_double),
                 ("dg", ctypes.c_double), ("vps", ctypes.c_void_p),
                 ("vps_g0", ctypes.c_void_p)]
+
+
+class NonLocal(ctypes.Structure):
+    """oo_nonlocal: non-local atoms, species, the spline tables of the l = 0
+    KB projectors and their normalisations."""
+    _fields_ = [("natoms", ctypes.c_int), ("nspecies", ctypes.c_int),
+                ("nspline", ctypes.c_int), ("pos", ctypes.c_void_p),
+                ("species", ctypes.c_void_p), ("gmin", ctypes.c_double),
+                ("dg", ctypes.c_double), ("proj", ctypes.c_void_p),
+                ("proj_g0", ctypes.c_void_p), ("vnorm", ctypes.c_void_p)]
 
 
 class VksEnergies(ctypes.Structure):
@@ -130,6 +147,24 @@ def local_pseudo(pos, species, gmin, dg, vps, vps_g0):
                      _ptr(keep[0]), _ptr(keep[1]), float(gmin), float(dg),
                      _ptr(keep[2]), _ptr(keep[3]))
     return lp, keep
+
+
+def nonlocal_(pos, species, gmin, dg, proj, proj_g0, vnorm):
+    """NonLocal over numpy arrays; returns (struct, arrays kept alive).
+    proj: [nspecies][4][nspline] spline coefficients c0..c3."""
+    keep = [np.ascontiguousarray(pos, np.float64).reshape(-1, 3),
+            np.ascontiguousarray(species, np.int32),
+            np.ascontiguousarray(proj, np.float64),
+            np.ascontiguousarray(proj_g0, np.float64),
+            np.ascontiguousarray(vnorm, np.float64)]
+    nl = NonLocal(keep[0].shape[0], keep[2].shape[0], keep[2].shape[2],
+                  _ptr(keep[0]), _ptr(keep[1]), float(gmin), float(dg),
+                  _ptr(keep[2]), _ptr(keep[3]), _ptr(keep[4]))
+    return nl, keep
+
+
+def nl_proj(nl, species, g):
+    return lib().oo_nl_proj(ctypes.byref(nl), int(species), float(g))
 
 
 def pz_lda(rho):
@@ -309,6 +344,23 @@ class System:
                      ctypes.byref(e))
         return v.reshape(self.nz, self.ny, self.nx), {k: getattr(e, k) for k, _ in e._fields_}
 
+    def nl_zmat(self, nl, psi, derivs=False):
+        """Z [nst][natoms] of the non-local projection (and dZ/dR as
+        [nst][3][natoms] if derivs)."""
+        p = np.ascontiguousarray(psi, np.complex128)
+        z = np.zeros((self.nst, nl.natoms))
+        dz = np.zeros((self.nst, 3, nl.natoms)) if derivs else None
+        lib().oo_nl_zmat(self.h, ctypes.byref(nl), p.view(np.float64).reshape(-1), z,
+                         _ptr(dz))
+        return (z, dz) if derivs else z
+
+    def nl_apply(self, nl, psi, f=None, fatm=None):
+        """E_NL; the non-local forces are added to f (complex [nst][npts])
+        and fatm ([natoms][3]) when given."""
+        p = np.ascontiguousarray(psi, np.complex128)
+        return lib().oo_nl_apply(self.h, ctypes.byref(nl), p.view(np.float64).reshape(-1),
+                                 _ptr(None if f is None else f.view(np.float64)), _ptr(fatm))
+
     def cg_update(self, cmass, dt, gamma, f, cg, psi):
         """cg = gamma cg + F (|G|^2 <= ecut), psi += dt/m cg, in place."""
         lib().oo_cg_update(self.h, np.ascontiguousarray(cmass, np.float64), float(dt),
@@ -316,14 +368,17 @@ class System:
                            cg.view(np.float64).reshape(-1), psi.view(np.float64).reshape(-1))
 
     def step(self, v, psi, dt=5.0, tau_fs=25.0, ecut_mass=2.0, ns_tol=1e-15,
-             ns_max_iter=100, use_lambda=True, want=(), cg=None, vloc=None):
+             ns_max_iter=100, use_lambda=True, want=(), cg=None, vloc=None, nl=None):
         """One minimisation step; psi (complex [nst][npts]) updated in place.
         cg: None for steepest descent, or a Minimizer from new_cg() whose
         direction and state the step updates (conjugate gradient).
         vloc: None for a fixed V(r) = v, or the half-grid V_loc(G) of
         vloc_g(): V_KS is then built from this step's rho (v is ignored and
-        may be None; want "v" returns the V_KS used)."""
+        may be None; want "v" returns the V_KS used).  nl: None, or the
+        NonLocal of nonlocal_() whose energy and forces join the step's."""
         o = StepOpts(dt, tau_fs, ecut_mass, ns_tol, ns_max_iter, int(use_lambda))
+        if nl is not None:
+            o.nl = ctypes.cast(ctypes.pointer(nl), ctypes.c_void_p)
         vk = None
         if vloc is not None:
             vloc = np.ascontiguousarray(vloc, np.complex128)
@@ -351,7 +406,7 @@ class System:
                 extra[k] = a
         res = dict(eke=out.eke, eloc=out.eloc, eelec=out.eelec, fnorm=out.fnorm,
                    ns_iters=out.ns_iters, ns_err=out.ns_err, ehart=out.ehart, exc=out.exc,
-                   rc=rc)
+                   enl=out.enl, rc=rc)
         res.update(extra)
         return res
 

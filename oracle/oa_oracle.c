@@ -645,6 +645,124 @@ void oo_vloc_g(const oo_sys *s, const oo_local_pseudo *lp, double *vloc_) {
       }
 }
 
+/* ---------------- non-local KB, l = 0 ---------------- */
+
+double oo_nl_proj(const oo_nonlocal *nl, int sp, double g) {
+  if (g == 0.0) return nl->proj_g0[sp];
+  int ns = nl->nspline;
+  int i = (int)((g - nl->gmin) / nl->dg) + 1;
+  if (i > ns) i = ns;
+  if (i < 1) i = 1;
+  double h = g - ((double)(i - 1) * nl->dg + nl->gmin);
+  const double *c = nl->proj + (size_t)sp * 4 * ns;
+  int k = i - 1;
+  return ((c[3 * ns + k] * h + c[2 * ns + k]) * h + c[ns + k]) * h + c[k];
+}
+
+/* G of stored point p (cp_struct_fact.C:160-163) */
+static void nl_gvec(const oo_sys *s, int p, double G[3]) {
+  const double tpi = 2.0 * M_PI, *hi = s->hmati;
+  int a = s->kx[p], b = s->ky[p], c = s->kz[p];
+  G[0] = tpi * (a * hi[0] + b * hi[1] + c * hi[2]);
+  G[1] = tpi * (a * hi[3] + b * hi[4] + c * hi[5]);
+  G[2] = tpi * (a * hi[6] + b * hi[7] + c * hi[8]);
+}
+
+/* sf[I][p] = P y00 exp(-i G.R_I); gq[p][3] = G */
+static void nl_table(const oo_sys *s, const oo_nonlocal *nl, cplx *sf, double *gq) {
+  const double y00 = 1.0 / sqrt(4.0 * M_PI);
+  int np = s->npts, na = nl->natoms;
+#pragma omp parallel for num_threads(oo_get_threads()) schedule(static)
+  for (int p = 0; p < np; p++) {
+    double G[3];
+    nl_gvec(s, p, G);
+    gq[3 * (size_t)p] = G[0];
+    gq[3 * (size_t)p + 1] = G[1];
+    gq[3 * (size_t)p + 2] = G[2];
+    double g = sqrt(G[0] * G[0] + G[1] * G[1] + G[2] * G[2]);
+    for (int ia = 0; ia < na; ia++) {
+      const double *R = nl->pos + 3 * (size_t)ia;
+      double arg = R[0] * G[0] + R[1] * G[1] + R[2] * G[2];
+      double pr = oo_nl_proj(nl, nl->species[ia], g) * y00;
+      sf[(size_t)ia * np + p] = pr * (cos(arg) - sin(arg) * I);
+    }
+  }
+}
+
+void oo_nl_zmat(const oo_sys *s, const oo_nonlocal *nl, const double *psi_, double *z,
+                double *dz) {
+  const cplx *psi = (const cplx *)psi_;
+  int np = s->npts, na = nl->natoms, nst = s->nst;
+  cplx *sf = malloc(sizeof(cplx) * (size_t)na * np);
+  double *gq = malloc(sizeof(double) * 3 * (size_t)np);
+  nl_table(s, nl, sf, gq);
+#pragma omp parallel for num_threads(oo_get_threads()) schedule(static)
+  for (int st = 0; st < nst; st++) {
+    const cplx *ps = psi + (size_t)st * np;
+    for (int ia = 0; ia < na; ia++) {
+      const cplx *f = sf + (size_t)ia * np;
+      double acc = 0.0, d[3] = {0.0, 0.0, 0.0};
+      for (int p = 0; p < np; p++) {
+        double w = s->kx[p] == 0 ? 1.0 : 2.0;
+        /* psi conj(SF) = psi P y00 exp(+i G.R) */
+        cplx t = ps[p] * conj(f[p]);
+        acc += w * creal(t);
+        /* d/dR Re(psi P y00 exp(i G.R)) = -G Im(...) */
+        for (int q = 0; q < 3; q++) d[q] -= w * gq[3 * (size_t)p + q] * cimag(t);
+      }
+      z[(size_t)st * na + ia] = acc;
+      if (dz)
+        for (int q = 0; q < 3; q++) dz[((size_t)st * 3 + q) * na + ia] = d[q];
+    }
+  }
+  free(sf);
+  free(gq);
+}
+
+double oo_nl_apply(const oo_sys *s, const oo_nonlocal *nl, const double *psi, double *f_,
+                   double *fatm) {
+  int np = s->npts, na = nl->natoms, nst = s->nst;
+  double *z = malloc(sizeof(double) * (size_t)nst * na);
+  double *dz = fatm ? malloc(sizeof(double) * (size_t)nst * 3 * na) : NULL;
+  oo_nl_zmat(s, nl, psi, z, dz);
+  double enl = 0.0;
+  for (int st = 0; st < nst; st++) {
+    double e = 0.0;
+    for (int ia = 0; ia < na; ia++) {
+      double norm = nl->vnorm[nl->species[ia]] / s->vol;
+      double zz = z[(size_t)st * na + ia];
+      e += zz * zz * norm;
+      z[(size_t)st * na + ia] = 2.0 * norm * zz; /* Z' */
+    }
+    enl += e * s->occ[st];
+  }
+  if (fatm)
+    for (int st = 0; st < nst; st++)
+      for (int ia = 0; ia < na; ia++)
+        for (int q = 0; q < 3; q++)
+          fatm[3 * (size_t)ia + q] -=
+              z[(size_t)st * na + ia] * dz[((size_t)st * 3 + q) * na + ia];
+  if (f_) {
+    cplx *f = (cplx *)f_;
+    cplx *sf = malloc(sizeof(cplx) * (size_t)na * np);
+    double *gq = malloc(sizeof(double) * 3 * (size_t)np);
+    nl_table(s, nl, sf, gq);
+#pragma omp parallel for num_threads(oo_get_threads()) schedule(static)
+    for (int st = 0; st < nst; st++)
+      for (int p = 0; p < np; p++) {
+        double w = s->kx[p] == 0 ? 1.0 : 2.0;
+        cplx acc = 0.0;
+        for (int ia = 0; ia < na; ia++) acc += z[(size_t)st * na + ia] * sf[(size_t)ia * np + p];
+        f[(size_t)st * np + p] -= w * acc;
+      }
+    free(sf);
+    free(gq);
+  }
+  free(z);
+  free(dz);
+  return enl;
+}
+
 /* cp_pz_lda.C:101-153, per point */
 void oo_pz_lda(size_t n, const double *rho, double *vxc, double sums[3]) {
   const double gamma = -0.14230, beta1 = 1.05290, beta2 = 0.33340;
@@ -793,6 +911,7 @@ int oo_step(const oo_sys *s, const oo_step_opts *o, const double *v, double *psi
   oo_vpsi_r2g_all(s, psir, v, f);
   double eke = oo_eke_force(s, psi, f);
   double eloc = o->vks_from_rho ? ve.eext : oo_eloc(s, v, rho);
+  double enl = o->nl ? oo_nl_apply(s, o->nl, psi, f, NULL) : 0.0;
   if (vk && !o->vks_out) free(vk);
   if (o->use_lambda) {
     double *lam = malloc(sizeof(double) * n * n);
@@ -818,8 +937,8 @@ int oo_step(const oo_sys *s, const oo_step_opts *o, const double *v, double *psi
   memcpy(psi, tmp, sizeof(double) * K * n);
   free(tmp);
   if (out) {
-    out->eke = eke; out->eloc = eloc; out->eelec = eke + eloc + ve.ehart + ve.exc;
-    out->ehart = ve.ehart; out->exc = ve.exc;
+    out->eke = eke; out->eloc = eloc; out->eelec = eke + eloc + ve.ehart + ve.exc + enl;
+    out->ehart = ve.ehart; out->exc = ve.exc; out->enl = enl;
     out->fnorm = sqrt(fn); out->ns_iters = it < 0 ? -it : it; out->ns_err = err;
   }
   free(psir); free(cm);

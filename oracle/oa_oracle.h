@@ -188,6 +188,44 @@ typedef struct { double ehart, exc, eext, muxc; } oo_vks_energies;
 void oo_vks(const oo_sys *s, const double *rho, const double *vloc, double *v,
             oo_vks_energies *e);
 
+/* ---- Non-local Kleinman-Bylander pseudopotential, l = 0 channel ----
+ * The reference's G-space ("N^3") method, its default (cp_nonloc_ees_opt
+ * off, set_sim_dict.C:626-630), which supports only the l = 0 channel
+ * (control_vps_params.C:1776-1792).  Per non-local atom I of species t:
+ *   SF_I(G) = P_t(|G|) y00 exp(-i G.R_I),  y00 = 1/sqrt(4 pi)
+ *             (CP_calc_Struct_Fact, cp_struct_fact.C:150-256: the table is
+ *             built with exp(+i G.R) and conjugated for the double pack)
+ *   Z_sI    = sum_stored w_G Re(psi_s(G) conj SF_I(G)), w = 1 on kx = 0,
+ *             2 elsewhere (CP_enl_matrix_calc, cp_nlmat.C:86-118)
+ *   E_NL    = sum_s occ_s sum_I (vnorm_t/Omega) Z_sI^2
+ *   F_s(G) -= w_G sum_I (2 vnorm_t/Omega) Z_sI SF_I(G)
+ *   f_I    -= sum_s (2 vnorm_t/Omega) Z_sI dZ_sI/dR_I
+ * (CP_enl_atm_forc_calc, cp_nl_energy_forc.C:155-243; CP_enl_force_calc,
+ * cp_nl_energy_forc.C:29-86).  As in the reference, the occupation
+ * weights the energy only, not the two forces.                          */
+typedef struct {
+  int natoms, nspecies, nspline;
+  const double *pos;      /* [natoms][3] Cartesian, bohr            */
+  const int *species;     /* [natoms], 0 .. nspecies-1              */
+  double gmin, dg;        /* spline grid: g_i = gmin + i dg         */
+  const double *proj;     /* [nspecies][4][nspline]: c0, c1, c2, c3 */
+  const double *proj_g0;  /* [nspecies]: P at G = 0 (gzvps0)        */
+  const double *vnorm;    /* [nspecies]: KB normalisation vnorm_0   */
+} oo_nonlocal;
+
+/* Radial projector of one species at |G| = g (get_rad_proj,
+ * cp_struct_fact.C:268-300). */
+double oo_nl_proj(const oo_nonlocal *nl, int species, double g);
+
+/* Z[nst][natoms] and, if dz is not NULL, dZ/dR as dz[nst][3][natoms]. */
+void oo_nl_zmat(const oo_sys *s, const oo_nonlocal *nl, const double *psi, double *z,
+                double *dz);
+
+/* E_NL of psi; f (complex [nst][npts], may be NULL) and fatm ([natoms][3],
+ * may be NULL) receive the non-local forces, added to what they hold. */
+double oo_nl_apply(const oo_sys *s, const oo_nonlocal *nl, const double *psi, double *f,
+                   double *fatm);
+
 typedef struct {
   double dt;          /* SD step (a.u.)                        */
   double tau_fs;      /* cp_mass_tau_def (fs), default 25      */
@@ -201,6 +239,7 @@ typedef struct {
   int vks_from_rho;   /* V = V_KS(rho) of this step (oo_vks)   */
   const double *vloc; /* V_KS only: V_loc(G) half grid or NULL */
   double *vks_out;    /* V_KS only: the step's V(r), or NULL   */
+  const oo_nonlocal *nl;  /* non-local KB part, or NULL        */
 } oo_step_opts;
 
 typedef struct {
@@ -209,12 +248,13 @@ typedef struct {
   int ns_iters;
   double ns_err;
   double ehart, exc;  /* V_KS only (eloc is then E_ext)          */
+  double enl;         /* non-local energy (opts.nl), else 0      */
 } oo_step_out;
 
 /* One minimisation step on the host (gspace.ci:106-190 order; with
  * vks_from_rho, V_KS is built from this step's rho before V psi):
- * G->R, rho, V psi, R->G, EKE + kinetic force, [Lambda], SD, red sync,
- * S, NS, Psi.T.  psi is updated in place; rho/S/T/f may be NULL. */
+ * G->R, rho, V psi, R->G, EKE + kinetic force, [non-local force],
+ * [Lambda], SD, red sync, S, NS, Psi.T.  psi is updated in place; rho/S/T/f may be NULL. */
 int oo_step(const oo_sys *s, const oo_step_opts *o, const double *v,
             double *psi, double *rho, double *S, double *T, double *f,
             oo_step_out *out);

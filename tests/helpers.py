@@ -18,9 +18,11 @@ def small_system(nmol=2, box=(5.0, 5.4, 5.8), ecut=14.0, grid=(24, 27, 30), nsta
     V, params = system.gaussian_vloc(pos, spec, hmat, grid, seed)
     tab = system.local_pseudo_tables(params, ecut)
     species = np.array([tab["order"].index(s) for s in spec], np.int32)
+    kb, nlt = system.kb_setup(pos, spec, ecut, seed)
     return dict(name="small", seed=seed, hmat=hmat, grid=tuple(grid), ecut_ry=ecut,
                 nstates=nstates, natoms=len(spec), kx=kx, ky=ky, kz=kz, psi=psi, vks=V,
-                pos=pos, spec=spec, vparams=params, species=species, pseudo=tab)
+                pos=pos, spec=spec, vparams=params, species=species, pseudo=tab,
+                kbparams=kb, kb=nlt)
 
 
 def oracle_pseudo(s):
@@ -29,6 +31,15 @@ def oracle_pseudo(s):
     t = s["pseudo"]
     return oracle.local_pseudo(s["pos"], s["species"], t["gmin"], t["dg"], t["vps"],
                                t["vps_g0"])
+
+
+def oracle_nonlocal(s, pos=None):
+    """(oracle NonLocal struct, arrays kept alive) of a built system's KB
+    atoms (optionally at other positions)."""
+    import oracle
+    t = s["kb"]
+    return oracle.nonlocal_(t["pos"] if pos is None else pos, t["species"], t["gmin"], t["dg"],
+                            t["proj"], t["proj_g0"], t["vnorm"])
 
 
 def periodic_gaussians(pos, spec, params, hmat, grid, images=2):
