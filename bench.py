@@ -73,10 +73,12 @@ def plane_bytes(stage, geo, csz, nst, launch):
     return 2 * nst * lines + launch["pcp"] * n * 8
 
 
-def run_precision(sysd, precision, steps, warmup, device=0, use_graph=0, vks=True, autotune=1):
+def run_precision(sysd, precision, steps, warmup, device=0, use_graph=0, vks=True, autotune=1,
+                  nonlocal_kb=False):
     """vks: V(r) = V_xc + V_H + V_loc rebuilt from each step's rho on the
     device (the density chares' work inside the step); else the seeded
-    local potential as a fixed input."""
+    local potential as a fixed input.  nonlocal_kb: add the l = 0 KB
+    non-local energy and force of the oxygen atoms to every step."""
     from openatom_amd.step import CPContext
     # The reference step has no ||Psi^H Psi - I|| evaluation (Ortho checks the
     # T iterates); it is measured once after the timed steps instead.
@@ -92,6 +94,10 @@ def run_precision(sysd, precision, steps, warmup, device=0, use_graph=0, vks=Tru
                            t["vps_g0"])
     else:
         c.set_vks(sysd["vks"])
+    if nonlocal_kb:
+        t = sysd["kb"]
+        c.set_nonlocal(t["pos"], t["species"], t["gmin"], t["dg"], t["proj"], t["proj_g0"],
+                       t["vnorm"])
     first = c.step()
     for _ in range(max(0, warmup - 1)):
         c.step(fetch=False)
@@ -309,6 +315,21 @@ def main():
             log(f"[bench] {prec} fixed V(r) {r['ms']:.3f} ms/step")
             fixed[prec] = {"steps_per_s": round(r["steps_per_s"], 3),
                            "ms_per_step": round(r["ms"], 4)}
+    # the same V_KS step with the non-local KB part of the oxygen atoms added
+    with_nl = None
+    if vks and not args.no_graph:
+        with_nl = {}
+        for prec in ("fp64", "mixed"):
+            r = run_precision(sysd, prec, args.steps, args.warmup, vks=True,
+                              autotune=not args.no_autotune, nonlocal_kb=True)
+            log(f"[bench] {prec} with non-local KB {r['ms']:.3f} ms/step")
+            with_nl[prec] = {"steps_per_s": round(r["steps_per_s"], 3),
+                             "ms_per_step": round(r["ms"], 4),
+                             "enl_last": r["last"]["enl"],
+                             "nonlocal_plus_lambda_ms": round(
+                                 r["timings"].get("nonlocal+reduce_lambda", 0.0), 4),
+                             "lambda_only_ms": round(
+                                 runs[(prec, 0)]["timings"].get("reduce_lambda", 0.0), 4)}
     r64, rmx = runs[("fp64", 0)], runs[("mixed", 0)]
     r64v, rmxv = r64, rmx
     graph = {p: round(runs[(p, 1)]["steps_per_s"], 3) for p in ("fp64", "mixed")
@@ -387,6 +408,7 @@ def main():
         "step_launch": "eager launches on one stream",
         "graph_replay_steps_per_s": graph,
         "fixed_v_steps": fixed,
+        "with_nonlocal_kb": with_nl,
         "fp64": {"steps_per_s": round(r64v["steps_per_s"], 3), "ms_per_step": round(r64v["ms"], 4),
                  "ns_iters": r64["last"]["ns_iters"], "resid_max": r64["last"]["resid_max"],
                  "roofline": rl64, "timings_ms": r64["timings"], "launch": r64["launch"],

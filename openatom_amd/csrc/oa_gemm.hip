@@ -125,42 +125,46 @@ __device__ __forceinline__ GSlice gram_slice(const GramArgs& a, int sp, int nlo)
   return s;
 }
 
-// f[e] = X[row][kk + e] (kk a multiple of 4), zero outside [rs, re) or for an
-// invalid row; `full` says the whole 16-wide chunk lies inside [rs, re)
-__device__ __forceinline__ void gfrag(const double* __restrict__ X, int lda, int row, bool vr,
-                                      int kk, bool full, int rs, int re, double (&f)[4]) {
-  if (vr && full) {
-    const double4 v = *reinterpret_cast<const double4*>(X + (size_t)row * lda + kk);
-    f[0] = v.x;
-    f[1] = v.y;
-    f[2] = v.z;
-    f[3] = v.w;
-  } else {
+// f[e] = X[row][kk + e]: one 32-byte load, kk a multiple of 4 inside the
+// row's allocation (chunks start on multiples of 16 below K <= lda, lda a
+// multiple of 16).  Unconditional: rows past the matrix are clamped by the
+// caller (their outputs are never stored) and elements outside the slice's
+// region are zeroed where they are consumed (gmask), so no load sits in a
+// branch and the ring's loads stay in flight.
+__device__ __forceinline__ void gfrag(const double* __restrict__ X, int lda, int row, int kk,
+                                      double (&f)[4]) {
+  const double4 v = *reinterpret_cast<const double4*>(X + (size_t)row * lda + kk);
+  f[0] = v.x;
+  f[1] = v.y;
+  f[2] = v.z;
+  f[3] = v.w;
+}
+
+// zero f[e] for kk + e outside [rs, re)
+__device__ __forceinline__ void gmask(int kk, int rs, int re, double (&f)[4]) {
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-      f[e] = 0.0;
-      if (vr && kk + e >= rs && kk + e < re) f[e] = X[(size_t)row * lda + kk + e];
-    }
-  }
+  for (int e = 0; e < 4; e++) f[e] = (kk + e >= rs && kk + e < re) ? f[e] : 0.0;
 }
 
 // Register ring of D chunk buffers: the loads of chunk c + D are issued as
 // soon as the MFMAs of chunk c have been issued, so D - 1 chunks of MFMAs
 // cover the memory latency.  load(c, buf) fills a buffer for chunk c;
-// mma(buf) issues the chunk's MFMAs.
+// mma(c, buf) issues the chunk's MFMAs.  Loads are issued unconditionally
+// (past the slice they repeat its last chunk): a load that only some paths
+// issue forces the compiler to wait for every outstanding load.
 template <int D, typename Buf, typename L, typename M>
 __device__ __forceinline__ void chunk_ring(int c0, int c1, Buf (&ring)[D], L&& load, M&& mma) {
+  if (c0 >= c1) return;
+  const int cl = c1 - 1;
   static_for<0, D>([&](auto dc) __attribute__((always_inline)) {
     constexpr int d = decltype(dc)::value;
-    if (c0 + d < c1) load(c0 + d, ring[d]);
+    load(min(c0 + d, cl), ring[d]);
   });
   for (int c = c0; c < c1; c += D) {
     static_for<0, D>([&](auto dc) __attribute__((always_inline)) {
       constexpr int d = decltype(dc)::value;
-      if (c + d < c1) {
-        mma(ring[d]);
-        if (c + d + D < c1) load(c + d + D, ring[d]);
-      }
+      if (c + d < c1) mma(c + d, ring[d]);
+      load(min(c + d + D, cl), ring[d]);
     });
   }
 }
@@ -179,33 +183,41 @@ __device__ __forceinline__ void gram_diag_wave(const GramArgs& a, const GSlice& 
   constexpr int NF = dg_nf(W), NTW = dg_nt(W);
   const int lane = threadIdx.x & 63, l16 = lane & 15, g = lane >> 4;
   int row[NF];
-  bool vr[NF];
 #pragma unroll
   for (int f = 0; f < NF; f++) {
-    row[f] = bi * kGB + 16 * dg_rt(W, f) + l16;
-    vr[f] = row[f] < a.n;
+    const int r = bi * kGB + 16 * dg_rt(W, f) + l16;
+    row[f] = r < a.n ? r : 0;  // rows past n: outputs not stored
   }
   v4d acc[NTW];
 #pragma unroll
   for (int t = 0; t < NTW; t++) acc[t] = vzero();
+  auto mfmas = [&](const double (&f)[NF][4]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+      static_for<0, NTW>([&](auto tc) __attribute__((always_inline)) {
+        constexpr int t = decltype(tc)::value;
+        constexpr int ia = dg_a(W, t), ib = dg_b(W, t);
+        acc[t] = mfma16(f[ia][e], f[ib][e], acc[t]);
+      });
+  };
   Frag<NF> ring[kGramRing];
   chunk_ring(
       s.c0, s.c1, ring,
       [&](int c, Frag<NF>& d) __attribute__((always_inline)) {
-        const int k0 = s.base + 16 * c;
-        const bool full = k0 >= s.rs && k0 + 16 <= s.re;
-        const int kk = k0 + 4 * g;
+        const int kk = s.base + 16 * c + 4 * g;
 #pragma unroll
-        for (int f = 0; f < NF; f++) gfrag(a.A, a.lda, row[f], vr[f], kk, full, s.rs, s.re, d.f[f]);
+        for (int f = 0; f < NF; f++) gfrag(a.A, a.lda, row[f], kk, d.f[f]);
       },
-      [&](const Frag<NF>& d) __attribute__((always_inline)) {
+      [&](int c, const Frag<NF>& d) __attribute__((always_inline)) {
+        const int k0 = s.base + 16 * c;
+        if (k0 >= s.rs && k0 + 16 <= s.re) {
+          mfmas(d.f);
+        } else {  // a chunk at the region's edge (both operands are these rows)
+          Frag<NF> m = d;
 #pragma unroll
-        for (int e = 0; e < 4; e++)
-          static_for<0, NTW>([&](auto tc) __attribute__((always_inline)) {
-            constexpr int t = decltype(tc)::value;
-            constexpr int ia = dg_a(W, t), ib = dg_b(W, t);
-            acc[t] = mfma16(d.f[ia][e], d.f[ib][e], acc[t]);
-          });
+          for (int f = 0; f < NF; f++) gmask(k0 + 4 * g, s.rs, s.re, m.f[f]);
+          mfmas(m.f);
+        }
       });
   double* part = a.part + (size_t)sp * a.n * a.n;
 #pragma unroll
@@ -228,39 +240,46 @@ __device__ __forceinline__ void gram_full_wave(const GramArgs& a, const double* 
   const int rbase = bi * kGB + wr * 64, cbase = bj * kGB + wc * 64;
   if (rbase >= a.n || cbase >= a.nb) return;
   int ra[4], rb[4];
-  bool va[4], vb[4];
 #pragma unroll
-  for (int x = 0; x < 4; x++) {
-    ra[x] = rbase + 16 * x + l16;
-    rb[x] = cbase + 16 * x + l16;
-    va[x] = ra[x] < a.n;
-    vb[x] = rb[x] < a.nb;
+  for (int x = 0; x < 4; x++) {  // rows past the matrices: outputs not stored
+    const int r = rbase + 16 * x + l16, q = cbase + 16 * x + l16;
+    ra[x] = r < a.n ? r : 0;
+    rb[x] = q < a.nb ? q : 0;
   }
   v4d acc[4][4];
 #pragma unroll
   for (int x = 0; x < 4; x++)
 #pragma unroll
     for (int y = 0; y < 4; y++) acc[x][y] = vzero();
+  auto mfmas = [&](const double (&f)[8][4]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) acc[x][y] = mfma16(f[x][e], f[4 + y][e], acc[x][y]);
+  };
   Frag<8> ring[kGramRing];  // fragments 0-3: rows of A, 4-7: rows of B
   chunk_ring(
       s.c0, s.c1, ring,
       [&](int c, Frag<8>& d) __attribute__((always_inline)) {
-        const int k0 = s.base + 16 * c;
-        const bool full = k0 >= s.rs && k0 + 16 <= s.re;
-        const int kk = k0 + 4 * g;
+        const int kk = s.base + 16 * c + 4 * g;
 #pragma unroll
         for (int x = 0; x < 4; x++) {
-          gfrag(a.A, a.lda, ra[x], va[x], kk, full, s.rs, s.re, d.f[x]);
-          gfrag(B, a.lda, rb[x], vb[x], kk, full, s.rs, s.re, d.f[4 + x]);
+          gfrag(a.A, a.lda, ra[x], kk, d.f[x]);
+          gfrag(B, a.lda, rb[x], kk, d.f[4 + x]);
         }
       },
-      [&](const Frag<8>& d) __attribute__((always_inline)) {
+      [&](int c, const Frag<8>& d) __attribute__((always_inline)) {
+        const int k0 = s.base + 16 * c;
+        if (k0 >= s.rs && k0 + 16 <= s.re) {
+          mfmas(d.f);
+        } else {  // a chunk at the region's edge (padding past K may hold anything)
+          Frag<8> m = d;
 #pragma unroll
-        for (int e = 0; e < 4; e++)
-#pragma unroll
-          for (int x = 0; x < 4; x++)
-#pragma unroll
-            for (int y = 0; y < 4; y++) acc[x][y] = mfma16(d.f[x][e], d.f[4 + y][e], acc[x][y]);
+          for (int x = 0; x < 8; x++) gmask(k0 + 4 * g, s.rs, s.re, m.f[x]);
+          mfmas(m.f);
+        }
       });
   double* part = a.part + (size_t)sp * a.n * a.nb;
 #pragma unroll
@@ -373,9 +392,60 @@ __global__ __launch_bounds__(1024) void k_gram_reduce(const double* __restrict__
   }
 }
 
+// Symmetric product: one workgroup per upper 16 x 16 tile (the tiles the
+// gram wrote), read along rows, written to C and, off the diagonal, to its
+// mirror.  Thread
+// (y, r, c): tile element (r, c), slices y, y + 4, ...; the four slice
+// groups are summed through LDS in a fixed order.
+__global__ __launch_bounds__(1024) void k_gram_reduce_sym(const double* __restrict__ part,
+                                                          int nslice, int n, double alpha,
+                                                          double* __restrict__ C) {
+  __shared__ double red[4][256];
+  const int nt = (n + 15) >> 4;
+  int b = blockIdx.x, ti = 0;
+  while (b >= nt - ti) {
+    b -= nt - ti;
+    ti++;
+  }
+  const int tj = ti + b;
+  const int y = threadIdx.x >> 8, t = threadIdx.x & 255, r = t >> 4, c = t & 15;
+  const int i = ti * 16 + r, j = tj * 16 + c;
+  const size_t nn = (size_t)n * n;
+  double s0 = 0.0, s1 = 0.0;
+  if (i < n && j < n) {
+    const double* p = part + (size_t)i * n + j;
+    for (int base = 0; base < nslice; base += 32) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int sp = base + y + 4 * u;
+        v[u] = sp < nslice ? p[(size_t)sp * nn] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u += 2) {
+        s0 += v[u];
+        s1 += v[u + 1];
+      }
+    }
+  }
+  red[y][t] = s0 + s1;
+  __syncthreads();
+  if (y == 0 && i < n && j < n) {
+    const double v = alpha * ((red[0][t] + red[1][t]) + (red[2][t] + red[3][t]));
+    C[(size_t)i * n + j] = v;
+    if (ti < tj) C[(size_t)j * n + i] = v;  // a diagonal tile holds both halves
+  }
+}
+
 hipError_t gram_reduce(const double* part, int nslice, int n, int nb, int sym, double alpha,
                        double* C, hipStream_t st) {
   if (sym && nb != n) return hipErrorInvalidValue;
+  if (sym) {
+    const int nt = (n + 15) >> 4;
+    hipLaunchKernelGGL(k_gram_reduce_sym, dim3(nt * (nt + 1) / 2), dim3(1024), 0, st, part,
+                       nslice, n, alpha, C);
+    return hipGetLastError();
+  }
   const size_t nn = (size_t)n * nb;
   hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)((nn + 63) / 64)), dim3(1024), 0, st, part,
                      nslice, n, nb, sym, alpha, C);
@@ -485,11 +555,13 @@ __global__ __launch_bounds__(256) void k_rotate(RotArgs a) {
 // workgroup b walks the 64-column k-blocks b, b + gridDim.x, ...  Wave w
 // owns the output rows j in [32 (w & 3), +32) and the k tiles 2 (w >> 2),
 // 2 (w >> 2) + 1 of a block.  The work items are (k-block, chunk of 16 i);
-// a ring of kRotRing X buffers keeps the loads of item q+4 in flight while
-// the MFMAs of items q+1..q+3 issue, across k-block boundaries.  The
+// a ring of R X buffers (R = 4, or 3 when the epilogue needs both Y and the
+// SD operands) keeps the loads of item q+R in flight while the MFMAs of
+// items q+1..q+R-1 issue, across k-block boundaries.  The
 // epilogue's operands (Y when beta != 0, psi and dt/m for the SD update)
-// are loaded when a block starts, behind its MFMAs.
-constexpr int kRotRing = 4;
+// are loaded a few items before the block ends, behind its MFMAs.
+constexpr int rot_ring(bool use_y, bool use_sd) { return use_y && use_sd ? 3 : 4; }
+constexpr int gcd_c(int a, int b) { return b == 0 ? a : gcd_c(b, a % b); }
 constexpr int kRotLd = 129;                    // LDS row stride of M
 constexpr int kRotThreads = 512;
 constexpr size_t kRotSmem = (size_t)128 * kRotLd * sizeof(double);
@@ -498,6 +570,13 @@ struct XFrag {
   double v[4][2];  // [i step][k tile]
 };
 
+// NCH: i chunks of 16 per k-block (1, 2, 4 or 8; M is zero beyond ni).  The
+// item loop is unrolled over G = lcm(NCH, R) items so that ring
+// slots and block epilogues sit at fixed places of the loop body.  USE_Y
+// (beta != 0) and USE_SD (sd_psi set) are compile-time so that the
+// epilogue operands only take registers when used: the ring's loads must
+// fit beside them to stay in flight.
+template <int NCH, bool USE_Y, bool USE_SD>
 __global__ __launch_bounds__(kRotThreads) void k_rotate_p(RotArgs a) {
   extern __shared__ __align__(16) double msm[];
   __shared__ double red[kRotThreads / 64];
@@ -509,7 +588,8 @@ __global__ __launch_bounds__(kRotThreads) void k_rotate_p(RotArgs a) {
 #pragma unroll
     for (int u = 0; u < PER; u++) {
       const int e = tid + kRotThreads * u, i = e >> 7, j = e & 127;
-      mv[u] = (i < a.ni && j < a.n) ? a.M[(size_t)i * a.n + j] : 0.0;
+      const double m = a.M[(size_t)min(i, a.ni - 1) * a.n + min(j, a.n - 1)];
+      mv[u] = (i < a.ni && j < a.n) ? m : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < PER; u++) {
@@ -523,19 +603,22 @@ __global__ __launch_bounds__(kRotThreads) void k_rotate_p(RotArgs a) {
   const int nkb = (a.K + kRK - 1) / kRK;
   const int nmine =
       (int)blockIdx.x < nkb ? (nkb - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x : 0;
-  const int nch = (a.ni + 15) >> 4;  // i chunks of 16 per k-block
-  const int nitems = nmine * nch;
-  auto kblock = [&](int q) { return (int)blockIdx.x + (q / nch) * (int)gridDim.x; };
+  constexpr int RING = rot_ring(USE_Y, USE_SD);
+  constexpr int G = NCH / gcd_c(NCH, RING) * RING;
+  const int nitems = nmine * NCH;
+  auto kblock = [&](int q) { return (int)blockIdx.x + (q / NCH) * (int)gridDim.x; };
+  // Branch-free loads (clamped to valid rows and columns) so that the
+  // ring's loads stay in flight: rows i >= ni meet zero rows of the staged
+  // M, columns k >= K produce outputs that are not stored.
   auto load = [&](int q, XFrag& d) __attribute__((always_inline)) {
-    const int kb = kblock(q) * kRK, ic = q % nch;
+    const int kb = kblock(q) * kRK, ic = q % NCH;
 #pragma unroll
     for (int s = 0; s < 4; s++) {
-      const int i = ic * 16 + 4 * s + g;
-      const bool vi = i < a.ni;
+      const int i = min(ic * 16 + 4 * s + g, a.ni - 1);
 #pragma unroll
       for (int y = 0; y < 2; y++) {
-        const int k = kb + 16 * (kt0 + y) + l16;
-        d.v[s][y] = (vi && k < a.K) ? a.X[(size_t)i * a.ldx + k] : 0.0;
+        const int k = min(kb + 16 * (kt0 + y) + l16, a.K - 1);
+        d.v[s][y] = a.X[(size_t)i * a.ldx + k];
       }
     }
   };
@@ -544,27 +627,26 @@ __global__ __launch_bounds__(kRotThreads) void k_rotate_p(RotArgs a) {
   for (int x = 0; x < 2; x++)
 #pragma unroll
     for (int y = 0; y < 2; y++) acc[x][y] = vzero();
-  double yold[2][2][4], pold[2][2][4], dm[2];
+  double yold[USE_Y ? 2 : 1][2][4], pold[USE_SD ? 2 : 1][2][4], dm[2];
   auto preload = [&](int kb) __attribute__((always_inline)) {
 #pragma unroll
     for (int y = 0; y < 2; y++) {
-      const int k = kb + 16 * (kt0 + y) + l16;
-      dm[y] = (a.sd_psi && k < a.K) ? a.dtm[k >> 1] : 0.0;
+      const int k = min(kb + 16 * (kt0 + y) + l16, a.K - 1);
+      if (USE_SD) dm[y] = a.dtm[k >> 1];
 #pragma unroll
       for (int x = 0; x < 2; x++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-          const int j = jb + 16 * x + g + 4 * v;
-          const bool ok = j < a.n && k < a.K;
-          yold[x][y][v] = (ok && a.beta != 0.0) ? a.Y[(size_t)j * a.ldx + k] : 0.0;
-          pold[x][y][v] = (ok && a.sd_psi) ? a.sd_psi[(size_t)j * a.ldx + k] : 0.0;
+          const int j = min(jb + 16 * x + g + 4 * v, a.n - 1);
+          if (USE_Y) yold[x][y][v] = a.Y[(size_t)j * a.ldx + k];
+          if (USE_SD) pold[x][y][v] = a.sd_psi[(size_t)j * a.ldx + k];
         }
     }
   };
   double fn = 0.0;
   // output of one k-block: Y = beta Y + s(k) acc, optionally the SD update
   auto epilogue = [&](int kb) __attribute__((always_inline)) {
-    if (a.sd_psi) __syncthreads();  // all waves are done reading these X columns
+    if (USE_SD) __syncthreads();  // all waves are done reading these X columns
 #pragma unroll
     for (int x = 0; x < 2; x++)
 #pragma unroll
@@ -573,29 +655,35 @@ __global__ __launch_bounds__(kRotThreads) void k_rotate_p(RotArgs a) {
         for (int v = 0; v < 4; v++) {
           const int j = jb + 16 * x + g + 4 * v;
           const int k = kb + 16 * (kt0 + y) + l16;
+          const double s = (k < a.k_lo) ? a.s_lo : a.s_hi;
+          double val = s * acc[x][y][v];
+          if (USE_Y) val = a.beta * yold[USE_Y ? x : 0][y][v] + val;
           if (j < a.n && k < a.K) {
-            const double s = (k < a.k_lo) ? a.s_lo : a.s_hi;
-            double val = s * acc[x][y][v];
-            if (a.beta != 0.0) val = a.beta * yold[x][y][v] + val;
             a.Y[(size_t)j * a.ldx + k] = val;
-            if (a.sd_psi) a.sd_psi[(size_t)j * a.ldx + k] = pold[x][y][v] + dm[y] * val;
+            if (USE_SD)
+              a.sd_psi[(size_t)j * a.ldx + k] = pold[USE_SD ? x : 0][y][v] + dm[y] * val;
             if (a.fpart) fn += val * val;
           }
           acc[x][y][v] = 0.0;
         }
   };
-  XFrag ring[kRotRing];
-  static_for<0, kRotRing>([&](auto dc) __attribute__((always_inline)) {
-    constexpr int d = decltype(dc)::value;
-    if (d < nitems) load(d, ring[d]);
-  });
-  if (nitems > 0) preload(kblock(0) * kRK);
-  for (int q0 = 0; q0 < nitems; q0 += kRotRing) {
-    static_for<0, kRotRing>([&](auto dc) __attribute__((always_inline)) {
+  XFrag ring[RING];
+  if (nitems > 0) {
+    // Every load below is issued on every path (items and blocks past the
+    // workgroup's share load clamped addresses and are never stored): a
+    // load that only some paths issue makes the compiler wait for all.
+    static_for<0, RING>([&](auto dc) __attribute__((always_inline)) {
       constexpr int d = decltype(dc)::value;
-      const int q = q0 + d;
-      if (q < nitems) {
-        const int ic = q % nch;
+      load(d, ring[d]);
+    });
+    // the epilogue's operands are loaded PRE items before the block ends
+    // (short register lifetime, still behind MFMAs)
+    constexpr int PRE = NCH >= 3 ? NCH - 3 : 0;
+    for (int q0 = 0; q0 < nitems; q0 += G) {
+      static_for<0, G>([&](auto dc) __attribute__((always_inline)) {
+        constexpr int d = decltype(dc)::value;
+        constexpr int sl = d % RING, ic = d % NCH;  // q0 is a multiple of G
+        const int q = q0 + d;
         if (wact) {
 #pragma unroll
           for (int s = 0; s < 4; s++) {
@@ -603,18 +691,16 @@ __global__ __launch_bounds__(kRotThreads) void k_rotate_p(RotArgs a) {
             const double am0 = mrow[0], am1 = mrow[16];
 #pragma unroll
             for (int y = 0; y < 2; y++) {
-              acc[0][y] = mfma16(am0, ring[d].v[s][y], acc[0][y]);
-              acc[1][y] = mfma16(am1, ring[d].v[s][y], acc[1][y]);
+              acc[0][y] = mfma16(am0, ring[sl].v[s][y], acc[0][y]);
+              acc[1][y] = mfma16(am1, ring[sl].v[s][y], acc[1][y]);
             }
           }
         }
-        if (q + kRotRing < nitems) load(q + kRotRing, ring[d]);
-        if (ic == nch - 1) {
-          epilogue(kblock(q) * kRK);
-          if (q + 1 < nitems) preload(kblock(q + 1) * kRK);
-        }
-      }
-    });
+        if (ic == PRE) preload(kblock(q) * kRK);
+        load(q + RING, ring[sl]);
+        if (ic == NCH - 1 && q < nitems) epilogue(kblock(q) * kRK);
+      });
+    }
   }
   if (a.fpart) {
     for (int o = 32; o > 0; o >>= 1) fn += __shfl_down(fn, o, 64);
@@ -652,14 +738,22 @@ hipError_t rotate_f64(const RotArgs& args, hipStream_t st) {
   if (a.sd_psi && (a.sd_psi != a.X || !a.dtm || !a.fpart || a.ni != a.n))
     return hipErrorInvalidValue;
   if (a.n <= 128 && a.ni <= 128) {
-    static bool granted = false;
-    if (!granted) {
+    const int nch = (a.ni + 15) >> 4;
+    // [chunk-count variant][epilogue mode: Y read (beta != 0), SD update]
+    const int v = 4 * (nch <= 1 ? 0 : nch <= 2 ? 1 : nch <= 4 ? 2 : 3) +
+                  (a.beta != 0.0 ? 1 : 0) + (a.sd_psi ? 2 : 0);
+#define OA_ROT(nc) k_rotate_p<nc, false, false>, k_rotate_p<nc, true, false>, \
+                   k_rotate_p<nc, false, true>, k_rotate_p<nc, true, true>
+    void (*const fns[16])(RotArgs) = {OA_ROT(1), OA_ROT(2), OA_ROT(4), OA_ROT(8)};
+#undef OA_ROT
+    static bool granted[16] = {};
+    if (!granted[v]) {
       const hipError_t e = hipFuncSetAttribute(
-          (const void*)k_rotate_p, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRotSmem);
+          (const void*)fns[v], hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRotSmem);
       if (e != hipSuccess) return e;
-      granted = true;
+      granted[v] = true;
     }
-    hipLaunchKernelGGL(k_rotate_p, dim3(rotate_pgrid(a.K)), dim3(kRotThreads), kRotSmem, st, a);
+    hipLaunchKernelGGL(fns[v], dim3(rotate_pgrid(a.K)), dim3(kRotThreads), kRotSmem, st, a);
     return hipGetLastError();
   }
   dim3 grid((a.K + kRK - 1) / kRK, (a.n + 127) / 128);

@@ -166,6 +166,7 @@ struct oa_ctx {
   int ev_cur = 0, ev_count = 0;
   bool have_S = false, have_T = false, have_v = false;
   bool ns_coop = false;  // Newton-Schulz as one cooperative launch
+  bool sd_fused = false;  // SD update inside the Lambda rotate (nst <= 128)
   int engine = 0;     // kEngFft2, kEngSep, kEngFused or kEngPlane
   // plane engine with V fixed during the step: rho partials come from the
   // R->G pass (psi(r) is recomputed there anyway), no G->R plane pass
@@ -630,6 +631,7 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
   const int nt = (n + 31) / 32;
   TRY(dalloc(c, &c->errpart, (size_t)(c->opt.ns_max_iter + 2) * nt * nt * sizeof(double)));
   c->ns_coop = (c->opt.ns_mode == 0) && oa::ns_coop_possible(n, c->device);
+  c->sd_fused = false;  // measured slower than a separate SD pass (registers)
   // |F|^2 partials: one per workgroup of the SD kernel or of the rotate that
   // applies Lambda and the SD update together
   c->nfpart = std::max(1024, oa::rotate_nwg(n, K));
@@ -868,8 +870,6 @@ oa_status st_lambda(oa_ctx* c, bool with_sd = false) {
   TRY(st_rotate(c, c->psi, c->lam, c->force, 1.0, -0.5, -1.0, with_sd));
   return OA_OK;
 }
-
-bool lambda_sd_fusable(const oa_ctx* c) { return c->nst <= 128; }
 
 // |F|^2 from the partials of the fused rotate, then the redundancy sync
 oa_status st_sd_finish(oa_ctx* c) {
@@ -1149,6 +1149,20 @@ oa_status autotune(oa_ctx* c) {
         c, e0, e1, "Newton-Schulz cooperative (1) or launches (0)", {1, 0}, 1,
         [&](int v) { c->ns_coop = v != 0; },
         [&]() { return st_ns(c, c->opt.ns_tol, c->opt.ns_max_iter); });
+  }
+  if (c->nst <= 128 && c->opt.use_lambda && !c->opt.use_cg) {
+    // F = 0 and psi = 0 at creation: the SD update leaves psi as it is
+    tune_knob(
+        c, e0, e1, "SD update in the Lambda rotate (1) or a pass of its own (0)", {1, 0}, 0,
+        [&](int v) { c->sd_fused = v != 0; },
+        [&]() -> oa_status {
+          if (c->sd_fused) {
+            TRY(st_lambda(c, true));
+            return st_sd_finish(c);
+          }
+          TRY(st_lambda(c));
+          return st_sd(c);
+        });
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -1698,7 +1712,7 @@ oa_status enqueue_pre(oa_ctx* c, bool marks) {
   if (c->opt.use_cg) {
     TRY(st_lambda_cg(c));
     mk(7);
-  } else if (c->opt.use_lambda && lambda_sd_fusable(c)) {
+  } else if (c->opt.use_lambda && c->sd_fused) {
     TRY(st_lambda(c, true));  // Lambda projection and SD update in one pass
     mk(7);
     TRY(st_sd_finish(c));
