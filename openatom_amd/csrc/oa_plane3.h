@@ -60,31 +60,36 @@ __device__ __forceinline__ cpx<T> twl(const cpx<T>* tw, int m) {
   return {w.x, S < 0 ? w.y : -w.y};
 }
 
-// copy the pass-2 twiddles into LDS (callers synchronise before use)
+// copy the pass-2 twiddles and the per-column ky ranges into LDS (callers
+// synchronise before use); the ranges are read for every state, and a
+// global load there would cost a memory latency per state
 template <typename T, int NTW, int NT>
-__device__ __forceinline__ void load_twl(cpx<T>* tw, const double2* __restrict__ g) {
+__device__ __forceinline__ void load_twl(cpx<T>* tw, const double2* __restrict__ g, int2* ky,
+                                         const FftCtx& c, int npx) {
   for (int i = threadIdx.x; i < NTW; i += NT) tw[i] = {(T)g[i].x, (T)g[i].y};
+  for (int i = threadIdx.x; i < npx; i += NT) ky[i] = make_int2(c.kylo[i], c.kyhi[i]);
 }
 
-// LDS bytes of either kernel: P (nplane_x columns) + E + twiddles
+// LDS bytes of either kernel: P (nplane_x columns) + E + twiddles + ky ranges
 template <typename T, int X1, int X2, int Y1, int Y2, int NT>
 size_t plane3_smem(int npx) {
   using S = PS<X1, X2, Y1, Y2, NT>;
-  return sizeof(cpx<T>) * ((size_t)npx * S::CY + S::EX + S::NTW);
+  return sizeof(cpx<T>) * ((size_t)npx * S::CY + S::EX + S::NTW) + sizeof(int2) * npx;
 }
 
 // y-FFT (FFTW_FORWARD) of the populated columns of one z-plane into P.
-// zs: zbuf plane [kx][NY]; only ky in [kylo, kyhi] of each column is read.
+// zs: zbuf plane [kx][NY]; only ky in KY[kx] = [kylo, kyhi] of each column
+// is read.
 template <typename T, int Y1, int Y2, int NT>
 __device__ __forceinline__ void y_fwd(const cpx<T>* __restrict__ zs, cpx<T>* P, const cpx<T>* TW,
-                                      const FftCtx& c, int npx) {
+                                      const int2* KY, int npx) {
   constexpr int NY = Y1 * Y2, GY = Y1 > Y2 ? Y1 : Y2, NTRY = NT / GY, CY = NY | 1;
   const int tid = threadIdx.x;
   const int tr = tid / GY, j = tid - tr * GY;
   for (int cb = 0; cb < npx; cb += NTRY) {
     const int kx = cb + tr;
     if (tr < NTRY && kx < npx && j < Y2) {
-      const int lo = c.kylo[kx], hi = c.kyhi[kx];
+      const int lo = KY[kx].x, hi = KY[kx].y;
       const cpx<T>* src = zs + (size_t)kx * NY;
       cpx<T> v[Y1];
 #pragma unroll
@@ -117,10 +122,10 @@ __device__ __forceinline__ void y_fwd(const cpx<T>* __restrict__ zs, cpx<T>* P, 
 }
 
 // y-FFT (FFTW_BACKWARD) of the nplane_x columns of P into the zout plane,
-// keeping ky in [kylo, kyhi]; E is the exchange buffer.
+// keeping ky in KY[kx]; E is the exchange buffer.
 template <typename T, int Y1, int Y2, int NT>
 __device__ __forceinline__ void y_bwd(const cpx<T>* P, cpx<T>* E, const cpx<T>* TW,
-                                      cpx<T>* __restrict__ zo, const FftCtx& c, int npx) {
+                                      cpx<T>* __restrict__ zo, const int2* KY, int npx) {
   constexpr int NY = Y1 * Y2, GY = Y1 > Y2 ? Y1 : Y2, NTRY = NT / GY, CY = NY | 1;
   const int tid = threadIdx.x;
   const int tr = tid / GY, j = tid - tr * GY;
@@ -143,7 +148,7 @@ __device__ __forceinline__ void y_bwd(const cpx<T>* P, cpx<T>* E, const cpx<T>* 
 #pragma unroll
       for (int t = 1; t < Y2; t++) u[t] = cm(u[t], twl<T, +1>(TW, t * j));
       Dft<T, Y2, +1>::run(u);
-      const int lo = c.kylo[kx], hi = c.kyhi[kx];
+      const int lo = KY[kx].x, hi = KY[kx].y;
       cpx<T>* dst = zo + (size_t)kx * NY;
 #pragma unroll
       for (int t = 0; t < Y2; t++) {
@@ -216,6 +221,7 @@ __global__ __launch_bounds__(NT) void k3_g2r(FftCtx c) {
   cpx<T>* P = reinterpret_cast<cpx<T>*>(smem_raw);
   cpx<T>* E = P + (size_t)npx * S::CY;
   cpx<T>* TW = E + S::EX;
+  int2* KY = reinterpret_cast<int2*>(TW + S::NTW);
   const int z = blockIdx.x, rp = blockIdx.y, chunk = blockIdx.z;
   const int r0 = rp * RPW;
   int st0, st1;
@@ -230,7 +236,7 @@ __global__ __launch_bounds__(NT) void k3_g2r(FftCtx c) {
   const int tr = tid / GX, j = tid - tr * GX;
   const size_t plane = (size_t)npx * NY;
   const size_t NP = (size_t)NX * NY;
-  load_twl<T, S::NTW, NT>(TW, c.twx);
+  load_twl<T, S::NTW, NT>(TW, c.twx, KY, c, npx);
   __syncthreads();
   double acc[RPW][2 * X2];
 #pragma unroll
@@ -239,7 +245,7 @@ __global__ __launch_bounds__(NT) void k3_g2r(FftCtx c) {
     for (int b = 0; b < 2 * X2; b++) acc[a][b] = 0.0;
   for (int s = st0; s < st1; s++) {
     const cpx<T>* zs = (const cpx<T>*)c.zbuf + ((size_t)s * c.g.nz + z) * plane;
-    y_fwd<T, Y1, Y2, NT>(zs, P, TW, c, npx);
+    y_fwd<T, Y1, Y2, NT>(zs, P, TW, KY, npx);
     const double oc = c.occ[s];
 #pragma unroll
     for (int rr = 0; rr < RPW; rr++) {
@@ -304,6 +310,7 @@ k3_r2g(FftCtx c) {
   cpx<T>* P = reinterpret_cast<cpx<T>*>(smem_raw);
   cpx<T>* E = P + (size_t)npx * CY;
   cpx<T>* TW = E + S::EX;
+  int2* KY = reinterpret_cast<int2*>(TW + S::NTW);
   const int z = blockIdx.x, chunk = blockIdx.y;
   const int st0 = chunk * c.cs_r, st1 = min(c.g.nst, st0 + c.cs_r);
   const int tid = threadIdx.x;
@@ -311,7 +318,7 @@ k3_r2g(FftCtx c) {
   const size_t plane = (size_t)npx * NY;
   const size_t NP = (size_t)NX * NY;
   const double* vz = c.vks + (size_t)z * NP;
-  load_twl<T, S::NTW, NT>(TW, c.twx);
+  load_twl<T, S::NTW, NT>(TW, c.twx, KY, c, npx);
   __syncthreads();
   T vr[VREG ? R : 1][VREG ? 2 * X2 : 1];
   double acc[RHO ? R : 1][RHO ? 2 * X2 : 1];
@@ -321,27 +328,43 @@ k3_r2g(FftCtx c) {
 #pragma unroll
       for (int b = 0; b < 2 * X2; b++) acc[r][b] = 0.0;
   }
+  // V loads are branch-free (clamped rows and columns, masked after the
+  // load): a load inside a divergent branch is waited for on its own, so a
+  // series of them costs one memory latency each
   if (VREG) {
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int q = r * NTRX + tr;
       const bool act = tr < NTRX && q < S::NPAIR && j < X1;
+      const int ya = 2 * min(q, S::NPAIR - 1), yb = min(ya + 1, NY - 1);
 #pragma unroll
       for (int t = 0; t < X2; t++) {
-        const int x = j + t * X1;
-        vr[r][2 * t] = act ? (T)vz[(size_t)(2 * q) * NX + x] : (T)0;
-        vr[r][2 * t + 1] = (act && 2 * q + 1 < NY) ? (T)vz[(size_t)(2 * q + 1) * NX + x] : (T)0;
+        const int x = min(j + t * X1, NX - 1);
+        const T a = (T)vz[(size_t)ya * NX + x], b = (T)vz[(size_t)yb * NX + x];
+        vr[r][2 * t] = act ? a : (T)0;
+        vr[r][2 * t + 1] = (act && 2 * q + 1 < NY) ? b : (T)0;
       }
     }
   }
   for (int s = st0; s < st1; s++) {
     const cpx<T>* zs = (const cpx<T>*)c.zbuf + ((size_t)s * c.g.nz + z) * plane;
-    y_fwd<T, Y1, Y2, NT>(zs, P, TW, c, npx);
+    y_fwd<T, Y1, Y2, NT>(zs, P, TW, KY, npx);
     const T oc = (T)c.occ[s];
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int q = r * NTRX + tr;
       const bool act = tr < NTRX && q < S::NPAIR;
+      // without VREG: this round's V, loaded (branch-free) ahead of the transform
+      T vv[VREG ? 1 : 2 * X2];
+      if (!VREG) {
+        const int ya = 2 * min(q, S::NPAIR - 1), yb = min(ya + 1, NY - 1);
+#pragma unroll
+        for (int t = 0; t < X2; t++) {
+          const int x = min(j + t * X1, NX - 1);
+          vv[VREG ? 0 : 2 * t] = (T)vz[(size_t)ya * NX + x];
+          vv[VREG ? 0 : 2 * t + 1] = (T)vz[(size_t)yb * NX + x];
+        }
+      }
       cpx<T> u[X2];
       x_c2r<T, X1, X2, NT>(P, E, TW, c, npx, NY, q, tr, j, act, u);
       __syncthreads();  // E is rewritten below
@@ -363,9 +386,8 @@ k3_r2g(FftCtx c) {
             v1 = vr[r][2 * t];
             v2 = vr[r][2 * t + 1];
           } else {
-            const int x = j + t * X1;
-            v1 = (T)vz[(size_t)y1 * NX + x];
-            v2 = (y2 < NY) ? (T)vz[(size_t)y2 * NX + x] : (T)0;
+            v1 = vv[VREG ? 0 : 2 * t];
+            v2 = vv[VREG ? 0 : 2 * t + 1];
           }
           w[t] = {u[t].x * (v1 * oc), (y2 < NY) ? u[t].y * (v2 * oc) : (T)0};
         }
@@ -400,7 +422,7 @@ k3_r2g(FftCtx c) {
       __syncthreads();
     }
     cpx<T>* zo = (cpx<T>*)c.zout + ((size_t)s * c.g.nz + z) * plane;
-    y_bwd<T, Y1, Y2, NT>(P, E, TW, zo, c, npx);
+    y_bwd<T, Y1, Y2, NT>(P, E, TW, zo, KY, npx);
   }
   if (!RHO) return;
   // rho partial of this chunk (same layout and per-element order as k3_g2r)
