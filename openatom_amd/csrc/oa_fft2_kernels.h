@@ -46,16 +46,15 @@ __global__ __launch_bounds__(kThreads) void k2_zfwd(FftCtx c) {
     if (tr < nl && j < R2) {
       const int4 L = c.lines[l0 + tr];
       const double2* ps = c.psi + (size_t)s * c.g.ld + L.x;
+      // branch-free: every load issued before any is waited for (a load in
+      // a divergent branch is waited for on its own); lines hold >= 1 point
       cpx<T> v[R1];
 #pragma unroll
       for (int t = 0; t < R1; t++) {
         const int d = signed_k(j + t * R2, N) - L.z;
-        if (d >= 0 && d < L.y) {
-          const double2 a = ps[d];
-          v[t] = {(T)a.x, (T)a.y};
-        } else {
-          v[t] = {(T)0, (T)0};
-        }
+        const double2 a = ps[min(max(d, 0), L.y - 1)];
+        const bool in = d >= 0 && d < L.y;
+        v[t] = {in ? (T)a.x : (T)0, in ? (T)a.y : (T)0};
       }
       Dft<T, R1, -1>::run(v);
 #pragma unroll
@@ -119,14 +118,24 @@ __global__ __launch_bounds__(kThreads) void k2_zbwd(FftCtx c) {
       const double2* ps = c.psi + (size_t)s * c.g.ld + L.x;
       double2* fs = c.force + (size_t)s * c.g.ld + L.x;
       const double* g2 = c.g2 + L.x;
+      // operands loaded branch-free (clamped to the line), then the
+      // masked update: a load in a divergent branch is waited for on its own
+      double gq[R2];
+      double2 cq[R2];
+#pragma unroll
+      for (int t = 0; t < R2; t++) {
+        const int dc = min(max(signed_k(j + t * R1, N) - L.z, 0), L.y - 1);
+        gq[t] = g2[dc];
+        cq[t] = ps[dc];
+      }
 #pragma unroll
       for (int t = 0; t < R2; t++) {
         const int d = signed_k(j + t * R1, N) - L.z;
         if (d >= 0 && d < L.y) {
-          const double gg = g2[d];
+          const double gg = gq[t];
           double2 f;
           if (gg <= c.ecut_ry) {
-            const double2 cc = ps[d];
+            const double2 cc = cq[t];
             const double wt = oc * w * gg;
             f.x = w * (c.scale * (double)u[t].x) - wt * cc.x;
             f.y = w * (c.scale * (double)u[t].y) - wt * cc.y;
