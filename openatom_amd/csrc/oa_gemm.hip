@@ -1089,6 +1089,142 @@ static hipError_t ns_run_launches(const NsArgs& a, hipStream_t st) {
   return hipGetLastError();
 }
 
+// ---- block-row variant: one grid barrier per iteration
+//
+// Workgroup b owns rows R = [16b, 16b + 16) of C, X and T.  X, T and C are
+// polynomials in S and commute, so the new X = X C / 2 of the recurrence
+// equals C X / 2 (up to rounding), and both new row blocks,
+//   X'[R,:] = C[R,:] X / 2,  T'[R,:] = C[R,:] T / 2,
+// need only this block's rows of C = 3I - T[R,:] X and the previous X and T.
+// C stays in LDS; the rows of T stay in LDS across iterations; X and T go
+// through global memory (double buffered) for the other blocks.  Wave w
+// handles the 16-column tiles w, w + 8, ...  MFMA operands: A from LDS, B
+// loaded branch-free (clamped, masked) from global memory.
+constexpr int kNrWaves = 8;
+__host__ __device__ constexpr int nr_ld(int n) { return ((n + 3) & ~3) + 2; }
+size_t ns_rows_smem(int n) { return (size_t)2 * 16 * nr_ld(n) * sizeof(double); }
+
+// acc1 += A[16 lds rows, k] B1[k][col tile ct] (and acc2 with B2 when TWO)
+// over k in [0, n): the B values of 32 k-steps (n <= 128: all of them) are
+// loaded at once, branch-free (clamped, masked), then the MFMAs issue
+template <bool TWO>
+__device__ __forceinline__ void nr_tile(const double* __restrict__ Al, int ld,
+                                        const double* __restrict__ B1,
+                                        const double* __restrict__ B2, int n, int ct, v4d& acc1,
+                                        v4d& acc2) {
+  constexpr int KB = 32;
+  const int lane = threadIdx.x & 63, l16 = lane & 15, g = lane >> 4;
+  const int col = ct * 16 + l16, colc = min(col, n - 1);
+  const int nk = (n + 3) >> 2;
+  for (int k0 = 0; k0 < nk; k0 += KB) {
+    double b1[KB], b2[TWO ? KB : 1];
+#pragma unroll
+    for (int u = 0; u < KB; u++) {
+      const int k = (k0 + u) * 4 + g;
+      const size_t idx = (size_t)min(k, n - 1) * n + colc;
+      const bool in = k < n && col < n;
+      const double v1 = B1[idx];
+      b1[u] = in ? v1 : 0.0;
+      if (TWO) {
+        const double v2 = B2[idx];
+        b2[TWO ? u : 0] = in ? v2 : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < KB; u++)
+      if (k0 + u < nk) {
+        const double av = Al[l16 * ld + (k0 + u) * 4 + g];
+        acc1 = mfma16(av, b1[u], acc1);
+        if (TWO) acc2 = mfma16(av, b2[TWO ? u : 0], acc2);
+      }
+  }
+}
+
+__global__ __launch_bounds__(kNrWaves * 64) void k_ns_rows(NsArgs a) {
+  namespace cg = cooperative_groups;
+  extern __shared__ __align__(16) double nr_smem[];
+  cg::grid_group grid = cg::this_grid();
+  const int n = a.n, ld = nr_ld(n), nb = gridDim.x, nct = (n + 15) >> 4;
+  double* Tr = nr_smem;        // [16][ld]: rows R of T
+  double* Cr = Tr + 16 * ld;   // [16][ld]: rows R of C
+  const int r0 = 16 * blockIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, l16 = lane & 15, g = lane >> 4;
+  // X0 = S/2 (ortho.C:236-239), T0 = I; LDS rows zero past n and past the k range
+  for (int e = threadIdx.x; e < 16 * ld; e += blockDim.x) {
+    const int r = e / ld, k = e - r * ld, row = r0 + r;
+    const bool in = row < n && k < n;
+    Tr[e] = (in && row == k) ? 1.0 : 0.0;
+    Cr[e] = 0.0;
+    if (in) {
+      a.X0[(size_t)row * n + k] = 0.5 * a.S[(size_t)row * n + k];
+      a.T0[(size_t)row * n + k] = (row == k) ? 1.0 : 0.0;
+    }
+  }
+  grid.sync();
+  double* X[2] = {a.X0, a.X1};
+  double* T[2] = {a.T0, a.T1};
+  int conv = 0;
+  double err = 0.0;
+  for (int it = 1; it <= a.max_iter; it++) {
+    const int o = (it - 1) & 1, w = it & 1;
+    // C[R,:] = 3I - T[R,:] X
+    for (int ct = wave; ct < nct; ct += kNrWaves) {
+      v4d acc = vzero(), unused = vzero();
+      nr_tile<false>(Tr, ld, X[o], nullptr, n, ct, acc, unused);
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = g + 4 * v, row = r0 + r, col = ct * 16 + l16;
+        if (col < n) Cr[r * ld + col] = (row < n ? (row == col ? 3.0 : 0.0) - acc[v] : 0.0);
+      }
+    }
+    __syncthreads();
+    // X'[R,:] = C[R,:] X / 2, T'[R,:] = C[R,:] T / 2, and the change of T
+    double e = 0.0;
+    for (int ct = wave; ct < nct; ct += kNrWaves) {
+      v4d ax = vzero(), at = vzero();
+      nr_tile<true>(Cr, ld, X[o], T[o], n, ct, ax, at);
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = g + 4 * v, row = r0 + r, col = ct * 16 + l16;
+        if (row < n && col < n) {
+          const double xn = 0.5 * ax[v], tn = 0.5 * at[v];
+          X[w][(size_t)row * n + col] = xn;
+          T[w][(size_t)row * n + col] = tn;
+          const double d = Tr[r * ld + col] - tn;
+          e += d * d;
+          Tr[r * ld + col] = tn;  // this thread's element only: read before written
+        }
+      }
+    }
+    {  // workgroup sum in a fixed order
+      __shared__ double red[kNrWaves];
+      for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
+      if (lane == 0) red[wave] = e;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < kNrWaves; i++) t += red[i];
+        a.errpart[(size_t)it * nb + blockIdx.x] = t;
+      }
+    }
+    grid.sync();
+    err = ns_err(a.errpart, it, nb, n);
+    if (err <= a.tol) {
+      conv = it;
+      break;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.res->ns_iter = conv ? conv : -a.max_iter;
+    a.res->ns_err = err;
+  }
+  // T of the last iteration is this block's LDS rows
+  for (int e = threadIdx.x; e < 16 * n; e += blockDim.x) {
+    const int r = e / n, k = e - r * n, row = r0 + r;
+    if (row < n) a.Tout[(size_t)row * n + k] = Tr[r * ld + k];
+  }
+}
+
 static bool ns_smem_granted(int device) {
   static int done = -1;
   if (done == device) return true;
@@ -1105,6 +1241,29 @@ static bool ns_smem_granted(int device) {
 }
 
 // Can all n x n tiles be resident at once for the cooperative launch?
+// The block-row variant: all ceil(n/16) workgroups resident, both LDS row
+// blocks within the device's limit.
+bool ns_rows_possible(int n, int device) {
+  int per_cu = 0, ncu = 0, coop = 0, lds_max = 0;
+  if (n < 1 || hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device) !=
+                   hipSuccess ||
+      !coop)
+    return false;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
+          hipSuccess)
+    return false;
+  const size_t smem = ns_rows_smem(n);
+  if (smem > (size_t)lds_max ||
+      hipFuncSetAttribute((const void*)k_ns_rows, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)smem) != hipSuccess)
+    return false;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_ns_rows,
+                                                   kNrWaves * 64, smem) != hipSuccess)
+    return false;
+  return (long)per_cu * ncu >= (long)((n + 15) / 16);
+}
+
 bool ns_coop_possible(int n, int device) {
   const int nt = (n + NT - 1) / NT;
   int per_cu = 0, ncu = 0, coop = 0;
@@ -1122,6 +1281,12 @@ bool ns_coop_possible(int n, int device) {
 
 hipError_t ns_run(const NsArgs& a, hipStream_t st) {
   if (!a.coop) return ns_run_launches(a, st);
+  if (a.coop == 2) {
+    NsArgs args = a;
+    void* params[] = {&args};
+    return hipLaunchCooperativeKernel((const void*)k_ns_rows, dim3((a.n + 15) / 16),
+                                      dim3(kNrWaves * 64), params, ns_rows_smem(a.n), st);
+  }
   const int nt = (a.n + NT - 1) / NT;
   NsArgs args = a;
   void* params[] = {&args};

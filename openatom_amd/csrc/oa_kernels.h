@@ -199,10 +199,13 @@ struct NsArgs {
   StepDev* res;
   int n, max_iter;
   double tol;
-  int coop;  // one cooperative launch (all tiles resident) or two launches per iteration
+  int coop;  // 0: two launches per iteration; 1: one cooperative launch over 32 x 32
+             // tiles (two grid barriers per iteration); 2: one cooperative launch over
+             // 16-row blocks (one barrier per iteration)
 };
 hipError_t ns_run(const NsArgs& a, hipStream_t st);
 bool ns_coop_possible(int n, int device);
+bool ns_rows_possible(int n, int device);
 
 // Conjugate-gradient state of a minimisation, on the device
 // (CP_State_GSpacePlane.C:120-136, 2646-2671)

@@ -165,7 +165,10 @@ struct oa_ctx {
   std::vector<hipEvent_t> ev;  // [kRing][OA_NTIMINGS] once timing is enabled
   int ev_cur = 0, ev_count = 0;
   bool have_S = false, have_T = false, have_v = false;
-  bool ns_coop = false;  // Newton-Schulz as one cooperative launch
+  // Newton-Schulz: 0 two launches per iteration, 1 one cooperative launch
+  // over 32 x 32 tiles, 2 one cooperative launch over 16-row blocks
+  int ns_kind = 0;
+  std::vector<int> ns_kinds;  // the kinds this context can run
   bool sd_fused = false;  // SD update inside the Lambda rotate (nst <= 128)
   int engine = 0;     // kEngFft2, kEngSep, kEngFused or kEngPlane
   // plane engine with V fixed during the step: rho partials come from the
@@ -629,8 +632,16 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
   TRY(dalloc(c, &c->nsT1, nn * sizeof(double)));
   TRY(dalloc(c, &c->nsC, nn * sizeof(double)));
   const int nt = (n + 31) / 32;
-  TRY(dalloc(c, &c->errpart, (size_t)(c->opt.ns_max_iter + 2) * nt * nt * sizeof(double)));
-  c->ns_coop = (c->opt.ns_mode == 0) && oa::ns_coop_possible(n, c->device);
+  // per-workgroup error partials of either cooperative kind
+  TRY(dalloc(c, &c->errpart, (size_t)(c->opt.ns_max_iter + 2) *
+                                 std::max(nt * nt, (n + 15) / 16) * sizeof(double)));
+  c->ns_kinds.clear();
+  if (c->opt.ns_mode == 0) {
+    if (!c->dry && oa::ns_rows_possible(n, c->device)) c->ns_kinds.push_back(2);
+    if (oa::ns_coop_possible(n, c->device)) c->ns_kinds.push_back(1);
+  }
+  c->ns_kinds.push_back(0);
+  c->ns_kind = c->ns_kinds.front();
   c->sd_fused = false;  // measured slower than a separate SD pass (registers)
   // |F|^2 partials: one per workgroup of the SD kernel or of the rotate that
   // applies Lambda and the SD update together
@@ -979,7 +990,7 @@ oa_status st_ns(oa_ctx* c, double tol, int max_iter) {
   a.n = c->nst;
   a.max_iter = max_iter;
   a.tol = tol;
-  a.coop = c->ns_coop ? 1 : 0;
+  a.coop = c->ns_kind;
   HIPCHK(oa::ns_run(a, c->stream));
   c->have_T = true;
   return OA_OK;
@@ -1140,14 +1151,15 @@ oa_status autotune(oa_ctx* c) {
       c, e0, e1, "Lambda K slices", c->tune_nslice_full, c->nslice_full,
       [&](int v) { c->nslice_full = v; },
       [&]() { return st_gram(c, c->psi, c->force, false, 0.5, 1.0, c->lam); });
-  if (c->ns_coop) {
+  if (c->ns_kinds.size() > 1) {
     // S = 2I converges in one iteration: the cost of one iteration either way
     std::vector<double> two((size_t)c->nst * c->nst, 0.0);
     for (int i = 0; i < c->nst; i++) two[(size_t)i * c->nst + i] = 2.0;
     HIPCHK(hipMemcpy(c->S, two.data(), two.size() * sizeof(double), hipMemcpyHostToDevice));
     tune_knob(
-        c, e0, e1, "Newton-Schulz cooperative (1) or launches (0)", {1, 0}, 1,
-        [&](int v) { c->ns_coop = v != 0; },
+        c, e0, e1,
+        "Newton-Schulz row blocks (2), cooperative tiles (1) or launches (0)", c->ns_kinds,
+        c->ns_kind, [&](int v) { c->ns_kind = v; },
         [&]() { return st_ns(c, c->opt.ns_tol, c->opt.ns_max_iter); });
   }
   if (c->nst <= 128 && c->opt.use_lambda && !c->opt.use_cg) {
