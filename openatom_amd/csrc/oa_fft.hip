@@ -539,18 +539,45 @@ __global__ __launch_bounds__(NTH) void k_plane_bwd(Geom g, FftPlan px, FftPlan p
 }
 
 // rho = (1/Omega) sum_chunks rho_part; per-workgroup partials of sum V rho.
+// rho = sum of the chunk partials / Omega, and the partials of sum V rho
+// when vks is set.  Two points per thread and round, all their loads issued
+// before any is used (the partials are read once: latency, not bandwidth,
+// bounds a plain loop).
 __global__ void k_rho_reduce(const double* __restrict__ part, int nchunk, size_t n,
                              double inv_vol, const double* __restrict__ vks,
                              double* __restrict__ rho, double* __restrict__ dpart) {
   __shared__ double red[16];
   double e = 0.0;
-  for (size_t r = blockIdx.x * (size_t)blockDim.x + threadIdx.x; r < n;
-       r += (size_t)gridDim.x * blockDim.x) {
-    double a = 0.0;
-    for (int c = 0; c < nchunk; c++) a += part[(size_t)c * n + r];
-    a *= inv_vol;
-    rho[r] = a;
-    e += a * vks[r];
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t r0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x; r0 < n; r0 += 2 * stride) {
+    const size_t r1 = r0 + stride, r1c = r1 < n ? r1 : r0;
+    double a0 = 0.0, a1 = 0.0;
+    int c = 0;
+    for (; c + 4 <= nchunk; c += 4) {
+      double v0[4], v1[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        v0[u] = part[(size_t)(c + u) * n + r0];
+        v1[u] = part[(size_t)(c + u) * n + r1c];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        a0 += v0[u];
+        a1 += v1[u];
+      }
+    }
+    for (; c < nchunk; c++) {
+      a0 += part[(size_t)c * n + r0];
+      a1 += part[(size_t)c * n + r1c];
+    }
+    a0 *= inv_vol;
+    a1 *= inv_vol;
+    rho[r0] = a0;
+    if (r1 < n) rho[r1] = a1;
+    if (vks) {
+      e += a0 * vks[r0];
+      if (r1 < n) e += a1 * vks[r1];
+    }
   }
   for (int o = 32; o > 0; o >>= 1) e += __shfl_down(e, o, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
@@ -633,10 +660,10 @@ hipError_t plane_bwd(const FftCtx& c, bool fp32, hipStream_t st) {
   return fp32 ? launch_plane_sel<float>(c, false, st) : launch_plane_sel<double>(c, false, st);
 }
 
-hipError_t rho_reduce(const FftCtx& c, double* dpart, int ndpart, hipStream_t st) {
+hipError_t rho_reduce(const FftCtx& c, double* dpart, int ndpart, hipStream_t st, bool with_v) {
   const size_t n = (size_t)c.g.nx * c.g.ny * c.g.nz;
   hipLaunchKernelGGL(k_rho_reduce, dim3(ndpart), dim3(256), 0, st, c.rho_part, c.nchunk, n,
-                     c.inv_vol, c.vks, c.rho, dpart);
+                     c.inv_vol, with_v ? c.vks : nullptr, c.rho, dpart);
   return hipGetLastError();
 }
 

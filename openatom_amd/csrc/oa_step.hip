@@ -646,7 +646,7 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
   // |F|^2 partials: one per workgroup of the SD kernel or of the rotate that
   // applies Lambda and the SD update together
   c->nfpart = std::max(1024, oa::rotate_nwg(n, K));
-  c->ndpart = 512;
+  c->ndpart = 1024;
   TRY(dalloc(c, &c->fpart, c->nfpart * sizeof(double)));
   TRY(dalloc(c, &c->dpart, c->ndpart * sizeof(double)));
   TRY(dalloc(c, &c->res, sizeof(StepDev)));
@@ -731,7 +731,7 @@ oa_status st_g2r(oa_ctx* c, int ev0 = -1) {
     mk(0);
     HIPCHK(oa::plane3_launch(0, c->nx, c->fp32, fctx(c), c->stream));
     mk(1);
-    HIPCHK(oa::rho_reduce(fctx(c), c->dpart, c->ndpart, c->stream));
+    HIPCHK(oa::rho_reduce(fctx(c), c->dpart, c->ndpart, c->stream, !c->opt.vks_from_rho));
     if (c->opt.vks_from_rho) TRY(st_vks(c));
     mk(2);
   } else if (c->engine == kEngFft2) {
@@ -740,7 +740,7 @@ oa_status st_g2r(oa_ctx* c, int ev0 = -1) {
     HIPCHK(oa::fft2_launch(oa::FFT_YFWD, c->ny, c->fp32, fctx(c), c->stream));
     mk(1);
     HIPCHK(oa::fft2_launch(oa::FFT_XC2R, c->nx, c->fp32, fctx(c), c->stream));
-    HIPCHK(oa::rho_reduce(fctx(c), c->dpart, c->ndpart, c->stream));
+    HIPCHK(oa::rho_reduce(fctx(c), c->dpart, c->ndpart, c->stream, !c->opt.vks_from_rho));
     if (c->opt.vks_from_rho) TRY(st_vks(c));
     mk(2);
   } else if (c->engine == kEngFused) {
@@ -748,7 +748,7 @@ oa_status st_g2r(oa_ctx* c, int ev0 = -1) {
     mk(0);
     HIPCHK(oa::plane_fwd(fctx(c), c->fp32, c->stream));
     mk(1);
-    HIPCHK(oa::rho_reduce(fctx(c), c->dpart, c->ndpart, c->stream));
+    HIPCHK(oa::rho_reduce(fctx(c), c->dpart, c->ndpart, c->stream, !c->opt.vks_from_rho));
     if (c->opt.vks_from_rho) TRY(st_vks(c));
     mk(2);
   } else {
@@ -1099,7 +1099,7 @@ oa_status autotune(oa_ctx* c) {
         },
         [&]() -> oa_status {
           HIPCHK(oa::plane3_launch(0, c->nx, c->fp32, fctx(c), c->stream));
-          HIPCHK(oa::rho_reduce(fctx(c), c->dpart, c->ndpart, c->stream));
+          HIPCHK(oa::rho_reduce(fctx(c), c->dpart, c->ndpart, c->stream, !c->opt.vks_from_rho));
           return OA_OK;
         });
     tune_knob(
@@ -1130,7 +1130,7 @@ oa_status autotune(oa_ctx* c) {
       });
       const double t_sep = time_stage(c, e0, e1, [&]() -> oa_status {
         HIPCHK(oa::plane3_launch(0, c->nx, c->fp32, fctx(c), c->stream));
-        HIPCHK(oa::rho_reduce(fctx(c), c->dpart, c->ndpart, c->stream));
+        HIPCHK(oa::rho_reduce(fctx(c), c->dpart, c->ndpart, c->stream, !c->opt.vks_from_rho));
         HIPCHK(oa::plane3_launch(1, c->nx, c->fp32, fctx(c), c->stream));
         return OA_OK;
       });

@@ -72,15 +72,17 @@ struct Bfly<T, 5, S> {
 };
 
 // One Stockham pass: nbatch transforms of length n, stored at src[tr*ldb+e],
-// result to dst.  tw[j] = exp(-2 pi i j / n).
-template <typename T, int R, int S>
+// result to dst.  tw[j] = exp(-2 pi i j / n).  NT: the workgroup size when
+// known at compile time (0: blockDim.x, which is read from memory).
+template <typename T, int R, int S, int NT = 0>
 __device__ __forceinline__ void fft_pass(const cpx<T>* __restrict__ src, cpx<T>* __restrict__ dst,
                                          int n, int Ns, int nbatch, int ldb,
                                          const cpx<T>* __restrict__ tw) {
   const int m = n / R;
   const int total = m * nbatch;
   const int twstep = n / (Ns * R);
-  for (int g = threadIdx.x; g < total; g += blockDim.x) {
+  const int step = NT > 0 ? NT : (int)blockDim.x;
+  for (int g = threadIdx.x; g < total; g += step) {
     const int tr = g / m;
     const int j = g - tr * m;
     const int k = j % Ns;
@@ -106,17 +108,25 @@ __device__ __forceinline__ void fft_pass(const cpx<T>* __restrict__ src, cpx<T>*
 
 // Full 1D transform on LDS.  Caller has synchronised after filling `a`.
 // Returns the buffer holding the result (a or b); synchronised on return.
-template <typename T, int S>
+template <typename T, int S, int NT = 0>
 __device__ cpx<T>* fft_lds(cpx<T>* a, cpx<T>* b, const FftPlan& pl, int nbatch, int ldb,
                            const cpx<T>* tw) {
+  // the plan's radices read once, all loads issued together (indexing the
+  // plan by the pass number would cost one memory latency per pass)
+  int rad[OA_MAXPASS];
+#pragma unroll
+  for (int q = 0; q < OA_MAXPASS; q++) rad[q] = pl.radix[q];
+  const int npass = pl.npass;
   int Ns = 1;
-  for (int p = 0; p < pl.npass; p++) {
-    const int R = pl.radix[p];
+  for (int p = 0; p < npass; p++) {
+    int R = rad[0];
+#pragma unroll
+    for (int q = 1; q < OA_MAXPASS; q++) R = (p == q) ? rad[q] : R;
     switch (R) {
-      case 2: fft_pass<T, 2, S>(a, b, pl.n, Ns, nbatch, ldb, tw); break;
-      case 3: fft_pass<T, 3, S>(a, b, pl.n, Ns, nbatch, ldb, tw); break;
-      case 4: fft_pass<T, 4, S>(a, b, pl.n, Ns, nbatch, ldb, tw); break;
-      default: fft_pass<T, 5, S>(a, b, pl.n, Ns, nbatch, ldb, tw); break;
+      case 2: fft_pass<T, 2, S, NT>(a, b, pl.n, Ns, nbatch, ldb, tw); break;
+      case 3: fft_pass<T, 3, S, NT>(a, b, pl.n, Ns, nbatch, ldb, tw); break;
+      case 4: fft_pass<T, 4, S, NT>(a, b, pl.n, Ns, nbatch, ldb, tw); break;
+      default: fft_pass<T, 5, S, NT>(a, b, pl.n, Ns, nbatch, ldb, tw); break;
     }
     __syncthreads();
     cpx<T>* t = a; a = b; b = t;
@@ -125,9 +135,10 @@ __device__ cpx<T>* fft_lds(cpx<T>* a, cpx<T>* b, const FftPlan& pl, int nbatch, 
   return a;
 }
 
-template <typename T>
+template <typename T, int NT = 0>
 __device__ __forceinline__ void load_tw(cpx<T>* tw, const double2* __restrict__ g, int n) {
-  for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = {(T)g[i].x, (T)g[i].y};
+  const int step = NT > 0 ? NT : (int)blockDim.x;
+  for (int i = threadIdx.x; i < n; i += step) tw[i] = {(T)g[i].x, (T)g[i].y};
 }
 
 }  // namespace oa

@@ -27,6 +27,10 @@ namespace {
 
 constexpr int kVT = 256;
 constexpr int kVW = kVT / 64;
+// Staging loops run KU iterations per round with every load issued first,
+// from a clamped (valid) index, and the mask applied afterwards: a load in
+// a divergent branch is waited for on its own.
+constexpr int KU = 4;
 
 __device__ __forceinline__ int sfreq(int i, int n) { return i <= (n - 1) / 2 ? i : i - n; }
 
@@ -94,33 +98,48 @@ __global__ __launch_bounds__(kVT) void k_vks_x(VksArgs a) {
   __shared__ double red[kVW * 3];
   const int z = blockIdx.x, p0 = blockIdx.y * a.pr;
   const int np = min(a.pr, (a.ny + 1) / 2 - p0);
-  load_tw(tw, a.twx, a.nx);
+  load_tw<double, kVT>(tw, a.twx, a.nx);
   const double sc = a.vol / ((double)a.nx * a.ny * a.nz);
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int i = threadIdx.x; i < a.pr * a.nx; i += kVT) {
-    const int q = i / a.nx, x = i - q * a.nx;
-    cpx<double> val = {0.0, 0.0};
-    if (q < np) {
+  const int cnt = a.pr * a.nx;
+  for (int i0 = threadIdx.x; i0 < cnt; i0 += KU * kVT) {
+    double rv[KU][2];
+    size_t iv[KU][2];
+    bool ok[KU][2];
+#pragma unroll
+    for (int u = 0; u < KU; u++) {
+      const int i = i0 + u * kVT, q = i / a.nx, x = i - q * a.nx;
 #pragma unroll
       for (int h = 0; h < 2; h++) {
         const int y = 2 * (p0 + q) + h;
-        if (y < a.ny) {
-          const size_t idx = ((size_t)z * a.ny + y) * a.nx + x;
-          const double r = a.rho[idx];
+        ok[u][h] = i < cnt && q < np && y < a.ny;
+        iv[u][h] = ok[u][h] ? ((size_t)z * a.ny + y) * a.nx + x : 0;
+        rv[u][h] = a.rho[iv[u][h]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < KU; u++) {
+      const int i = i0 + u * kVT;
+      if (i >= cnt) break;
+      cpx<double> val = {0.0, 0.0};
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        if (ok[u][h]) {
+          const double r = rv[u][h];
           double xf, cf, mf;
           pz_lda(r, xf, cf, mf);
-          a.v[idx] = xf + mf;
+          a.v[iv[u][h]] = xf + mf;
           acc[0] += xf * r;
           acc[1] += cf * r;
           acc[2] += (xf + mf) * r;
           if (h == 0) val.x = r * sc; else val.y = r * sc;
         }
       }
+      A[i] = val;
     }
-    A[i] = val;
   }
   __syncthreads();
-  const cpx<double>* R = fft_lds<double, -1>(A, B, a.px, a.pr, a.nx, tw);
+  const cpx<double>* R = fft_lds<double, -1, kVT>(A, B, a.px, a.pr, a.nx, tw);
   // Z = X_a + i X_b: X_a(k) = (Z_k + conj Z_-k)/2, X_b(k) = (Z_k - conj Z_-k)/(2i)
   const int rows = 2 * a.pr;
   for (int i = threadIdx.x; i < a.nkx * rows; i += kVT) {
@@ -156,23 +175,34 @@ __global__ __launch_bounds__(kVT) void k_vks_y(VksArgs a) {
   cpx<double>* B = A + a.cb * a.ny;
   const int z = blockIdx.x, k0 = blockIdx.y * a.cb;
   const int nc = min(a.cb, a.nkx - k0);
-  load_tw(tw, a.twy, a.ny);
-  for (int i = threadIdx.x; i < a.cb * a.ny; i += kVT) {
-    const int c = i / a.ny, y = i - c * a.ny;
-    double2 val = make_double2(0.0, 0.0);
-    if (c < nc) {
+  load_tw<double, kVT>(tw, a.twy, a.ny);
+  const int cnt = a.cb * a.ny;
+  for (int i0 = threadIdx.x; i0 < cnt; i0 += KU * kVT) {
+    double2 v[KU];
+    bool ok[KU];
+#pragma unroll
+    for (int u = 0; u < KU; u++) {
+      const int i = i0 + u * kVT, c = i / a.ny, y = i - c * a.ny;
       const size_t col = (size_t)z * a.nkx + k0 + c;
+      size_t idx;
       if (S < 0) {
-        val = a.hx[col * a.ny + y];
+        ok[u] = i < cnt && c < nc;
+        idx = col * a.ny + y;
       } else {
         const int ky = sfreq(y, a.ny);
-        if (ky >= -a.kyr && ky <= a.kyr) val = a.hl[col * a.nky + ky + a.kyr];
+        ok[u] = i < cnt && c < nc && ky >= -a.kyr && ky <= a.kyr;
+        idx = col * a.nky + ky + a.kyr;
       }
+      v[u] = (S < 0 ? a.hx : a.hl)[ok[u] ? idx : 0];
     }
-    A[i] = {val.x, val.y};
+#pragma unroll
+    for (int u = 0; u < KU; u++) {
+      const int i = i0 + u * kVT;
+      if (i < cnt) A[i] = ok[u] ? cpx<double>{v[u].x, v[u].y} : cpx<double>{0.0, 0.0};
+    }
   }
   __syncthreads();
-  const cpx<double>* R = fft_lds<double, S>(A, B, a.py, a.cb, a.ny, tw);
+  const cpx<double>* R = fft_lds<double, S, kVT>(A, B, a.py, a.cb, a.ny, tw);
   if (S < 0) {
     for (int i = threadIdx.x; i < a.cb * a.nky; i += kVT) {
       const int c = i / a.nky, j = i - c * a.nky;
@@ -203,27 +233,38 @@ __global__ __launch_bounds__(kVT) void k_vks_lines(VksArgs a) {
   __shared__ double red[kVW * 2];
   const int kx = blockIdx.x, j0 = blockIdx.y * a.lb;
   const int nl = min(a.lb, a.nky - j0);
-  load_tw(tw, a.twz, a.nz);
-  for (int i = threadIdx.x; i < a.lb * a.nz; i += kVT) {
-    const int z = i / a.lb, l = i - z * a.lb;
-    double2 val = make_double2(0.0, 0.0);
-    if (l < nl) val = a.hl[((size_t)z * a.nkx + kx) * a.nky + j0 + l];
-    A[l * a.nz + z] = {val.x, val.y};
+  load_tw<double, kVT>(tw, a.twz, a.nz);
+  const int cnt = a.lb * a.nz;
+  for (int i0 = threadIdx.x; i0 < cnt; i0 += KU * kVT) {
+    double2 v[KU];
+    bool ok[KU];
+#pragma unroll
+    for (int u = 0; u < KU; u++) {
+      const int i = i0 + u * kVT, z = i / a.lb, l = i - z * a.lb;
+      ok[u] = i < cnt && l < nl;
+      v[u] = a.hl[ok[u] ? ((size_t)z * a.nkx + kx) * a.nky + j0 + l : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < KU; u++) {
+      const int i = i0 + u * kVT, z = i / a.lb, l = i - z * a.lb;
+      if (i < cnt) A[l * a.nz + z] = ok[u] ? cpx<double>{v[u].x, v[u].y} : cpx<double>{0.0, 0.0};
+    }
   }
   __syncthreads();
-  cpx<double>* R = fft_lds<double, -1>(A, B, a.pz, a.lb, a.nz, tw);
+  cpx<double>* R = fft_lds<double, -1, kVT>(A, B, a.pz, a.lb, a.nz, tw);
   cpx<double>* O = (R == A) ? B : A;
   const double fpi = 4.0 * 3.14159265358979323846;
   const double w = kx == 0 ? 1.0 : 2.0;
   double acc[2] = {0.0, 0.0};
-  for (int i = threadIdx.x; i < a.lb * a.nz; i += kVT) {
+  for (int i = threadIdx.x; i < cnt; i += kVT) {
     const int l = i / a.nz, z = i - l * a.nz;
+    // V_loc loaded before the branch (clamped row): not waited for alone
+    const double2 vl = a.vloc[((size_t)kx * a.nky + j0 + min(l, nl - 1)) * a.nz + z];
     cpx<double> out = {0.0, 0.0};
     if (l < nl) {
       const int ky = j0 + l - a.kyr, kz = sfreq(z, a.nz);
       const double g2 = gsq(a.hmati, kx, ky, kz);
       const cpx<double> r = R[i];
-      const double2 vl = a.vloc[((size_t)kx * a.nky + j0 + l) * a.nz + z];
       if (g2 == 0.0) {
         out = {vl.x, 0.0};
         acc[1] += vl.x * r.x;
@@ -237,7 +278,7 @@ __global__ __launch_bounds__(kVT) void k_vks_lines(VksArgs a) {
     O[i] = out;
   }
   __syncthreads();
-  const cpx<double>* R2 = fft_lds<double, 1>(O, R, a.pz, a.lb, a.nz, tw);
+  const cpx<double>* R2 = fft_lds<double, 1, kVT>(O, R, a.pz, a.lb, a.nz, tw);
   for (int i = threadIdx.x; i < a.lb * a.nz; i += kVT) {
     const int z = i / a.lb, l = i - z * a.lb;
     if (l < nl) {
@@ -264,14 +305,23 @@ __global__ __launch_bounds__(kVT) void k_vks_xb(VksArgs a) {
   const int z = blockIdx.x, p0 = blockIdx.y * a.pr;
   const int np = min(a.pr, (a.ny + 1) / 2 - p0);
   const int rows = 2 * a.pr;
-  load_tw(tw, a.twx, a.nx);
+  load_tw<double, kVT>(tw, a.twx, a.nx);
   // B[k][yy] = hx[z][k][2 p0 + yy] (coalesced along y)
-  for (int i = threadIdx.x; i < a.nkx * rows; i += kVT) {
-    const int k = i / rows, yy = i - k * rows;
-    const int y = 2 * p0 + yy;
-    double2 val = make_double2(0.0, 0.0);
-    if ((yy >> 1) < np && y < a.ny) val = a.hx[((size_t)z * a.nkx + k) * a.ny + y];
-    B[i] = {val.x, val.y};
+  const int cnt = a.nkx * rows;
+  for (int i0 = threadIdx.x; i0 < cnt; i0 += KU * kVT) {
+    double2 v[KU];
+    bool ok[KU];
+#pragma unroll
+    for (int u = 0; u < KU; u++) {
+      const int i = i0 + u * kVT, k = i / rows, yy = i - k * rows, y = 2 * p0 + yy;
+      ok[u] = i < cnt && (yy >> 1) < np && y < a.ny;
+      v[u] = a.hx[ok[u] ? ((size_t)z * a.nkx + k) * a.ny + y : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < KU; u++) {
+      const int i = i0 + u * kVT;
+      if (i < cnt) B[i] = ok[u] ? cpx<double>{v[u].x, v[u].y} : cpx<double>{0.0, 0.0};
+    }
   }
   __syncthreads();
   // Z_k = X_a(k) + i X_b(k), Z_-k = conj X_a(k) + i conj X_b(k)
@@ -294,14 +344,28 @@ __global__ __launch_bounds__(kVT) void k_vks_xb(VksArgs a) {
     A[i] = zv;
   }
   __syncthreads();
-  const cpx<double>* R = fft_lds<double, 1>(A, B, a.px, a.pr, a.nx, tw);
-  for (int i = threadIdx.x; i < a.pr * a.nx; i += kVT) {
-    const int q = i / a.nx, x = i - q * a.nx;
-    if (q < np) {
-      const int y = 2 * (p0 + q);
-      const cpx<double> r = R[i];
-      a.v[((size_t)z * a.ny + y) * a.nx + x] += r.x;
-      if (y + 1 < a.ny) a.v[((size_t)z * a.ny + y + 1) * a.nx + x] += r.y;
+  const cpx<double>* R = fft_lds<double, 1, kVT>(A, B, a.px, a.pr, a.nx, tw);
+  const int cnt2 = a.pr * a.nx;
+  for (int i0 = threadIdx.x; i0 < cnt2; i0 += KU * kVT) {
+    double v0[KU], v1[KU];
+    size_t i0v[KU], i1v[KU];
+#pragma unroll
+    for (int u = 0; u < KU; u++) {  // V_xc of both rows, loads first
+      const int i = i0 + u * kVT, q = i / a.nx, x = i - q * a.nx, y = 2 * (p0 + q);
+      const bool ok = i < cnt2 && q < np;
+      i0v[u] = ok ? ((size_t)z * a.ny + y) * a.nx + x : 0;
+      i1v[u] = (ok && y + 1 < a.ny) ? ((size_t)z * a.ny + y + 1) * a.nx + x : 0;
+      v0[u] = a.v[i0v[u]];
+      v1[u] = a.v[i1v[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < KU; u++) {
+      const int i = i0 + u * kVT, q = i / a.nx, y = 2 * (p0 + q);
+      if (i < cnt2 && q < np) {
+        const cpx<double> r = R[i];
+        a.v[i0v[u]] = v0[u] + r.x;
+        if (y + 1 < a.ny) a.v[i1v[u]] = v1[u] + r.y;
+      }
     }
   }
 }
