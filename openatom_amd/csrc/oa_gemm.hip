@@ -293,6 +293,68 @@ __device__ __forceinline__ void gram_full_wave(const GramArgs& a, const double* 
       }
 }
 
+// Skinny block (nb <= 32 columns, e.g. the non-local projections of one
+// atom set): wave w owns rows [32 w, 32 w + 32) of the 128-row block and
+// both 16-column tiles, so no wave idles on absent columns.
+__device__ __forceinline__ void gram_skinny_wave(const GramArgs& a, const double* __restrict__ B,
+                                                 const GSlice& s, int sp, int bi) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, l16 = lane & 15, g = lane >> 4;
+  const int rbase = bi * kGB + wave * 32;
+  if (rbase >= a.n) return;
+  int ra[2], rb[2];
+#pragma unroll
+  for (int x = 0; x < 2; x++) {  // rows past the matrices: outputs not stored
+    const int r = rbase + 16 * x + l16, q = 16 * x + l16;
+    ra[x] = r < a.n ? r : 0;
+    rb[x] = q < a.nb ? q : 0;
+  }
+  v4d acc[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++) acc[x][y] = vzero();
+  auto mfmas = [&](const double (&f)[4][4]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+#pragma unroll
+      for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) acc[x][y] = mfma16(f[x][e], f[2 + y][e], acc[x][y]);
+  };
+  Frag<4> ring[kGramRing];  // fragments 0-1: rows of A, 2-3: rows of B
+  chunk_ring(
+      s.c0, s.c1, ring,
+      [&](int c, Frag<4>& d) __attribute__((always_inline)) {
+        const int kk = s.base + 16 * c + 4 * g;
+#pragma unroll
+        for (int x = 0; x < 2; x++) {
+          gfrag(a.A, a.lda, ra[x], kk, d.f[x]);
+          gfrag(B, a.lda, rb[x], kk, d.f[2 + x]);
+        }
+      },
+      [&](int c, const Frag<4>& d) __attribute__((always_inline)) {
+        const int k0 = s.base + 16 * c;
+        if (k0 >= s.rs && k0 + 16 <= s.re) {
+          mfmas(d.f);
+        } else {  // a chunk at the region's edge (padding past K may hold anything)
+          Frag<4> m = d;
+#pragma unroll
+          for (int x = 0; x < 4; x++) gmask(k0 + 4 * g, s.rs, s.re, m.f[x]);
+          mfmas(m.f);
+        }
+      });
+  double* part = a.part + (size_t)sp * a.n * a.nb;
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = rbase + 16 * x + g + 4 * v, col = 16 * y + l16;
+        if (r < a.n && col < a.nb) part[(size_t)r * a.nb + col] = s.w * acc[x][y][v];
+      }
+}
+
 // grid (output blocks, K slices); nbe = row blocks (blocks per edge when
 // symmetric), nbe_b = column blocks, nlo = slices of the kx=0 region
 __global__ __launch_bounds__(256) void k_gram(GramArgs a, int nbe, int nbe_b, int nlo) {
@@ -317,6 +379,8 @@ __global__ __launch_bounds__(256) void k_gram(GramArgs a, int nbe, int nbe_b, in
       case 2: gram_diag_wave<2>(a, s, sp, bi); break;
       default: gram_diag_wave<3>(a, s, sp, bi); break;
     }
+  } else if (!a.sym && a.nb <= 32) {
+    gram_skinny_wave(a, a.B, s, sp, bi);
   } else {
     gram_full_wave(a, a.sym ? a.A : a.B, s, sp, bi, bj);
   }
