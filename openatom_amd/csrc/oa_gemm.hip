@@ -664,18 +664,23 @@ __global__ __launch_bounds__(kRotThreads) void k_rotate_p(RotArgs a) {
   __syncthreads();
   const int jb = (wave & 3) * 32, kt0 = 2 * (wave >> 2);
   const bool wact = jb < a.n;
-  const int nkb = (a.K + kRK - 1) / kRK;
-  const int nmine =
-      (int)blockIdx.x < nkb ? (nkb - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x : 0;
+  // Workgroup b owns the 16-column tiles [t0, t1), an even split of all
+  // tiles (so at most one tile more than any other workgroup), taken in
+  // blocks of four; tiles past t1 in a last partial block cost their waves
+  // no MFMAs and are not stored.
+  const int ntile = (a.K + 15) >> 4;
+  const int t0 = (int)(((long long)blockIdx.x * ntile) / gridDim.x);
+  const int t1 = (int)(((long long)(blockIdx.x + 1) * ntile) / gridDim.x);
+  const int nmine = (t1 - t0 + 3) >> 2;
   constexpr int RING = rot_ring(USE_Y, USE_SD);
   constexpr int G = NCH / gcd_c(NCH, RING) * RING;
   const int nitems = nmine * NCH;
-  auto kblock = [&](int q) { return (int)blockIdx.x + (q / NCH) * (int)gridDim.x; };
+  auto kcol = [&](int q) { return (t0 + 4 * (q / NCH)) * 16; };  // first column of q's block
   // Branch-free loads (clamped to valid rows and columns) so that the
   // ring's loads stay in flight: rows i >= ni meet zero rows of the staged
   // M, columns k >= K produce outputs that are not stored.
   auto load = [&](int q, XFrag& d) __attribute__((always_inline)) {
-    const int kb = kblock(q) * kRK, ic = q % NCH;
+    const int kb = kcol(q), ic = q % NCH;
 #pragma unroll
     for (int s = 0; s < 4; s++) {
       const int i = min(ic * 16 + 4 * s + g, a.ni - 1);
@@ -722,7 +727,7 @@ __global__ __launch_bounds__(kRotThreads) void k_rotate_p(RotArgs a) {
           const double s = (k < a.k_lo) ? a.s_lo : a.s_hi;
           double val = s * acc[x][y][v];
           if (USE_Y) val = a.beta * yold[USE_Y ? x : 0][y][v] + val;
-          if (j < a.n && k < a.K) {
+          if (j < a.n && k < a.K && (kb >> 4) + kt0 + y < t1) {
             a.Y[(size_t)j * a.ldx + k] = val;
             if (USE_SD)
               a.sd_psi[(size_t)j * a.ldx + k] = pold[USE_SD ? x : 0][y][v] + dm[y] * val;
@@ -748,21 +753,24 @@ __global__ __launch_bounds__(kRotThreads) void k_rotate_p(RotArgs a) {
         constexpr int d = decltype(dc)::value;
         constexpr int sl = d % RING, ic = d % NCH;  // q0 is a multiple of G
         const int q = q0 + d;
-        if (wact) {
+        const int tb = t0 + 4 * (q / NCH) + kt0;  // this wave's first tile of the block
+        const bool v0 = wact && tb < t1, v1 = wact && tb + 1 < t1;
+        if (v0) {
 #pragma unroll
           for (int s = 0; s < 4; s++) {
             const double* mrow = msm + (ic * 16 + 4 * s + g) * kRotLd + jb + l16;
             const double am0 = mrow[0], am1 = mrow[16];
-#pragma unroll
-            for (int y = 0; y < 2; y++) {
-              acc[0][y] = mfma16(am0, ring[sl].v[s][y], acc[0][y]);
-              acc[1][y] = mfma16(am1, ring[sl].v[s][y], acc[1][y]);
+            acc[0][0] = mfma16(am0, ring[sl].v[s][0], acc[0][0]);
+            acc[1][0] = mfma16(am1, ring[sl].v[s][0], acc[1][0]);
+            if (v1) {
+              acc[0][1] = mfma16(am0, ring[sl].v[s][1], acc[0][1]);
+              acc[1][1] = mfma16(am1, ring[sl].v[s][1], acc[1][1]);
             }
           }
         }
-        if (ic == PRE) preload(kblock(q) * kRK);
+        if (ic == PRE) preload(kcol(q));
         load(q + RING, ring[sl]);
-        if (ic == NCH - 1 && q < nitems) epilogue(kblock(q) * kRK);
+        if (ic == NCH - 1 && q < nitems) epilogue(kcol(q));
       });
     }
   }
