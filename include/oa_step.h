@@ -297,7 +297,10 @@ oa_status oa_step_result(oa_ctx *ctx, oa_step_out *out);
  * [6] EKE/E_loc reductions + Lambda (for n <= 128 with the SD update in the
  * same pass), [7] SD (or its |F| reduction) + redundancy sync, [8] S,
  * [9] Newton-Schulz, [10] Psi T, [11] residual, [12] whole step.
- * Fills min(n, OA_NTIMINGS) entries. */
+ * Fills min(n, OA_NTIMINGS) entries.  oa_set_timing(ctx, 2) records only
+ * [3] and [12] and reports 0 for the other entries: every recorded event
+ * delays the stream by a few microseconds, so level 1 slows the step it
+ * measures. */
 #define OA_NTIMINGS 13
 oa_status oa_timings(oa_ctx *ctx, double *ms, int n);
 oa_status oa_set_timing(oa_ctx *ctx, int on);

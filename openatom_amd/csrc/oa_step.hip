@@ -159,6 +159,7 @@ struct oa_ctx {
   StepDev* res_host = nullptr;
   FftCtx fc{};
   bool timing = false;
+  int timing_level = 1;  // 1: every stage, 2: the step and the R->G plane pass
   // ring of per-step event sets so a run of steps can be timed without host
   // synchronisation inside it
   static constexpr int kRing = 64;
@@ -703,8 +704,15 @@ FftCtx& fctx(oa_ctx* c) {
 // Timing marks inside oa_step: ev[0] start, ev[1..6] after each FFT stage,
 // ev[7] reductions + Lambda, ev[8] SD + sync, ev[9] S, ev[10] NS,
 // ev[11] Psi T, ev[12] residual (end).
+// Timing level 2 records only the step's ends and the R->G plane pass
+// (marks 0, 3, 4, 12): each recorded event costs the stream a few
+// microseconds, which level 1 pays at every stage boundary.
+bool mark_on(const oa_ctx* c, int i) {
+  return c->timing_level == 1 || i == 0 || i == 3 || i == 4 || i == 12;
+}
+
 void mark(oa_ctx* c, int i) {
-  if (c->timing && i >= 0)
+  if (c->timing && i >= 0 && mark_on(c, i))
     (void)hipEventRecord(c->ev[(size_t)c->ev_cur * OA_NTIMINGS + i], c->stream);
 }
 
@@ -1843,8 +1851,9 @@ oa_status oa_step_result(oa_ctx* c, oa_step_out* out) {
 }
 
 oa_status oa_set_timing(oa_ctx* c, int on) {
-  if (!c) return OA_EINVAL;
+  if (!c || on < 0 || on > 2) return OA_EINVAL;
   HIPCHK(hipSetDevice(c->device));
+  if (on) c->timing_level = on;
   if (on && c->ev.empty()) {
     c->ev.assign((size_t)oa_ctx::kRing * OA_NTIMINGS, nullptr);
     for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
@@ -1865,6 +1874,10 @@ oa_status oa_timings(oa_ctx* c, double* ms, int n) {
   for (int i = 0; i < n && i < OA_NTIMINGS; i++) {
     const int a = (i == OA_NTIMINGS - 1) ? 0 : i;
     const int b = (i == OA_NTIMINGS - 1) ? 12 : i + 1;
+    if (!mark_on(c, a) || !mark_on(c, b)) {  // not recorded at this level
+      ms[i] = 0.0;
+      continue;
+    }
     double acc = 0.0;
     for (int k = 0; k < nsteps; k++) {
       const size_t base = (size_t)((c->ev_count - 1 - k) % oa_ctx::kRing) * OA_NTIMINGS;

@@ -272,8 +272,10 @@ class CPContext:
     def sync(self):
         self._chk(self._L.oa_sync(self.h))
 
-    def set_timing(self, on=True):
-        self._chk(self._L.oa_set_timing(self.h, int(bool(on))))
+    def set_timing(self, on=True, level=1):
+        """level 1: every stage; level 2: only the step and the R->G plane
+        pass (fewer events in the stream, so the step runs at full speed)."""
+        self._chk(self._L.oa_set_timing(self.h, int(level) if on else 0))
 
     TIMING_NAMES = ("k_zfwd", "k_y_fwd", "k_xc2r", "k_xr2c", "k_y_bwd", "k_zbwd",
                     "reduce_lambda", "sd_sync", "overlap", "ns", "rotate", "resid", "step")
