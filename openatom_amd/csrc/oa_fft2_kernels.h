@@ -41,6 +41,9 @@ __global__ __launch_bounds__(kThreads) void k2_zfwd(FftCtx c) {
   const int s = blockIdx.y, l0 = blockIdx.x * NTR;
   const int nl = min(NTR, c.g.nlines - l0);
   const int tid = threadIdx.x;
+  // pass 2's zbuf column, loaded up front (behind pass 1)
+  const int tr2 = tid % NTR, j2 = tid / NTR;
+  const int w2 = c.lines[l0 + min(tr2, nl - 1)].w;
   {  // pass 1: lanes along z (contiguous sphere points)
     const int tr = tid / G, j = tid - tr * G;
     if (tr < nl && j < R2) {
@@ -63,7 +66,7 @@ __global__ __launch_bounds__(kThreads) void k2_zfwd(FftCtx c) {
   }
   __syncthreads();
   {  // pass 2: lanes along lines (adjacent (kx,ky) columns of zbuf)
-    const int tr = tid % NTR, j = tid / NTR;
+    const int tr = tr2, j = j2;
     if (tr < nl && j < R1) {
       cpx<T> u[R2];
 #pragma unroll
@@ -72,7 +75,7 @@ __global__ __launch_bounds__(kThreads) void k2_zfwd(FftCtx c) {
       for (int t = 1; t < R2; t++) u[t] = cm(u[t], twiddle<T, -1>(c.twz, t * j));
       Dft<T, R2, -1>::run(u);
       const size_t plane = (size_t)c.g.nplane_x * c.g.ny;
-      cpx<T>* zs = (cpx<T>*)c.zbuf + (size_t)s * c.g.nz * plane + c.lines[l0 + tr].w;
+      cpx<T>* zs = (cpx<T>*)c.zbuf + (size_t)s * c.g.nz * plane + w2;
 #pragma unroll
       for (int t = 0; t < R2; t++) zs[(size_t)(j + t * R1) * plane] = u[t];
     }
@@ -89,6 +92,26 @@ __global__ __launch_bounds__(kThreads) void k2_zbwd(FftCtx c) {
   const int nl = min(NTR, c.g.nlines - l0);
   const int tid = threadIdx.x;
   const size_t plane = (size_t)c.g.nplane_x * c.g.ny;
+  // pass 2's operands (twiddles, the line, kinetic factors and old psi),
+  // loaded up front so that their latency hides behind pass 1
+  const int tr2 = tid / G, j2 = tid - tr2 * G;
+  cpx<T> tw2[R2];
+#pragma unroll
+  for (int t = 1; t < R2; t++) tw2[t] = twiddle<T, +1>(c.twz, t * min(j2, R1 - 1));
+  const int4 L = c.lines[l0 + min(tr2, nl - 1)];
+  const double oc = c.occ[s];
+  double gq[R2];
+  double2 cq[R2];
+  {
+    const double2* ps = c.psi + (size_t)s * c.g.ld + L.x;
+    const double* g2 = c.g2 + L.x;
+#pragma unroll
+    for (int t = 0; t < R2; t++) {  // branch-free, clamped to the line
+      const int dc = min(max(signed_k(j2 + t * R1, N) - L.z, 0), L.y - 1);
+      gq[t] = g2[dc];
+      cq[t] = ps[dc];
+    }
+  }
   {  // pass 1: lanes along lines
     const int tr = tid % NTR, j = tid / NTR;
     if (tr < nl && j < R2) {
@@ -104,30 +127,16 @@ __global__ __launch_bounds__(kThreads) void k2_zbwd(FftCtx c) {
   __syncthreads();
   double eke = 0.0;
   {  // pass 2: lanes along z, pack to the sphere
-    const int tr = tid / G, j = tid - tr * G;
+    const int tr = tr2, j = j2;
     if (tr < nl && j < R1) {
       cpx<T> u[R2];
 #pragma unroll
       for (int t = 0; t < R2; t++) u[t] = ex[tr * N + j + t * R1];
 #pragma unroll
-      for (int t = 1; t < R2; t++) u[t] = cm(u[t], twiddle<T, +1>(c.twz, t * j));
+      for (int t = 1; t < R2; t++) u[t] = cm(u[t], tw2[t]);
       Dft<T, R2, +1>::run(u);
-      const int4 L = c.lines[l0 + tr];
       const double w = (L.w < c.g.ny) ? 1.0 : 2.0;  // kx = 0 lines have dst < ny
-      const double oc = c.occ[s];
-      const double2* ps = c.psi + (size_t)s * c.g.ld + L.x;
       double2* fs = c.force + (size_t)s * c.g.ld + L.x;
-      const double* g2 = c.g2 + L.x;
-      // operands loaded branch-free (clamped to the line), then the
-      // masked update: a load in a divergent branch is waited for on its own
-      double gq[R2];
-      double2 cq[R2];
-#pragma unroll
-      for (int t = 0; t < R2; t++) {
-        const int dc = min(max(signed_k(j + t * R1, N) - L.z, 0), L.y - 1);
-        gq[t] = g2[dc];
-        cq[t] = ps[dc];
-      }
 #pragma unroll
       for (int t = 0; t < R2; t++) {
         const int d = signed_k(j + t * R1, N) - L.z;
