@@ -82,8 +82,13 @@ typedef struct {
                           the second call; the Newton-Schulz launch stays outside); stage
                           timings then read: [0] everything before Newton-Schulz, [8]
                           Newton-Schulz, [9] Psi T (+ residual), [12] whole step */
-  int ns_mode;         /* 0: one cooperative launch when all tiles fit (default),
-                          1: two launches per iteration with a device-side early exit */
+  int ns_mode;         /* 0: one cooperative launch when all workgroups fit (default; the
+                          autotuner picks among the forms below that fit),
+                          1: two launches per iteration with a device-side early exit,
+                          2: one cooperative launch over 32 x 32 tiles,
+                          3, 4, 5: one cooperative launch over 16-row blocks, each split
+                          over 1, 2 or 4 workgroups by 16-column tiles (4 and 5 need
+                          n > 16 and n > 48); 2..5 give OA_EINVAL where they do not fit */
   int vks_from_rho;    /* 1: each step builds V(r) = V_KS[rho] from its own density
                           (PZ-LDA exchange-correlation, Hartree, local pseudopotential of
                           oa_local_pseudo_set) between rho and V psi, as the density chares

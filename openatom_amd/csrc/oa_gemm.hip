@@ -1163,137 +1163,245 @@ static hipError_t ns_run_launches(const NsArgs& a, hipStream_t st) {
 
 // ---- block-row variant: one grid barrier per iteration
 //
-// Workgroup b owns rows R = [16b, 16b + 16) of C, X and T.  X, T and C are
-// polynomials in S and commute, so the new X = X C / 2 of the recurrence
-// equals C X / 2 (up to rounding), and both new row blocks,
-//   X'[R,:] = C[R,:] X / 2,  T'[R,:] = C[R,:] T / 2,
-// need only this block's rows of C = 3I - T[R,:] X and the previous X and T.
-// C stays in LDS; the rows of T stay in LDS across iterations; X and T go
-// through global memory (double buffered) for the other blocks.  Wave w
-// handles the 16-column tiles w, w + 8, ...  MFMA operands: A from LDS, B
-// loaded branch-free (clamped, masked) from global memory.
+// Workgroup (b, h) owns rows R = [16b, 16b + 16) and the 16-column tiles
+// ct = h, h + split, h + 2 split, ... of X and T (split = gridDim.y).  X, T
+// and C are polynomials in S and commute, so the new X = X C / 2 of the
+// recurrence equals C X / 2 (up to rounding), and the new tiles
+//   X'[R,ct] = C[R,:] X[:,ct] / 2,  T'[R,ct] = C[R,:] T[:,ct] / 2
+// need only C[R,:] = 3I - T[R,:] X, which every workgroup of the row block
+// computes whole, and the previous X and T.  C[R,:] and T[R,:] sit in LDS;
+// X and T go through global memory (double buffered) for the other
+// workgroups.  Iteration 1 starts from T0 = I and X0 = S/2 (ortho.C:236-239):
+// there C = 3I - S/2 and T1 = C/2 need no products, so no barrier precedes
+// it.  After each barrier the next iteration's operands are loaded before
+// the convergence test reads the error partials, so the two latencies
+// overlap.  MFMA operands: A from LDS, B from global memory in batches of KB
+// k-steps, branch-free (clamped addresses, masked values): a load that only
+// some paths use is waited for on its own.
 constexpr int kNrWaves = 8;
-__host__ __device__ constexpr int nr_ld(int n) { return ((n + 3) & ~3) + 2; }
+constexpr int kNrMaxWg = 256;  // error partials staged in LDS
+// k-steps per B batch (two batches in registers at a time)
+__host__ __device__ constexpr int nr_kb(int n) { return n <= 32 ? 8 : 16; }
+// LDS row length: whole batches of k-steps (columns past n zero), + 2
+// against bank conflicts
+__host__ __device__ constexpr int nr_ld(int n) {
+  return 4 * ((((n + 3) >> 2) + nr_kb(n) - 1) / nr_kb(n) * nr_kb(n)) + 2;
+}
 size_t ns_rows_smem(int n) { return (size_t)2 * 16 * nr_ld(n) * sizeof(double); }
 
-// acc1 += A[16 lds rows, k] B1[k][col tile ct] (and acc2 with B2 when TWO)
-// over k in [0, n): the B values of 32 k-steps (n <= 128: all of them) are
-// loaded at once, branch-free (clamped, masked), then the MFMAs issue
-template <bool TWO>
-__device__ __forceinline__ void nr_tile(const double* __restrict__ Al, int ld,
-                                        const double* __restrict__ B1,
-                                        const double* __restrict__ B2, int n, int ct, v4d& acc1,
-                                        v4d& acc2) {
-  constexpr int KB = 32;
-  const int lane = threadIdx.x & 63, l16 = lane & 15, g = lane >> 4;
-  const int col = ct * 16 + l16, colc = min(col, n - 1);
-  const int nk = (n + 3) >> 2;
-  for (int k0 = 0; k0 < nk; k0 += KB) {
-    double b1[KB], b2[TWO ? KB : 1];
+// b[u] = scale * B[k][col] for the k-steps k0 .. k0 + KB - 1 of this lane
+// group (k = 4 (k0 + u) + g): each load covers four rows of 16 contiguous
+// columns.  k past n is clamped to the last row (finite values that meet
+// zero A columns), col past n to the last column (results not stored): no
+// per-k masks, which the compiler would hoist as scalar pairs and spill.
+// Buffer loads: a scalar resource and one 32-bit offset register per load
+// (n * n * 8 < 2^31), where flat addressing holds a register pair each.
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+__device__ __forceinline__ rsrc_t nr_rsrc(const double* p, int n) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(p), (short)0, n * n * 8,
+                                           0x00020000);  // raw buffer, gfx9 data format
+}
+
+template <int KB>
+__device__ __forceinline__ void nr_load(double (&b)[KB], rsrc_t B, int col, int n, int k0,
+                                        double scale) {
+  const int g = (threadIdx.x & 63) >> 4;
+  const int colc = min(col, n - 1);
 #pragma unroll
-    for (int u = 0; u < KB; u++) {
-      const int k = (k0 + u) * 4 + g;
-      const size_t idx = (size_t)min(k, n - 1) * n + colc;
-      const bool in = k < n && col < n;
-      const double v1 = B1[idx];
-      b1[u] = in ? v1 : 0.0;
-      if (TWO) {
-        const double v2 = B2[idx];
-        b2[TWO ? u : 0] = in ? v2 : 0.0;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < KB; u++)
-      if (k0 + u < nk) {
-        const double av = Al[l16 * ld + (k0 + u) * 4 + g];
-        acc1 = mfma16(av, b1[u], acc1);
-        if (TWO) acc2 = mfma16(av, b2[TWO ? u : 0], acc2);
-      }
+  for (int u = 0; u < KB; u++) {
+    const int k = min((k0 + u) * 4 + g, n - 1);
+    const double v = __builtin_bit_cast(
+        double, __builtin_amdgcn_raw_buffer_load_b64(B, (k * n + colc) * 8, 0, 0));
+    b[u] = scale * v;
   }
 }
 
+// acc += A[16 LDS rows][k] b over one batch (A columns past n are zero)
+template <int KB>
+__device__ __forceinline__ v4d nr_mma(const double* __restrict__ Al, int ld, const double (&b)[KB],
+                                      int k0, v4d acc) {
+  const int lane = threadIdx.x & 63, l16 = lane & 15, g = lane >> 4;
+  const double* __restrict__ a = Al + l16 * ld + 4 * k0 + g;
+#pragma unroll
+  for (int u = 0; u < KB; u++) acc = mfma16(a[4 * u], b[u], acc);
+  return acc;
+}
+
+// A[16 LDS rows][0, n) B[0, n)[col tile]; b holds the first batch on entry
+template <int KB>
+__device__ __forceinline__ v4d nr_product(const double* __restrict__ Al, int ld, double (&b)[KB],
+                                          rsrc_t B, int col, int n, double scale) {
+  const int nk = (n + 3) >> 2;
+  v4d acc = vzero();
+#pragma nounroll
+  for (int k0 = 0;; k0 += KB) {
+    // the next batch is in flight while this one's MFMAs issue
+    double nx[KB];
+    const bool more = k0 + KB < nk;
+    if (more) nr_load<KB>(nx, B, col, n, k0 + KB, scale);
+    acc = nr_mma<KB>(Al, ld, b, k0, acc);
+    if (!more) break;
+#pragma unroll
+    for (int u = 0; u < KB; u++) b[u] = nx[u];
+  }
+  return acc;
+}
+
+template <int KB>
 __global__ __launch_bounds__(kNrWaves * 64) void k_ns_rows(NsArgs a) {
   namespace cg = cooperative_groups;
+  static_assert(kNrWaves * 64 == 16 * 32, "A panel staging: 32 threads per row");
   extern __shared__ __align__(16) double nr_smem[];
+  __shared__ double red[kNrWaves];
+  __shared__ double epart[kNrMaxWg];
+  __shared__ double bcast;
   cg::grid_group grid = cg::this_grid();
-  const int n = a.n, ld = nr_ld(n), nb = gridDim.x, nct = (n + 15) >> 4;
-  double* Tr = nr_smem;        // [16][ld]: rows R of T
-  double* Cr = Tr + 16 * ld;   // [16][ld]: rows R of C
-  const int r0 = 16 * blockIdx.x;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, l16 = lane & 15, g = lane >> 4;
-  // X0 = S/2 (ortho.C:236-239), T0 = I; LDS rows zero past n and past the k range
-  for (int e = threadIdx.x; e < 16 * ld; e += blockDim.x) {
-    const int r = e / ld, k = e - r * ld, row = r0 + r;
-    const bool in = row < n && k < n;
-    Tr[e] = (in && row == k) ? 1.0 : 0.0;
-    Cr[e] = 0.0;
-    if (in) {
-      a.X0[(size_t)row * n + k] = 0.5 * a.S[(size_t)row * n + k];
-      a.T0[(size_t)row * n + k] = (row == k) ? 1.0 : 0.0;
+  const int nb = gridDim.x, split = gridDim.y;
+  const int wg = blockIdx.y * nb + blockIdx.x, nwg = nb * split, h = blockIdx.y;
+  const int r0 = 16 * blockIdx.x, tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63, l16 = lane & 15, g = lane >> 4;
+  const int pr = tid >> 5, pk = tid & 31;  // A panel staging: row, first column
+  constexpr int nthr = kNrWaves * 64;
+  double bc[KB], bx[KB], ap[4];
+  {
+    const int n = a.n, ld = nr_ld(n);
+    double* Tr = nr_smem;  // [16][ld]: rows R of T (previous iteration, then own tiles new)
+    double* Cr = Tr + 16 * ld;  // [16][ld]: rows R of C
+    nr_load<KB>(bx, nr_rsrc(a.S, n), (h + split * (wave >> 1)) * 16 + l16, n, 0, 0.5);  // X0 = S/2
+    // iteration 1: T0 = I, C = 3I - S/2; LDS zero past n in both directions
+    for (int e = tid; e < 16 * ld; e += nthr) {
+      const int r = e / ld, k = e - r * ld, row = r0 + r;
+      const bool in = row < n && k < n;
+      const double sv = a.S[(size_t)min(row, n - 1) * n + min(k, n - 1)];
+      Tr[e] = (in && row == k) ? 1.0 : 0.0;
+      Cr[e] = in ? (row == k ? 3.0 : 0.0) - 0.5 * sv : 0.0;
     }
   }
-  grid.sync();
-  double* X[2] = {a.X0, a.X1};
-  double* T[2] = {a.T0, a.T1};
+  __syncthreads();
   int conv = 0;
   double err = 0.0;
   for (int it = 1; it <= a.max_iter; it++) {
-    const int o = (it - 1) & 1, w = it & 1;
-    // C[R,:] = 3I - T[R,:] X
-    for (int ct = wave; ct < nct; ct += kNrWaves) {
-      v4d acc = vzero(), unused = vzero();
-      nr_tile<false>(Tr, ld, X[o], nullptr, n, ct, acc, unused);
+    // n and what derives from it are recomputed in every iteration instead
+    // of being hoisted into registers that the operand batches need
+    int n = a.n;
+    asm volatile("" : "+s"(n));
+    const int ld = nr_ld(n), nct = (n + 15) >> 4, nown = (nct - h + split - 1) / split;
+    double* Tr = nr_smem;
+    double* Cr = Tr + 16 * ld;
+    // X_k and T_k live in buffer k & 1
+    const double* To = (it & 1) ? a.T0 : a.T1;
+    double* Xw = (it & 1) ? a.X1 : a.X0;
+    double* Tw = (it & 1) ? a.T1 : a.T0;
+    const rsrc_t rXo = nr_rsrc((it & 1) ? a.X0 : a.X1, n), rTo = nr_rsrc(To, n);
+    if (it > 1) {
+      if (split > 1) {  // rows R of T_{it-1}: the other tiles come from other workgroups
+        const bool rin = r0 + pr < n;
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int r = g + 4 * v, row = r0 + r, col = ct * 16 + l16;
-        if (col < n) Cr[r * ld + col] = (row < n ? (row == col ? 3.0 : 0.0) - acc[v] : 0.0);
+        for (int q = 0; q < 4; q++)
+          if (rin && pk + 32 * q < n) Tr[pr * ld + pk + 32 * q] = ap[q];
+        for (int k = pk + 128; k < n; k += 32)
+          if (rin) Tr[pr * ld + k] = To[(size_t)(r0 + pr) * n + k];
       }
+      __syncthreads();
+      // C[R,:] = 3I - T[R,:] X
+#pragma nounroll
+      for (int ct = wave, i = 0; ct < nct; ct += kNrWaves, i++) {
+        const int col = ct * 16 + l16;
+        if (i > 0) nr_load<KB>(bc, rXo, col, n, 0, 1.0);
+        const v4d acc = nr_product<KB>(Tr, ld, bc, rXo, col, n, 1.0);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int r = g + 4 * v, row = r0 + r;
+          if (col < n) Cr[r * ld + col] = (row < n ? (row == col ? 3.0 : 0.0) - acc[v] : 0.0);
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    // X'[R,:] = C[R,:] X / 2, T'[R,:] = C[R,:] T / 2, and the change of T
+    // own tiles: X'[R,ct] = C[R,:] X[:,ct] / 2, T'[R,ct] = C[R,:] T[:,ct] / 2,
+    // and the change of T
     double e = 0.0;
-    for (int ct = wave; ct < nct; ct += kNrWaves) {
-      v4d ax = vzero(), at = vzero();
-      nr_tile<true>(Cr, ld, X[o], T[o], n, ct, ax, at);
+#pragma nounroll
+    for (int q = wave, i = 0; q < 2 * nown; q += kNrWaves, i++) {
+      const int m = q & 1, col = (h + split * (q >> 1)) * 16 + l16;
+      v4d acc;
+      if (it == 1 && m) {  // T0 = I: C T0 = C
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc[v] = Cr[(g + 4 * v) * ld + min(col, ld - 1)];
+      } else {
+        const rsrc_t B = it == 1 ? nr_rsrc(a.S, n) : (m ? rTo : rXo);
+        const double scale = it == 1 ? 0.5 : 1.0;
+        if (i > 0) nr_load<KB>(bx, B, col, n, 0, scale);
+        acc = nr_product<KB>(Cr, ld, bx, B, col, n, scale);
+      }
+      double* D = m ? Tw : Xw;
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int r = g + 4 * v, row = r0 + r, col = ct * 16 + l16;
+        const int r = g + 4 * v, row = r0 + r;
         if (row < n && col < n) {
-          const double xn = 0.5 * ax[v], tn = 0.5 * at[v];
-          X[w][(size_t)row * n + col] = xn;
-          T[w][(size_t)row * n + col] = tn;
-          const double d = Tr[r * ld + col] - tn;
-          e += d * d;
-          Tr[r * ld + col] = tn;  // this thread's element only: read before written
+          const double nv = 0.5 * acc[v];
+          D[(unsigned)(row * n + col)] = nv;
+          if (m) {
+            const double d = Tr[r * ld + col] - nv;
+            e += d * d;
+            Tr[r * ld + col] = nv;  // this thread's element only: read before written
+          }
         }
       }
     }
     {  // workgroup sum in a fixed order
-      __shared__ double red[kNrWaves];
       for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
       if (lane == 0) red[wave] = e;
       __syncthreads();
-      if (threadIdx.x == 0) {
+      if (tid == 0) {
         double t = 0.0;
         for (int i = 0; i < kNrWaves; i++) t += red[i];
-        a.errpart[(size_t)it * nb + blockIdx.x] = t;
+        a.errpart[(size_t)it * nwg + wg] = t;
       }
     }
     grid.sync();
-    err = ns_err(a.errpart, it, nb, n);
+    // the next iteration's first operands, then the error partials
+    if (it < a.max_iter) {
+      if (split > 1) {
+        const double* p = Tw + (size_t)min(r0 + pr, n - 1) * n;
+#pragma unroll
+        for (int q = 0; q < 4; q++) ap[q] = p[min(pk + 32 * q, n - 1)];
+      }
+      const rsrc_t rXw = nr_rsrc(Xw, n);
+      nr_load<KB>(bc, rXw, wave * 16 + l16, n, 0, 1.0);
+      nr_load<KB>(bx, (wave & 1) ? nr_rsrc(Tw, n) : rXw, (h + split * (wave >> 1)) * 16 + l16, n,
+                  0, 1.0);
+    }
+    for (int i = tid; i < nwg; i += nthr) epart[i] = a.errpart[(size_t)it * nwg + i];
+    __syncthreads();
+    if (tid == 0) {
+      double sum = 0.0;
+      for (int i = 0; i < nwg; i++) sum += epart[i];
+      bcast = sqrt(sum / ((double)n * n));
+    }
+    __syncthreads();
+    err = bcast;
     if (err <= a.tol) {
       conv = it;
       break;
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (wg == 0 && tid == 0) {
     a.res->ns_iter = conv ? conv : -a.max_iter;
     a.res->ns_err = err;
   }
-  // T of the last iteration is this block's LDS rows
-  for (int e = threadIdx.x; e < 16 * n; e += blockDim.x) {
-    const int r = e / n, k = e - r * n, row = r0 + r;
-    if (row < n) a.Tout[(size_t)row * n + k] = Tr[r * ld + k];
+  // T of the last iteration: this workgroup's tiles are in its LDS rows
+  const int n = a.n, ld = nr_ld(n), nct = (n + 15) >> 4, nown = (nct - h + split - 1) / split;
+  for (int e = tid; e < 16 * nown * 16; e += nthr) {
+    const int r = e / (16 * nown), t = e - r * 16 * nown;
+    const int row = r0 + r, col = (h + split * (t >> 4)) * 16 + (t & 15);
+    if (row < n && col < n) a.Tout[(size_t)row * n + col] = nr_smem[r * ld + col];
+  }
+}
+
+static const void* nr_kernel(int n) {
+  switch (nr_kb(n)) {
+    case 8: return (const void*)k_ns_rows<8>;
+    default: return (const void*)k_ns_rows<16>;
   }
 }
 
@@ -1313,12 +1421,14 @@ static bool ns_smem_granted(int device) {
 }
 
 // Can all n x n tiles be resident at once for the cooperative launch?
-// The block-row variant: all ceil(n/16) workgroups resident, both LDS row
-// blocks within the device's limit.
-bool ns_rows_possible(int n, int device) {
+// The block-row variant: all ceil(n/16) x split workgroups resident, both
+// LDS row blocks within the device's limit, at least one column tile per
+// column group.
+bool ns_rows_possible(int n, int split, int device) {
   int per_cu = 0, ncu = 0, coop = 0, lds_max = 0;
-  if (n < 1 || hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device) !=
-                   hipSuccess ||
+  const int nb = (n + 15) / 16;
+  if (n < 1 || split < 1 || split > (n + 15) / 16 || nb * split > kNrMaxWg ||
+      hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device) != hipSuccess ||
       !coop)
     return false;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
@@ -1326,14 +1436,14 @@ bool ns_rows_possible(int n, int device) {
           hipSuccess)
     return false;
   const size_t smem = ns_rows_smem(n);
+  const void* fn = nr_kernel(n);
   if (smem > (size_t)lds_max ||
-      hipFuncSetAttribute((const void*)k_ns_rows, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)smem) != hipSuccess)
+      hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem) != hipSuccess)
     return false;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_ns_rows,
-                                                   kNrWaves * 64, smem) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kNrWaves * 64, smem) !=
+      hipSuccess)
     return false;
-  return (long)per_cu * ncu >= (long)((n + 15) / 16);
+  return (long)per_cu * ncu >= (long)nb * split;
 }
 
 bool ns_coop_possible(int n, int device) {
@@ -1353,10 +1463,10 @@ bool ns_coop_possible(int n, int device) {
 
 hipError_t ns_run(const NsArgs& a, hipStream_t st) {
   if (!a.coop) return ns_run_launches(a, st);
-  if (a.coop == 2) {
+  if (a.coop >= 2) {
     NsArgs args = a;
     void* params[] = {&args};
-    return hipLaunchCooperativeKernel((const void*)k_ns_rows, dim3((a.n + 15) / 16),
+    return hipLaunchCooperativeKernel(nr_kernel(a.n), dim3((a.n + 15) / 16, ns_rows_split(a.coop)),
                                       dim3(kNrWaves * 64), params, ns_rows_smem(a.n), st);
   }
   const int nt = (a.n + NT - 1) / NT;

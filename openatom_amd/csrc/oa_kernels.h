@@ -202,12 +202,15 @@ struct NsArgs {
   int n, max_iter;
   double tol;
   int coop;  // 0: two launches per iteration; 1: one cooperative launch over 32 x 32
-             // tiles (two grid barriers per iteration); 2: one cooperative launch over
-             // 16-row blocks (one barrier per iteration)
+             // tiles (two grid barriers per iteration); 2, 3, 4: one cooperative launch
+             // over 16-row blocks, each split over 1, 2 or 4 workgroups by column
+             // tiles (one barrier per iteration)
 };
 hipError_t ns_run(const NsArgs& a, hipStream_t st);
 bool ns_coop_possible(int n, int device);
-bool ns_rows_possible(int n, int device);
+bool ns_rows_possible(int n, int split, int device);
+inline int ns_rows_split(int coop) { return 1 << (coop - 2); }  // coop 2, 3, 4
+constexpr int kNsRowsMaxSplit = 4;
 
 // Conjugate-gradient state of a minimisation, on the device
 // (CP_State_GSpacePlane.C:120-136, 2646-2671)

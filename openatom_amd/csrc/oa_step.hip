@@ -634,14 +634,35 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
   TRY(dalloc(c, &c->nsC, nn * sizeof(double)));
   const int nt = (n + 31) / 32;
   // per-workgroup error partials of either cooperative kind
-  TRY(dalloc(c, &c->errpart, (size_t)(c->opt.ns_max_iter + 2) *
-                                 std::max(nt * nt, (n + 15) / 16) * sizeof(double)));
+  TRY(dalloc(c, &c->errpart,
+             (size_t)(c->opt.ns_max_iter + 2) *
+                 std::max(nt * nt, oa::kNsRowsMaxSplit * ((n + 15) / 16)) * sizeof(double)));
   c->ns_kinds.clear();
-  if (c->opt.ns_mode == 0) {
-    if (!c->dry && oa::ns_rows_possible(n, c->device)) c->ns_kinds.push_back(2);
+  const int nsm = c->opt.ns_mode;
+  if (nsm == 0) {
+    // row blocks split over 4, 2 or 1 workgroups (kinds 4, 3, 2)
+    for (int kind = 4; kind >= 2; kind--)
+      if (!c->dry && oa::ns_rows_possible(n, oa::ns_rows_split(kind), c->device))
+        c->ns_kinds.push_back(kind);
     if (oa::ns_coop_possible(n, c->device)) c->ns_kinds.push_back(1);
+    c->ns_kinds.push_back(0);
+  } else if (nsm == 1) {
+    c->ns_kinds.push_back(0);
+  } else if (nsm == 2) {
+    if (!c->dry && !oa::ns_coop_possible(n, c->device))
+      return fail(c, OA_EINVAL, "ns_mode 2: the 32 x 32 tiles of n do not fit on the device at once");
+    c->ns_kinds.push_back(1);
+  } else if (nsm >= 3 && nsm <= 5) {
+    const int kind = nsm - 1;
+    if (!c->dry && !oa::ns_rows_possible(n, oa::ns_rows_split(kind), c->device))
+      return fail(c, OA_EINVAL,
+                  "ns_mode " + std::to_string(nsm) +
+                      ": the row blocks of n (16 rows each, split over that many workgroups "
+                      "by 16-column tiles) do not fit on the device at once");
+    c->ns_kinds.push_back(kind);
+  } else {
+    return fail(c, OA_EINVAL, "ns_mode must be 0..5");
   }
-  c->ns_kinds.push_back(0);
   c->ns_kind = c->ns_kinds.front();
   c->sd_fused = false;  // measured slower than a separate SD pass (registers)
   // |F|^2 partials: one per workgroup of the SD kernel or of the rotate that
@@ -1166,7 +1187,7 @@ oa_status autotune(oa_ctx* c) {
     HIPCHK(hipMemcpy(c->S, two.data(), two.size() * sizeof(double), hipMemcpyHostToDevice));
     tune_knob(
         c, e0, e1,
-        "Newton-Schulz row blocks (2), cooperative tiles (1) or launches (0)", c->ns_kinds,
+        "Newton-Schulz row blocks split over 4, 2, 1 workgroups (4, 3, 2), cooperative tiles (1) or launches (0)", c->ns_kinds,
         c->ns_kind, [&](int v) { c->ns_kind = v; },
         [&]() { return st_ns(c, c->opt.ns_tol, c->opt.ns_max_iter); });
   }

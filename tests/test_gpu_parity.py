@@ -99,10 +99,15 @@ def test_r2g_force_eke(gpu, name, mode):
     assert abs(eke - eko) < TOL * abs(eko)
 
 
-@pytest.mark.parametrize("ns_mode", [0, 1])  # cooperative single launch, launches per iteration
-@pytest.mark.parametrize("name", ["small", "ragged37", "water8"])
+# Newton-Schulz forms: 0 autotuned, 1 launches per iteration, 2 cooperative
+# 32 x 32 tiles, 3/4/5 cooperative 16-row blocks split over 1/2/4 workgroups
+@pytest.mark.parametrize("ns_mode", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("name", ["small", "ragged37", "water8", "states150"])
 def test_lambda_sd_overlap_ns_rotate(gpu, name, ns_mode):
     s = get_system(name)
+    split = {4: 2, 5: 4}.get(ns_mode, 1)
+    if split > (s["nstates"] + 15) // 16:
+        pytest.skip(f"{split} column groups need at least {split} column tiles")
     o = make_oracle(s)
     dt = 5.0
     with make_ctx(s, dt=dt, ns_mode=ns_mode) as c:
