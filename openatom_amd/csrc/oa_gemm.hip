@@ -530,18 +530,13 @@ __global__ __launch_bounds__(256) void k_rotate(RotArgs a) {
   const int l16 = lane & 15, g = lane >> 4;
   const int kb = blockIdx.x * kRK;
   const int jb = blockIdx.y * 128 + wave * 32;
+  // operand columns clamped into range: columns past n or K are computed
+  // from duplicated data and never stored
   int jc[2], kc[4];
-  bool vj[2], vk[4];
 #pragma unroll
-  for (int x = 0; x < 2; x++) {
-    jc[x] = jb + 16 * x + l16;
-    vj[x] = jc[x] < a.n;
-  }
+  for (int x = 0; x < 2; x++) jc[x] = min(jb + 16 * x + l16, a.n - 1);
 #pragma unroll
-  for (int y = 0; y < 4; y++) {
-    kc[y] = kb + 16 * y + l16;
-    vk[y] = kc[y] < a.K;
-  }
+  for (int y = 0; y < 4; y++) kc[y] = min(kb + 16 * y + l16, a.K - 1);
   v4d acc[2][4];
 #pragma unroll
   for (int x = 0; x < 2; x++)
@@ -549,16 +544,20 @@ __global__ __launch_bounds__(256) void k_rotate(RotArgs a) {
     for (int y = 0; y < 4; y++) acc[x][y] = vzero();
   if (jb < a.n) {
     double am0[4][2], bx0[4][4], am1[4][2], bx1[4][4];
+    // branch-free: rows past ni are clamped and their M values zeroed (a
+    // load that only some lanes issue is waited for on its own)
     auto load = [&](int i0, double (&am)[4][2], double (&bx)[4][4]) {
 #pragma unroll
       for (int s = 0; s < 4; s++) {
-        const int i = i0 + 4 * s + g;
+        const int i = i0 + 4 * s + g, ic = min(i, a.ni - 1);
         const bool vi = i < a.ni;
 #pragma unroll
-        for (int x = 0; x < 2; x++) am[s][x] = (vi && vj[x]) ? a.M[(size_t)i * a.n + jc[x]] : 0.0;
+        for (int x = 0; x < 2; x++) {
+          const double m = a.M[(size_t)ic * a.n + jc[x]];
+          am[s][x] = vi ? m : 0.0;
+        }
 #pragma unroll
-        for (int y = 0; y < 4; y++)
-          bx[s][y] = (vi && vk[y]) ? a.X[(size_t)i * a.ldx + kc[y]] : 0.0;
+        for (int y = 0; y < 4; y++) bx[s][y] = a.X[(size_t)ic * a.ldx + kc[y]];
       }
     };
     auto mma = [&](const double (&am)[4][2], const double (&bx)[4][4]) {
@@ -592,7 +591,7 @@ __global__ __launch_bounds__(256) void k_rotate(RotArgs a) {
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const int j = jb + 16 * x + g + 4 * v;
-        const int k = kc[y];
+        const int k = kb + 16 * y + l16;  // unclamped: each output once
         if (j < a.n && k < a.K) {
           const double s = (k < a.k_lo) ? a.s_lo : a.s_hi;
           double* yp = a.Y + (size_t)j * a.ldx + k;
