@@ -524,7 +524,7 @@ hipError_t gram_reduce(const double* part, int nslice, int n, int nb, int sym, d
 // loaded while the current chunk's 32 MFMAs issue.
 constexpr int kRK = 64;
 
-__global__ __launch_bounds__(256) void k_rotate(RotArgs a) {
+__global__ __launch_bounds__(256, 2) void k_rotate(RotArgs a) {
   __shared__ double red[4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int l16 = lane & 15, g = lane >> 4;
