@@ -86,36 +86,46 @@ template <typename T, int R1, int R2>
 __global__ __launch_bounds__(kThreads) void k2_zbwd(FftCtx c) {
   using Sh = Shape<R1, R2>;
   constexpr int N = Sh::N, G = Sh::G, NTR = Sh::NTR;
+  constexpr int kPre = 4;  // passes over the points whose operands are loaded up front
   __shared__ cpx<T> ex[NTR * N];
+  __shared__ unsigned char wt_l[NTR * N];  // per point: 1 on kx = 0 lines, 2 elsewhere
   __shared__ double red[kThreads / 64];
   const int s = blockIdx.y, l0 = blockIdx.x * NTR;
   const int nl = min(NTR, c.g.nlines - l0);
   const int tid = threadIdx.x;
   const size_t plane = (size_t)c.g.nplane_x * c.g.ny;
-  // pass 2's operands (twiddles, the line, kinetic factors and old psi),
-  // loaded up front so that their latency hides behind pass 1
+  // The workgroup's lines are one contiguous run of stored points [p0, p1):
+  // the pack stage reads psi and |G|^2 and writes F along it, lanes on
+  // consecutive points.  Lanes along the z of each line would store runs
+  // of R1 points at arbitrary alignment, partial sectors of HBM that cost
+  // more traffic than the data.
+  const int p0 = c.lines[l0].x;
+  const int4 Llast = c.lines[l0 + nl - 1];
+  const int np = Llast.x + Llast.y - p0;  // <= nl * N
+  const double2* ps = c.psi + (size_t)s * c.g.ld + p0;
+  double2* fs = c.force + (size_t)s * c.g.ld + p0;
+  const double* g2 = c.g2 + p0;
+  // pass 2's twiddles and line, and the first pack operands, loaded up
+  // front (branch-free, clamped) so that their latency hides behind pass 1
   const int tr2 = tid / G, j2 = tid - tr2 * G;
   cpx<T> tw2[R2];
 #pragma unroll
   for (int t = 1; t < R2; t++) tw2[t] = twiddle<T, +1>(c.twz, t * min(j2, R1 - 1));
   const int4 L = c.lines[l0 + min(tr2, nl - 1)];
+  const int w1 = c.lines[l0 + min(tid % NTR, nl - 1)].w;
   const double oc = c.occ[s];
-  double gq[R2];
-  double2 cq[R2];
-  {
-    const double2* ps = c.psi + (size_t)s * c.g.ld + L.x;
-    const double* g2 = c.g2 + L.x;
+  double gq[kPre];
+  double2 cq[kPre];
 #pragma unroll
-    for (int t = 0; t < R2; t++) {  // branch-free, clamped to the line
-      const int dc = min(max(signed_k(j2 + t * R1, N) - L.z, 0), L.y - 1);
-      gq[t] = g2[dc];
-      cq[t] = ps[dc];
-    }
+  for (int i = 0; i < kPre; i++) {
+    const int o = min(tid + i * kThreads, np - 1);
+    gq[i] = g2[o];
+    cq[i] = ps[o];
   }
   {  // pass 1: lanes along lines
     const int tr = tid % NTR, j = tid / NTR;
     if (tr < nl && j < R2) {
-      const cpx<T>* zs = (const cpx<T>*)c.zbuf + (size_t)s * c.g.nz * plane + c.lines[l0 + tr].w;
+      const cpx<T>* zs = (const cpx<T>*)c.zbuf + (size_t)s * c.g.nz * plane + w1;
       cpx<T> v[R1];
 #pragma unroll
       for (int t = 0; t < R1; t++) v[t] = zs[(size_t)(j + t * R2) * plane];
@@ -125,39 +135,55 @@ __global__ __launch_bounds__(kThreads) void k2_zbwd(FftCtx c) {
     }
   }
   __syncthreads();
-  double eke = 0.0;
-  {  // pass 2: lanes along z, pack to the sphere
+  {  // pass 2: lanes along z; the results go back to LDS in stored-point order
     const int tr = tr2, j = j2;
-    if (tr < nl && j < R1) {
-      cpx<T> u[R2];
+    const bool act = tr < nl && j < R1;
+    cpx<T> u[R2];
+    if (act) {
 #pragma unroll
       for (int t = 0; t < R2; t++) u[t] = ex[tr * N + j + t * R1];
 #pragma unroll
       for (int t = 1; t < R2; t++) u[t] = cm(u[t], tw2[t]);
       Dft<T, R2, +1>::run(u);
-      const double w = (L.w < c.g.ny) ? 1.0 : 2.0;  // kx = 0 lines have dst < ny
-      double2* fs = c.force + (size_t)s * c.g.ld + L.x;
+    }
+    __syncthreads();  // ex is rewritten below
+    if (act) {
+      const int ob = L.x - p0;
+      const unsigned char w = (L.w < c.g.ny) ? 1 : 2;  // kx = 0 lines have dst < ny
 #pragma unroll
       for (int t = 0; t < R2; t++) {
         const int d = signed_k(j + t * R1, N) - L.z;
         if (d >= 0 && d < L.y) {
-          const double gg = gq[t];
-          double2 f;
-          if (gg <= c.ecut_ry) {
-            const double2 cc = cq[t];
-            const double wt = oc * w * gg;
-            f.x = w * (c.scale * (double)u[t].x) - wt * cc.x;
-            f.y = w * (c.scale * (double)u[t].y) - wt * cc.y;
-            eke += wt * (cc.x * cc.x + cc.y * cc.y);
-          } else {
-            f.x = 0.0;
-            f.y = 0.0;
-          }
-          fs[d] = f;
+          ex[ob + d] = u[t];
+          wt_l[ob + d] = w;
         }
       }
     }
   }
+  __syncthreads();
+  // pack: -2/N scale (w doubles the kx > 0 half), kinetic force, EKE
+  double eke = 0.0;
+  auto pack = [&](int o, double gg, double2 cc) __attribute__((always_inline)) {
+    const cpx<T> u = ex[o];
+    const double w = (double)wt_l[o];
+    double2 f;
+    if (gg <= c.ecut_ry) {
+      const double wt = oc * w * gg;
+      f.x = w * (c.scale * (double)u.x) - wt * cc.x;
+      f.y = w * (c.scale * (double)u.y) - wt * cc.y;
+      eke += wt * (cc.x * cc.x + cc.y * cc.y);
+    } else {
+      f.x = 0.0;
+      f.y = 0.0;
+    }
+    fs[o] = f;
+  };
+#pragma unroll
+  for (int i = 0; i < kPre; i++) {
+    const int o = tid + i * kThreads;
+    if (o < np) pack(o, gq[i], cq[i]);
+  }
+  for (int o = tid + kPre * kThreads; o < np; o += kThreads) pack(o, g2[o], ps[o]);
   for (int o = 32; o > 0; o >>= 1) eke += __shfl_down(eke, o, 64);
   if ((tid & 63) == 0) red[tid >> 6] = eke;
   __syncthreads();
