@@ -431,13 +431,15 @@ __global__ __launch_bounds__(1024) void k_gram_reduce(const double* __restrict__
       j = t;
     }
     const double* p = part + (size_t)i * nb + j;
-    // thread y sums slices y, y+16, y+32, ...: up to 16 loads in flight
+    // thread y sums slices y, y+16, y+32, ...: 16 loads in flight,
+    // branch-free (slice index clamped, value masked)
     for (int base = 0; base < nslice; base += 256) {
       double v[16];
 #pragma unroll
       for (int u = 0; u < 16; u++) {
         const int sp = base + y + 16 * u;
-        v[u] = sp < nslice ? p[(size_t)sp * nn] : 0.0;
+        const double w = p[(size_t)min(sp, nslice - 1) * nn];
+        v[u] = sp < nslice ? w : 0.0;
       }
 #pragma unroll
       for (int u = 0; u < 16; u += 2) {
@@ -456,46 +458,49 @@ __global__ __launch_bounds__(1024) void k_gram_reduce(const double* __restrict__
   }
 }
 
-// Symmetric product: one workgroup per upper 16 x 16 tile (the tiles the
-// gram wrote), read along rows, written to C and, off the diagonal, to its
-// mirror.  Thread
-// (y, r, c): tile element (r, c), slices y, y + 4, ...; the four slice
-// groups are summed through LDS in a fixed order.
+// Symmetric product: four workgroups per upper 16 x 16 tile (the tiles the
+// gram wrote), each a quarter of the tile's rows; written to C and, off the
+// diagonal, to its mirror.  Thread (y, x): element x of the 4 x 16 quarter,
+// slices y, y + 16, ...; loads branch-free (slice index clamped, value
+// masked), 16 in flight; the 16 slice groups are summed through LDS in a
+// fixed order.
 __global__ __launch_bounds__(1024) void k_gram_reduce_sym(const double* __restrict__ part,
                                                           int nslice, int n, double alpha,
                                                           double* __restrict__ C) {
-  __shared__ double red[4][256];
+  __shared__ double red[16][64];
   const int nt = (n + 15) >> 4;
-  int b = blockIdx.x, ti = 0;
+  int b = blockIdx.x >> 2, ti = 0;
   while (b >= nt - ti) {
     b -= nt - ti;
     ti++;
   }
   const int tj = ti + b;
-  const int y = threadIdx.x >> 8, t = threadIdx.x & 255, r = t >> 4, c = t & 15;
-  const int i = ti * 16 + r, j = tj * 16 + c;
+  const int x = threadIdx.x & 63, y = threadIdx.x >> 6;
+  const int i = ti * 16 + (blockIdx.x & 3) * 4 + (x >> 4), j = tj * 16 + (x & 15);
   const size_t nn = (size_t)n * n;
+  const double* p = part + (size_t)min(i, n - 1) * n + min(j, n - 1);
   double s0 = 0.0, s1 = 0.0;
-  if (i < n && j < n) {
-    const double* p = part + (size_t)i * n + j;
-    for (int base = 0; base < nslice; base += 32) {
-      double v[8];
+  for (int base = 0; base < nslice; base += 256) {
+    double v[16];
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const int sp = base + y + 4 * u;
-        v[u] = sp < nslice ? p[(size_t)sp * nn] : 0.0;
-      }
+    for (int u = 0; u < 16; u++) {
+      const int sp = base + y + 16 * u;
+      const double w = p[(size_t)min(sp, nslice - 1) * nn];
+      v[u] = sp < nslice ? w : 0.0;
+    }
 #pragma unroll
-      for (int u = 0; u < 8; u += 2) {
-        s0 += v[u];
-        s1 += v[u + 1];
-      }
+    for (int u = 0; u < 16; u += 2) {
+      s0 += v[u];
+      s1 += v[u + 1];
     }
   }
-  red[y][t] = s0 + s1;
+  red[y][x] = s0 + s1;
   __syncthreads();
   if (y == 0 && i < n && j < n) {
-    const double v = alpha * ((red[0][t] + red[1][t]) + (red[2][t] + red[3][t]));
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < 16; w++) t += red[w][x];
+    const double v = alpha * t;
     C[(size_t)i * n + j] = v;
     if (ti < tj) C[(size_t)j * n + i] = v;  // a diagonal tile holds both halves
   }
@@ -506,8 +511,8 @@ hipError_t gram_reduce(const double* part, int nslice, int n, int nb, int sym, d
   if (sym && nb != n) return hipErrorInvalidValue;
   if (sym) {
     const int nt = (n + 15) >> 4;
-    hipLaunchKernelGGL(k_gram_reduce_sym, dim3(nt * (nt + 1) / 2), dim3(1024), 0, st, part,
-                       nslice, n, alpha, C);
+    hipLaunchKernelGGL(k_gram_reduce_sym, dim3(4 * (nt * (nt + 1) / 2)), dim3(1024), 0, st,
+                       part, nslice, n, alpha, C);
     return hipGetLastError();
   }
   const size_t nn = (size_t)n * nb;
