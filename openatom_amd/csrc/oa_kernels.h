@@ -226,9 +226,12 @@ hipError_t cg_gamma(const StepDev* res, CgDev* cg, hipStream_t st);
 hipError_t cg_update(double2* psi, double2* cgdir, const double2* f, const double* dtm,
                      const double* g2, double ecut, int nst, int npts, int ld, const CgDev* cg,
                      hipStream_t st);
-// psi += dtm[p] * F (cp_min_std.C:56-86); accumulates |F|^2 partials
-hipError_t sd_update(double2* psi, const double2* f, const double* dtm, int nst, int npts,
-                     int ld, double* fpart, int nfpart, hipStream_t st);
+// psi += dtm[p] * F (cp_min_std.C:56-86); accumulates |F|^2 partials.
+// twin[p] (may be null): -1, or -2 for a redundant kx=0 point (not
+// written), or the redundant twin of p, which receives conj(new psi[p])
+// (the redundancy sync, CP_State_GSpacePlane.C:2862-3061)
+hipError_t sd_update(double2* psi, const double2* f, const double* dtm, const int* twin, int nst,
+                     int npts, int ld, double* fpart, int nfpart, hipStream_t st);
 // redundant kx=0 points <- conj(partner) (CP_State_GSpacePlane.C:2862-3061)
 hipError_t red_sync(double2* psi, const int* red, const int* partner, int nred, int nst, int ld,
                     hipStream_t st);

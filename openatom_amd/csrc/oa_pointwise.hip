@@ -27,8 +27,8 @@ __device__ __forceinline__ double block_sum(double v) {
 }
 
 __global__ void k_sd(double2* __restrict__ psi, const double2* __restrict__ f,
-                     const double* __restrict__ dtm, int nst, int npts, int ld,
-                     double* __restrict__ fpart) {
+                     const double* __restrict__ dtm, const int* __restrict__ twin, int nst,
+                     int npts, int ld, double* __restrict__ fpart) {
   const size_t tot = (size_t)nst * npts;
   double fn = 0.0;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
@@ -37,19 +37,22 @@ __global__ void k_sd(double2* __restrict__ psi, const double2* __restrict__ f,
     const size_t o = (size_t)s * ld + p;
     const double2 fv = f[o];
     const double m = dtm[p];
+    const int tw = twin ? twin[p] : -1;
     double2 c = psi[o];
     c.x += fv.x * m;
     c.y += fv.y * m;
-    psi[o] = c;
+    if (tw != -2) psi[o] = c;  // a redundant point is written by its partner
+    if (tw >= 0) psi[(size_t)s * ld + tw] = make_double2(c.x, -c.y);
     fn += fv.x * fv.x + fv.y * fv.y;
   }
   const double t = block_sum(fn);
   if (threadIdx.x == 0) fpart[blockIdx.x] = t;
 }
 
-hipError_t sd_update(double2* psi, const double2* f, const double* dtm, int nst, int npts, int ld,
-                     double* fpart, int nfpart, hipStream_t st) {
-  hipLaunchKernelGGL(k_sd, dim3(nfpart), dim3(256), 0, st, psi, f, dtm, nst, npts, ld, fpart);
+hipError_t sd_update(double2* psi, const double2* f, const double* dtm, const int* twin, int nst,
+                     int npts, int ld, double* fpart, int nfpart, hipStream_t st) {
+  hipLaunchKernelGGL(k_sd, dim3(nfpart), dim3(256), 0, st, psi, f, dtm, twin, nst, npts, ld,
+                     fpart);
   return hipGetLastError();
 }
 

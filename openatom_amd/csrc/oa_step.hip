@@ -143,7 +143,8 @@ struct oa_ctx {
   std::vector<void*> allocs;
   double2 *psi = nullptr, *psi2 = nullptr, *force = nullptr;
   int *blk_pt0 = nullptr, *pt_code = nullptr, *line_dst = nullptr, *kylo = nullptr,
-      *kyhi = nullptr, *red = nullptr, *partner = nullptr;
+      *kyhi = nullptr, *red = nullptr, *partner = nullptr,
+      *sd_twin = nullptr;  // per point: -1, -2 (redundant kx=0 point) or its redundant twin
   double2 *twx = nullptr, *twy = nullptr, *twz = nullptr;
   double *g2 = nullptr, *occ = nullptr, *dtm = nullptr, *vks = nullptr, *rho = nullptr;
   void *zbuf = nullptr, *xbuf = nullptr, *psir = nullptr, *zout = nullptr;
@@ -486,6 +487,7 @@ oa_status build_layout(oa_ctx* c) {
   TRY(dalloc(c, &c->kyhi, sizeof(int) * c->nplane_x));
   TRY(dalloc(c, &c->red, sizeof(int) * std::max(1, c->nred)));
   TRY(dalloc(c, &c->partner, sizeof(int) * std::max(1, c->nred)));
+  TRY(dalloc(c, &c->sd_twin, sizeof(int) * n));
   TRY(dalloc(c, &c->g2, sizeof(double) * n));
   TRY(dalloc(c, &c->dtm, sizeof(double) * n));
   TRY(upload(c, c->blk_pt0, blk_pt0.data(), sizeof(int) * blk_pt0.size()));
@@ -498,6 +500,14 @@ oa_status build_layout(oa_ctx* c) {
   if (c->nred) {
     TRY(upload(c, c->red, red.data(), sizeof(int) * c->nred));
     TRY(upload(c, c->partner, partner.data(), sizeof(int) * c->nred));
+  }
+  {  // the SD update writes each redundant point with its partner
+    std::vector<int> twin(n, -1);
+    for (int r = 0; r < c->nred; r++) {
+      twin[partner[r]] = red[r];
+      twin[red[r]] = -2;
+    }
+    TRY(upload(c, c->sd_twin, twin.data(), sizeof(int) * n));
   }
   TRY(upload(c, c->g2, g2.data(), sizeof(double) * n));
   TRY(upload(c, c->dtm, dtm.data(), sizeof(double) * n));
@@ -988,11 +998,11 @@ oa_status st_nonlocal(oa_ctx* c) {
   return OA_OK;
 }
 
+// SD update with the kx=0 redundancy sync folded in (sd_twin)
 oa_status st_sd(oa_ctx* c) {
-  HIPCHK(oa::sd_update(c->psi, c->force, c->dtm, c->nst, c->npts, c->ld, c->fpart, c->nfpart,
-                       c->stream));
+  HIPCHK(oa::sd_update(c->psi, c->force, c->dtm, c->sd_twin, c->nst, c->npts, c->ld, c->fpart,
+                       c->nfpart, c->stream));
   HIPCHK(oa::reduce_to(c->fpart, c->nfpart, 1.0, &c->res->fnorm2, c->stream));
-  HIPCHK(oa::red_sync(c->psi, c->red, c->partner, c->nred, c->nst, c->ld, c->stream));
   return OA_OK;
 }
 
