@@ -132,6 +132,45 @@ def run_precision(sysd, precision, steps, warmup, device=0, use_graph=0, vks=Tru
                 tuning=[t for t in tuning.split("\n") if t])
 
 
+def quality_trace(sysd, nsteps, device=0):
+    """Mixed precision against the GPU FP64 path from the same start, in
+    lockstep, every step: dE per atom, max|d rho| (absolute and relative to
+    max rho) and ||Psi^H Psi/2 - I|| of the mixed coefficients (FP64)."""
+    import numpy as np
+    from openatom_amd.step import CPContext
+    ctx = {}
+    for prec in ("mixed", "fp64"):
+        c = CPContext(sysd["grid"], sysd["hmat"], sysd["ecut_ry"], sysd["nstates"], sysd["kx"],
+                      sysd["ky"], sysd["kz"], precision=prec, device=device, dt=5.0,
+                      ns_tol=1e-15 if prec == "fp64" else 1e-14, ns_max_iter=30, want_resid=0,
+                      vks_from_rho=1, autotune=1)
+        t = sysd["pseudo"]
+        c.set_local_pseudo(sysd["pos"], sysd["species"], t["gmin"], t["dg"], t["vps"],
+                           t["vps_g0"])
+        c.set_psi(sysd["psi"])
+        ctx[prec] = c
+    natoms = sysd["natoms"]
+    tr = dict(dE_per_atom=[], max_drho=[], drho_rel=[], resid_max=[])
+    first_mixed = None
+    for k in range(nsteps):
+        gm = ctx["mixed"].step()
+        gd = ctx["fp64"].step()
+        if first_mixed is None:
+            first_mixed = gm
+        rm, rd = ctx["mixed"].get_density(), ctx["fp64"].get_density()
+        d = float(np.abs(rm - rd).max())
+        tr["dE_per_atom"].append(abs(gm["eelec"] - gd["eelec"]) / natoms)
+        tr["max_drho"].append(d)
+        tr["drho_rel"].append(d / float(np.abs(rd).max()))
+        tr["resid_max"].append(ctx["mixed"].residual()[0])
+    for c in ctx.values():
+        c.close()
+    worst = {k: max(v) for k, v in tr.items()}
+    return dict(steps=nsteps, worst=worst, last={k: v[-1] for k, v in tr.items()},
+                meets_bars=bool(worst["dE_per_atom"] <= 1e-6 and worst["resid_max"] <= 1e-10),
+                first_step_mixed=first_mixed)
+
+
 def roofline(res, nst, traffic=None):
     """Dominant kernel (from the per-stage breakdown) and its roofline, with
     its duration measured over the timed steps when they recorded it."""
@@ -167,10 +206,8 @@ def roofline(res, nst, traffic=None):
 
 def _serving_system(args):
     from openatom_amd import system
-    import oracle
     nmol, ecut, seed = args
-    return system.build(system.serving_workload(nmol, ecut), seed=seed,
-                        kvec_source=oracle.stored_kvecs)
+    return system.build(system.serving_workload(nmol, ecut), seed=seed)
 
 
 def serving_mix(nreq, rate, seed, max_mol, workers=8):
@@ -230,7 +267,7 @@ def serving_run(systems, arrive, precision, f_tol, max_steps, budget, max_active
         "peak_hbm_gib": round(st["peak_hbm_bytes"] / 2**30, 3)})
 
 
-def cpu_baseline(sysd, nsteps):
+def cpu_baseline(sysd, nsteps, warmup=3):
     """The C oracle's FP64 steps (V_KS rebuilt from rho each step, as on
     the GPU) on the host cores; V_loc(G) is set up before the clock."""
     import numpy as np
@@ -245,16 +282,19 @@ def cpu_baseline(sysd, nsteps):
                                    t["vps_g0"])
     vloc = o.vloc_g(lp)
     psi = np.array(sysd["psi"], copy=True)
-    t0 = time.perf_counter()
-    outs = []
-    for _ in range(nsteps):
+    outs, times = [], []
+    for k in range(warmup + nsteps):
+        t0 = time.perf_counter()
         outs.append(o.step(None, psi, dt=5.0, ns_tol=1e-15, ns_max_iter=30, vloc=vloc))
-    dt = time.perf_counter() - t0
-    return dict(value=nsteps / dt, unit="steps/s", cores=nthr, kind="port",
-                sample=f"{nsteps} full FP64 CP steps (V_KS from rho included) of the same "
-                       f"workload on the C oracle (OpenMP, {nthr} threads of {ncores} "
-                       f"visible cores)",
-                seconds=dt, eelec=[x["eelec"] for x in outs], psi=psi)
+        times.append(time.perf_counter() - t0)
+    tt = np.array(times[warmup:])
+    return dict(value=nsteps / tt.sum(), unit="steps/s", cores=nthr, kind="port",
+                sample=f"{warmup} warm-up + {nsteps} timed full FP64 CP steps (V_KS from rho "
+                       f"included) of the same workload on the C oracle (OpenMP, {nthr} threads "
+                       f"of {ncores} visible cores); value = timed steps / their total time",
+                seconds=float(tt.sum()), median_s=round(float(np.median(tt)), 4),
+                p99_s=round(float(np.percentile(tt, 99)), 4),
+                eelec=[x["eelec"] for x in outs], psi=psi)
 
 
 def main():
@@ -265,6 +305,9 @@ def main():
     ap.add_argument("--workload", default="water32")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--cpu-warmup", type=int, default=3)
+    ap.add_argument("--quality-steps", type=int, default=100,
+                    help="steps of the lockstep mixed-vs-FP64 error trace (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="eager launches only")
     ap.add_argument("--no-autotune", action="store_true",
@@ -290,7 +333,6 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import oracle
     from openatom_amd import system
 
     mix = None
@@ -301,7 +343,7 @@ def main():
     if args.copy_calib_mb > 0:
         copy_calibration(args.copy_calib_mb)
     log(f"[bench] building workload {args.workload} (seed {args.seed})")
-    sysd = system.build(args.workload, seed=args.seed, kvec_source=oracle.stored_kvecs)
+    sysd = system.build(args.workload, seed=args.seed)
     nst = sysd["nstates"]
     natoms = sysd["natoms"]
 
@@ -342,6 +384,11 @@ def main():
                                  r["timings"].get("nonlocal+reduce_lambda", 0.0), 4),
                              "lambda_only_ms": round(
                                  runs[(prec, 0)]["timings"].get("reduce_lambda", 0.0), 4)}
+    qt = None
+    if vks and args.quality_steps > 0:
+        log(f"[bench] quality trace: mixed vs FP64 in lockstep over {args.quality_steps} steps")
+        qt = quality_trace(sysd, args.quality_steps)
+        log(f"[bench] quality worst: {qt['worst']}")
     r64, rmx = runs[("fp64", 0)], runs[("mixed", 0)]
     r64v, rmxv = r64, rmx
     graph = {p: round(runs[(p, 1)]["steps_per_s"], 3) for p in ("fp64", "mixed")
@@ -389,12 +436,16 @@ def main():
     cpu = None
     if not args.no_cpu:
         log(f"[bench] cpu baseline: {args.cpu_steps} oracle steps")
-        cb = cpu_baseline(sysd, args.cpu_steps)
+        cb = cpu_baseline(sysd, args.cpu_steps, args.cpu_warmup)
         # the GPU FP64 path after the same number of steps from the same start
         cpu = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
         cpu["seconds"] = round(cb["seconds"], 3)
         cpu["gpu_fp64_first_step_dE_rel_vs_cpu"] = abs(
             r64["first"]["eelec"] - cb["eelec"][0]) / abs(cb["eelec"][0])
+        cpu["gpu_mixed_first_step_dE_per_atom_vs_cpu"] = abs(
+            rmx["first"]["eelec"] - cb["eelec"][0]) / natoms
+        cpu["median_step_s"] = cb["median_s"]
+        cpu["p99_step_s"] = cb["p99_s"]
 
     out = {
         "metric": METRIC,
@@ -433,7 +484,9 @@ def main():
                                            ("eelec", "eke", "eloc", "ehart", "exc")},
                     "resid_max_mixed": rmx["last"]["resid_max"],
                     "ns_iters_mixed": rmx["last"]["ns_iters"],
-                    "steps_compared": args.warmup + args.steps + PROF_STEPS},
+                    "steps_compared": args.warmup + args.steps + PROF_STEPS,
+                    "per_step_trace": None if qt is None else
+                    {k: qt[k] for k in ("steps", "worst", "last", "meets_bars")}},
         "timings_ms": rmx["timings"],
         "timed_steps_events_ms": rmx["timings_timed"],
         "timings_note": ("timed steps record HIP events around the step and the R->G plane pass "

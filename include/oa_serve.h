@@ -34,9 +34,11 @@ typedef struct oa_server oa_server;
 
 typedef struct {
   oa_system sys;        /* the request's cell, grid, cutoff, states (k-vectors copied) */
-  oa_options opt;       /* step options (precision, dt, use_cg, ...) */
+  oa_options opt;       /* step options (precision, dt, use_cg, ...); vks_from_rho is
+                           set from vks / pseudo below */
   const double *psi0;   /* [nstates][npts] complex starting coefficients (copied) */
-  const double *vks;    /* [Nz][Ny][Nx] V(r) (copied) */
+  const double *vks;    /* [Nz][Ny][Nx] fixed V(r) (copied), or NULL for a
+                           self-consistent request (pseudo) */
   double f_tol;         /* converged when |F| <= f_tol */
   int max_steps;        /* OA_ENOCONV after this many steps */
   double *psi_out;      /* optional [nstates][npts] final coefficients; must stay valid
@@ -46,6 +48,9 @@ typedef struct {
                            coefficients with a fresh CG direction, once |F| <= fp64_below or
                            |F| has not improved for 50 steps (the FP32 FFT chain sets a
                            floor on |F|); 0 disables the switch */
+  const oa_local_pseudo *pseudo; /* with vks NULL: the fixed ions (copied); every step
+                           rebuilds V = V_xc + V_H + V_loc from its own rho, as the
+                           density chares do (CP_Rho_RealSpacePlane.C:739-864) */
 } oa_request;
 
 typedef struct {

@@ -98,6 +98,15 @@ typedef struct {
                           the S and Lambda products, Newton-Schulz as one cooperative launch
                           or two launches per iteration.  Chunking changes the order of the
                           rho sums (rounding only).  0 (default): fixed heuristics */
+  int gemm_split;      /* mixed precision only: products run as split-BF16 matrix-core
+                          products (each FP64 operand split into three BF16 terms, the six
+                          term products with i + j <= 2 summed in FP32; K-slice partials
+                          summed in FP64), bit 0 (default on): Lambda = Re Psi^H F and
+                          F -= w Psi Lambda; bit 1: S = Psi^H Psi; bit 2: Psi T, in the
+                          update form Psi + Psi (T - I).  Bits 0 and 2 apply the split to the
+                          projection and the rotation up to 128 states (FP64 above).  The
+                          ||Psi^H Psi/2 - I|| residual, Newton-Schulz and the energies stay
+                          FP64.  Ignored in FP64 contexts. */
 } oa_options;
 
 typedef struct {
@@ -157,7 +166,8 @@ typedef struct {
 } oa_nonlocal;
 
 /* default options: FP64, ns_tol 1e-15, ns_max_iter 30, dt 5, tau 25 fs,
- * ecut_mass 2, Lambda on, residual on */
+ * ecut_mass 2, Lambda on, residual on, gemm_split 1 (Lambda split-BF16 when
+ * the context is mixed precision) */
 void oa_default_options(oa_options *opt);
 
 /* Stored k-vectors of a cell and cutoff in the reference order:
@@ -236,6 +246,18 @@ oa_status oa_overlap(oa_ctx *ctx, double *S);
  * (ortho.C:109-146, 862-985) and CPORTHOG::CP_orthoTransform
  * (cp_lowdin.C:24-46, 180-265).  OA_ENOCONV if max_iter is reached. */
 oa_status oa_ortho_ns(oa_ctx *ctx, double tol, int max_iter, int *iters, double *err, double *T);
+/* T = (S/2)^{-1/2} of a caller's S, no context: the static
+ * CPORTHOG::CP_orthoTransform(double *T, const double *S, int nstate)
+ * (class_cporthog.h:24, cp_lowdin.C:24-46).  S and T are host arrays, n x n,
+ * row-major, S in the reference's scaling (about 2I for orthonormal states).
+ * Runs get_iter_Tmat's Newton-Schulz iteration (cp_lowdin.C:180-265: tol
+ * 1e-15 on the rms change of T, at most 10 iterations) on the current HIP
+ * device, for every n (the reference diagonalises instead when n <= 64,
+ * get_diag_Tmat, whose T is the same matrix to the iteration's tolerance).
+ * OA_ENOCONV where the reference EXITs (cp_lowdin.C:247-253); T then holds
+ * the last iterate.  Allocates its device work space per call. */
+oa_status oa_ortho_transform(double *T, const double *S, int n);
+
 /* load a T for oa_rotate (the PairCalculator's orthoT input) */
 oa_status oa_tmat_set(oa_ctx *ctx, const double *T);
 

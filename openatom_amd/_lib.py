@@ -23,7 +23,7 @@ EXPORTS = (
     "oa_psi_get", "oa_vks_set",
     "oa_fft_g2r", "oa_psir_get", "oa_density_get", "oa_vpsi_r2g", "oa_force_get",
     "oa_force_set", "oa_lambda", "oa_sd_update", "oa_overlap", "oa_ortho_ns",
-    "oa_tmat_set", "oa_rotate", "oa_step", "oa_residual", "oa_sync", "oa_step_result", "oa_timings",
+    "oa_tmat_set", "oa_ortho_transform", "oa_rotate", "oa_step", "oa_residual", "oa_sync", "oa_step_result", "oa_timings",
     "oa_set_timing", "oa_local_pseudo_set", "oa_vks_build", "oa_vks_get", "oa_tuning_log",
     "oa_nonlocal_set", "oa_nonlocal_get", "oa_nonlocal_force",
     # state files (oa_state.h)
@@ -49,7 +49,7 @@ class OaOptions(ctypes.Structure):
                 ("want_resid", ctypes.c_int),
                 ("fft_mode", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("ns_mode", ctypes.c_int), ("vks_from_rho", ctypes.c_int),
-                ("autotune", ctypes.c_int)]
+                ("autotune", ctypes.c_int), ("gemm_split", ctypes.c_int)]
 
 
 class OaStepOut(ctypes.Structure):
@@ -118,6 +118,7 @@ def load():
                        ("oa_overlap", [vp, vp]),
                        ("oa_ortho_ns", [vp, cd, ci, vp, vp, vp]),
                        ("oa_tmat_set", [vp, vp]), ("oa_rotate", [vp]),
+                       ("oa_ortho_transform", [vp, vp, ci]),
                        ("oa_step", [vp, vp]), ("oa_residual", [vp, vp, vp]), ("oa_sync", [vp]),
                        ("oa_step_result", [vp, vp]), ("oa_timings", [vp, vp, ci]),
                        ("oa_set_timing", [vp, ci]),
@@ -138,10 +139,36 @@ def ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
 
 
+def local_pseudo_struct(pos, species, gmin, dg, vps, vps_g0):
+    """(OaLocalPseudo, arrays it points into): fixed ions (bohr), species
+    indices and per-species spline tables vps[nspecies][4][nspline]."""
+    keep = [np.ascontiguousarray(pos, np.float64).reshape(-1, 3),
+            np.ascontiguousarray(species, np.int32),
+            np.ascontiguousarray(vps, np.float64),
+            np.ascontiguousarray(vps_g0, np.float64)]
+    lp = OaLocalPseudo(keep[0].shape[0], keep[2].shape[0], keep[2].shape[2],
+                       ptr(keep[0]), ptr(keep[1]), float(gmin), float(dg),
+                       ptr(keep[2]), ptr(keep[3]))
+    return lp, keep
+
+
 def default_options():
     o = OaOptions()
     load().oa_default_options(ctypes.byref(o))
     return o
+
+
+def ortho_transform(S):
+    """(T, status) with T = (S/2)^{-1/2} of a host S (reference scaling) on
+    the current device: oa_ortho_transform, the stateless
+    CPORTHOG::CP_orthoTransform."""
+    S = np.ascontiguousarray(S, np.float64)
+    n = S.shape[0]
+    T = np.zeros_like(S)
+    st = load().oa_ortho_transform(ptr(T), ptr(S), n)
+    if st not in (OA_OK, OA_ENOCONV):
+        raise OaError(st, "oa_ortho_transform failed")
+    return T, st
 
 
 def kvecs(ecut_ry, hmat):

@@ -605,27 +605,15 @@ hipError_t launch_plane(const FftCtx& c, bool fwd, hipStream_t st) {
   const Geom& g = c.g;
   const dim3 grid(g.nz, c.nchunk);
   const size_t smem = plane_smem(c, sizeof(T) == 4);
-  // raise the dynamic LDS limit once per kernel instantiation
-  static size_t granted_fwd = 65536, granted_bwd = 65536;
   if (fwd) {
     auto k = k_plane_fwd<T, NTH, MAXS>;
-    if (smem > granted_fwd) {
-      hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)smem);
-      if (e != hipSuccess) return e;
-      granted_fwd = smem;
-    }
+    if (hipError_t e = grant_lds((const void*)k, smem); e != hipSuccess) return e;
     hipLaunchKernelGGL(k, grid, dim3(NTH), smem, st, g, c.px, c.py, c.kcp, c.pcp, c.cs, c.kylo,
                        c.kyhi, c.twx, c.twy, (const cpx<T>*)c.zbuf, c.occ, (T*)c.psir,
                        c.rho_part);
   } else {
     auto k = k_plane_bwd<T, NTH, MAXS>;
-    if (smem > granted_bwd) {
-      hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)smem);
-      if (e != hipSuccess) return e;
-      granted_bwd = smem;
-    }
+    if (hipError_t e = grant_lds((const void*)k, smem); e != hipSuccess) return e;
     hipLaunchKernelGGL(k, grid, dim3(NTH), smem, st, g, c.px, c.py, c.kcp, c.pcp, c.cs, c.kylo,
                        c.kyhi, c.twx, c.twy, (const T*)c.psir, c.vks, c.occ, (cpx<T>*)c.zbuf);
   }

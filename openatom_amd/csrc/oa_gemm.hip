@@ -31,6 +31,7 @@
 #include <type_traits>
 
 #include "oa_common.h"
+#include "oa_gemm_common.h"
 #include "oa_kernels.h"
 
 namespace oa {
@@ -43,19 +44,8 @@ __device__ __forceinline__ v4d mfma16(double a, double b, v4d c) {
 
 __device__ __forceinline__ v4d vzero() { return (v4d){0.0, 0.0, 0.0, 0.0}; }
 
-// f(integral_constant<int, I>) for I in [B, E): compile-time indices into
-// register arrays
-template <int B, int E, typename F>
-__device__ __forceinline__ void static_for(F&& f) {
-  if constexpr (B < E) {
-    f(std::integral_constant<int, B>{});
-    static_for<B + 1, E>(f);
-  }
-}
-
 // ================================================================== gram
 constexpr int kGB = 128;        // output block edge
-
 
 // Diagonal block of the symmetric product: its 36 upper tiles (ti <= tj) in
 // four groups with few distinct row fragments per wave:
@@ -93,38 +83,6 @@ __host__ __device__ constexpr int dg_b(int W, int t) {
   return (W == 0 || W == 3) ? up_j(t, 4) : 2 + t % 4;
 }
 
-// One K slice: chunks [c0, c1) of 16 k values starting at `base`; loads are
-// masked to the region [rs, re), which is either the kx=0 part [0, k_lo) or
-// the rest [k_lo, K), so a slice has one weight.
-struct GSlice {
-  int rs, re, base, c0, c1;
-  double w;
-};
-
-__device__ __forceinline__ GSlice gram_slice(const GramArgs& a, int sp, int nlo) {
-  GSlice s;
-  int idx, cnt;
-  const int klo = min(max(a.k_lo, 0), a.K);
-  if (sp < nlo) {
-    s.rs = 0;
-    s.re = klo;
-    idx = sp;
-    cnt = nlo;
-    s.w = a.w_lo;
-  } else {
-    s.rs = klo;
-    s.re = a.K;
-    idx = sp - nlo;
-    cnt = a.nslice - nlo;
-    s.w = a.w_hi;
-  }
-  s.base = s.rs & ~15;
-  const int nch = (s.re - s.base + 15) >> 4;
-  s.c0 = (int)(((long long)idx * nch) / cnt);
-  s.c1 = (int)(((long long)(idx + 1) * nch) / cnt);
-  return s;
-}
-
 // f[e] = X[row][kk + e]: one 32-byte load, kk a multiple of 4 inside the
 // row's allocation (chunks start on multiples of 16 below K <= lda, lda a
 // multiple of 16).  Unconditional: rows past the matrix are clamped by the
@@ -144,29 +102,6 @@ __device__ __forceinline__ void gfrag(const double* __restrict__ X, int lda, int
 __device__ __forceinline__ void gmask(int kk, int rs, int re, double (&f)[4]) {
 #pragma unroll
   for (int e = 0; e < 4; e++) f[e] = (kk + e >= rs && kk + e < re) ? f[e] : 0.0;
-}
-
-// Register ring of D chunk buffers: the loads of chunk c + D are issued as
-// soon as the MFMAs of chunk c have been issued, so D - 1 chunks of MFMAs
-// cover the memory latency.  load(c, buf) fills a buffer for chunk c;
-// mma(c, buf) issues the chunk's MFMAs.  Loads are issued unconditionally
-// (past the slice they repeat its last chunk): a load that only some paths
-// issue forces the compiler to wait for every outstanding load.
-template <int D, typename Buf, typename L, typename M>
-__device__ __forceinline__ void chunk_ring(int c0, int c1, Buf (&ring)[D], L&& load, M&& mma) {
-  if (c0 >= c1) return;
-  const int cl = c1 - 1;
-  static_for<0, D>([&](auto dc) __attribute__((always_inline)) {
-    constexpr int d = decltype(dc)::value;
-    load(min(c0 + d, cl), ring[d]);
-  });
-  for (int c = c0; c < c1; c += D) {
-    static_for<0, D>([&](auto dc) __attribute__((always_inline)) {
-      constexpr int d = decltype(dc)::value;
-      if (c + d < c1) mma(c + d, ring[d]);
-      load(min(c + d + D, cl), ring[d]);
-    });
-  }
 }
 
 template <int NF>
@@ -386,13 +321,6 @@ __global__ __launch_bounds__(256) void k_gram(GramArgs a, int nbe, int nbe_b, in
   }
 }
 
-static int gram_nlo(const GramArgs& a) {
-  if (a.k_lo <= 0) return 0;
-  if (a.k_lo >= a.K) return a.nslice;
-  const int nlo = (int)((double)a.nslice * a.k_lo / a.K + 0.5);
-  return std::min(std::max(nlo, 1), a.nslice - 1);
-}
-
 int gram_slices(int n, int K, int k_lo, int sym, int target) {
   const int nbe = (n + kGB - 1) / kGB;
   const int blocks = sym ? nbe * (nbe + 1) / 2 : nbe * nbe;
@@ -415,9 +343,10 @@ hipError_t gram_f64(const GramArgs& a, hipStream_t st) {
 
 // C[e] = alpha * sum_sp part[sp][e'] with e' = e or its mirror; 64 outputs
 // per workgroup, 16 groups of slices summed through LDS in a fixed order
-__global__ __launch_bounds__(1024) void k_gram_reduce(const double* __restrict__ part,
-                                                      int nslice, int n, int nb, int sym,
-                                                      double alpha, double* __restrict__ C) {
+template <typename P>
+__global__ __launch_bounds__(1024) void k_gram_reduce(const P* __restrict__ part, int nslice,
+                                                      int n, int nb, int sym, double alpha,
+                                                      double* __restrict__ C) {
   __shared__ double red[16][64];
   const int x = threadIdx.x & 63, y = threadIdx.x >> 6;
   const size_t nn = (size_t)n * nb;
@@ -430,7 +359,7 @@ __global__ __launch_bounds__(1024) void k_gram_reduce(const double* __restrict__
       i = j;
       j = t;
     }
-    const double* p = part + (size_t)i * nb + j;
+    const P* p = part + (size_t)i * nb + j;
     // thread y sums slices y, y+16, y+32, ...: 16 loads in flight,
     // branch-free (slice index clamped, value masked)
     for (int base = 0; base < nslice; base += 256) {
@@ -438,7 +367,7 @@ __global__ __launch_bounds__(1024) void k_gram_reduce(const double* __restrict__
 #pragma unroll
       for (int u = 0; u < 16; u++) {
         const int sp = base + y + 16 * u;
-        const double w = p[(size_t)min(sp, nslice - 1) * nn];
+        const double w = (double)p[(size_t)min(sp, nslice - 1) * nn];
         v[u] = sp < nslice ? w : 0.0;
       }
 #pragma unroll
@@ -464,7 +393,8 @@ __global__ __launch_bounds__(1024) void k_gram_reduce(const double* __restrict__
 // slices y, y + 16, ...; loads branch-free (slice index clamped, value
 // masked), 16 in flight; the 16 slice groups are summed through LDS in a
 // fixed order.
-__global__ __launch_bounds__(1024) void k_gram_reduce_sym(const double* __restrict__ part,
+template <typename P>
+__global__ __launch_bounds__(1024) void k_gram_reduce_sym(const P* __restrict__ part,
                                                           int nslice, int n, double alpha,
                                                           double* __restrict__ C) {
   __shared__ double red[16][64];
@@ -478,14 +408,14 @@ __global__ __launch_bounds__(1024) void k_gram_reduce_sym(const double* __restri
   const int x = threadIdx.x & 63, y = threadIdx.x >> 6;
   const int i = ti * 16 + (blockIdx.x & 3) * 4 + (x >> 4), j = tj * 16 + (x & 15);
   const size_t nn = (size_t)n * n;
-  const double* p = part + (size_t)min(i, n - 1) * n + min(j, n - 1);
+  const P* p = part + (size_t)min(i, n - 1) * n + min(j, n - 1);
   double s0 = 0.0, s1 = 0.0;
   for (int base = 0; base < nslice; base += 256) {
     double v[16];
 #pragma unroll
     for (int u = 0; u < 16; u++) {
       const int sp = base + y + 16 * u;
-      const double w = p[(size_t)min(sp, nslice - 1) * nn];
+      const double w = (double)p[(size_t)min(sp, nslice - 1) * nn];
       v[u] = sp < nslice ? w : 0.0;
     }
 #pragma unroll
@@ -506,19 +436,31 @@ __global__ __launch_bounds__(1024) void k_gram_reduce_sym(const double* __restri
   }
 }
 
-hipError_t gram_reduce(const double* part, int nslice, int n, int nb, int sym, double alpha,
-                       double* C, hipStream_t st) {
+template <typename P>
+static hipError_t gram_reduce_t(const P* part, int nslice, int n, int nb, int sym, double alpha,
+                                double* C, hipStream_t st) {
   if (sym && nb != n) return hipErrorInvalidValue;
   if (sym) {
     const int nt = (n + 15) >> 4;
-    hipLaunchKernelGGL(k_gram_reduce_sym, dim3(4 * (nt * (nt + 1) / 2)), dim3(1024), 0, st,
+    hipLaunchKernelGGL(k_gram_reduce_sym<P>, dim3(4 * (nt * (nt + 1) / 2)), dim3(1024), 0, st,
                        part, nslice, n, alpha, C);
     return hipGetLastError();
   }
   const size_t nn = (size_t)n * nb;
-  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)((nn + 63) / 64)), dim3(1024), 0, st, part,
-                     nslice, n, nb, sym, alpha, C);
+  hipLaunchKernelGGL(k_gram_reduce<P>, dim3((unsigned)((nn + 63) / 64)), dim3(1024), 0, st,
+                     part, nslice, n, nb, sym, alpha, C);
   return hipGetLastError();
+}
+
+hipError_t gram_reduce(const double* part, int nslice, int n, int nb, int sym, double alpha,
+                       double* C, hipStream_t st) {
+  return gram_reduce_t(part, nslice, n, nb, sym, alpha, C, st);
+}
+
+// FP32 partials (split-BF16 gram), summed in FP64
+hipError_t gram_reduce_f32(const float* part, int nslice, int n, int nb, int sym, double alpha,
+                           double* C, hipStream_t st) {
+  return gram_reduce_t(part, nslice, n, nb, sym, alpha, C, st);
 }
 
 // ================================================================== rotate
@@ -792,15 +734,7 @@ __global__ __launch_bounds__(kRotThreads) void k_rotate_p(RotArgs a) {
 }
 
 static int rotate_pgrid(int K) {
-  static int ncu = 0;
-  if (ncu <= 0) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        ncu <= 0)
-      ncu = 256;
-  }
-  return std::max(1, std::min(ncu, (K + kRK - 1) / kRK));
+  return std::max(1, std::min(cu_count(), (K + kRK - 1) / kRK));
 }
 
 int rotate_nwg(int n, int K) {
@@ -822,13 +756,7 @@ hipError_t rotate_f64(const RotArgs& args, hipStream_t st) {
                    k_rotate_p<nc, false, true>, k_rotate_p<nc, true, true>
     void (*const fns[16])(RotArgs) = {OA_ROT(1), OA_ROT(2), OA_ROT(4), OA_ROT(8)};
 #undef OA_ROT
-    static bool granted[16] = {};
-    if (!granted[v]) {
-      const hipError_t e = hipFuncSetAttribute(
-          (const void*)fns[v], hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRotSmem);
-      if (e != hipSuccess) return e;
-      granted[v] = true;
-    }
+    if (const hipError_t e = grant_lds((const void*)fns[v], kRotSmem); e != hipSuccess) return e;
     hipLaunchKernelGGL(fns[v], dim3(rotate_pgrid(a.K)), dim3(kRotThreads), kRotSmem, st, a);
     return hipGetLastError();
   }
@@ -1410,18 +1338,12 @@ static const void* nr_kernel(int n) {
 }
 
 static bool ns_smem_granted(int device) {
-  static int done = -1;
-  if (done == device) return true;
   int lds_max = 0;
   if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
           hipSuccess ||
       (size_t)lds_max < kNsSmem)
     return false;
-  if (hipFuncSetAttribute((const void*)k_ns_coop, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)kNsSmem) != hipSuccess)
-    return false;
-  done = device;
-  return true;
+  return grant_lds((const void*)k_ns_coop, kNsSmem) == hipSuccess;
 }
 
 // Can all n x n tiles be resident at once for the cooperative launch?
@@ -1441,9 +1363,7 @@ bool ns_rows_possible(int n, int split, int device) {
     return false;
   const size_t smem = ns_rows_smem(n);
   const void* fn = nr_kernel(n);
-  if (smem > (size_t)lds_max ||
-      hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem) != hipSuccess)
-    return false;
+  if (smem > (size_t)lds_max || grant_lds(fn, smem) != hipSuccess) return false;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kNrWaves * 64, smem) !=
       hipSuccess)
     return false;

@@ -5,6 +5,12 @@
 
 namespace oa {
 
+// Raise fn's dynamic-LDS limit on the current device to at least `bytes`
+// (once per kernel, device and larger request; thread-safe).  oa_attr.hip
+hipError_t grant_lds(const void* fn, size_t bytes);
+// compute units of the current device (read once per device; thread-safe)
+int cu_count();
+
 // Everything the FFT chain needs for one system (device pointers).
 struct FftCtx {
   Geom g;
@@ -175,6 +181,9 @@ hipError_t gram_reduce(const double* part, int nslice, int n, int nb, int sym, d
 // With sd_psi set (n <= 128 only) the epilogue also applies the steepest-
 // descent update to the rows of X: sd_psi[j][k] += dtm[k/2] * Ynew[j][k]
 // (sd_psi == X), and writes per-workgroup partials of sum Ynew^2 to fpart.
+// rotate_bf16 only: Xadd (may be null, not Y) is added to the result,
+// Y = beta Y + s(k) M^T X + Xadd, and m_minus_i uses M - I for M (with
+// Xadd = X: the update form X + X (M - I) of X M).
 struct RotArgs {
   const double* X;
   const double* M;
@@ -185,9 +194,25 @@ struct RotArgs {
   const double* dtm;
   double* fpart;
   int ni;
+  const double* Xadd;
+  int m_minus_i;
 };
 hipError_t rotate_f64(const RotArgs& a, hipStream_t st);
 int rotate_nwg(int n, int K);  // workgroups of a rotate launch (fpart entries)
+
+// ---- split-BF16 products of the mixed-precision step (oa_gemm_bf16.hip) ----
+// Each FP64 operand is split into three BF16 terms; the six term products
+// with i + j <= 2 are summed on the matrix cores in FP32 (error ~ FP32
+// accumulation over one K slice or over the inner dimension).
+// gram_bf16: the product of gram_f64 with FP32 partials part[nslice][n][nb]
+// (every block whole, also when sym), summed by gram_reduce_f32.
+hipError_t gram_bf16(const GramArgs& a, float* part, hipStream_t st);
+hipError_t gram_reduce_f32(const float* part, int nslice, int n, int nb, int sym, double alpha,
+                           double* C, hipStream_t st);
+// rotate_bf16: the product of rotate_f64 for n, ni <= 128 and ldx a multiple
+// of 64 (no SD epilogue); fpart, if set, gets per-workgroup sums of Ynew^2
+hipError_t rotate_bf16(const RotArgs& a, hipStream_t st);
+int rotate_bf16_nwg(int K);  // workgroups of a rotate_bf16 launch (fpart entries)
 
 // Newton-Schulz for T = (S/2)^{-1/2} with device-side convergence test
 // (ortho.C:862-985).  Work: X[2], T[2] (n*n each), errpart[(max_iter+1) * nwg].

@@ -486,14 +486,8 @@ hipError_t plane3_info(int npx, int* rowparts, int* occ_g2r, int* occ_r2g, int* 
   const void* r = (const void*)k3_r2g<T, X1, X2, Y1, Y2, NT, VREG, false>;
   *rho_ok = r2g_rho_ok<X1, X2, NT>() ? 1 : 0;
   hipError_t e;
-  if (smem > 65536) {
-    if ((e = hipFuncSetAttribute(g, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem)) !=
-        hipSuccess)
-      return e;
-    if ((e = hipFuncSetAttribute(r, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem)) !=
-        hipSuccess)
-      return e;
-  }
+  if ((e = oa::grant_lds(g, smem)) != hipSuccess || (e = oa::grant_lds(r, smem)) != hipSuccess)
+    return e;
   if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(occ_g2r, g, NT, smem)) != hipSuccess)
     return e;
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ_r2g, r, NT, smem);
@@ -508,26 +502,14 @@ hipError_t launch_plane3(int kind, const FftCtx& c, hipStream_t st) {
   constexpr bool VREG = sizeof(T) == 4 && S::ROUNDS * 2 * X2 <= 48;
   const size_t smem = plane3_smem<T, X1, X2, Y1, Y2, NT>(c.g.nplane_x);
   if (c.g.nx != S::NX || c.g.ny != S::NY) return hipErrorInvalidValue;
-  // raise the dynamic LDS limit of an instantiation once per larger request
-  static size_t granted[4] = {65536, 65536, 65536, 65536};
-  static size_t granted_wide = 65536;
-  auto grant_fn = [&](const void* fn, size_t& have) -> hipError_t {
-    if (smem <= have) return hipSuccess;
-    const hipError_t e =
-        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-    if (e == hipSuccess) have = smem;
-    return e;
-  };
-  auto grant = [&](const void* fn) -> hipError_t {
-    if (kind < 0 || kind > 3) return hipSuccess;
-    return grant_fn(fn, granted[kind]);
-  };
+  // raise the dynamic LDS limit of an instantiation (once per device)
+  auto grant = [&](const void* fn) -> hipError_t { return oa::grant_lds(fn, smem); };
   hipError_t e = hipSuccess;
   switch (kind) {
     case G2R: {
       if (g2r_wide<T, X1, X2, Y1, Y2, NT>(c.g.nplane_x)) {
         auto fn = k3_g2r<T, X1, X2, Y1, Y2, NT, S::ROUNDS, false>;
-        if ((e = grant_fn((const void*)fn, granted_wide)) != hipSuccess) return e;
+        if ((e = grant((const void*)fn)) != hipSuccess) return e;
         hipLaunchKernelGGL(fn, dim3(c.g.nz, 1, c.nchunk), dim3(NT), smem, st, c);
       } else {
         auto fn = k3_g2r<T, X1, X2, Y1, Y2, NT, kRowRounds, false>;

@@ -9,7 +9,9 @@
 //      flight, on that request's own stream, followed by an event;
 //   3. retires finished steps (event complete): reads the step's energies and
 //      |F|, and finishes the request at |F| <= f_tol (tol_cp_min,
-//      CP_State_GSpacePlane.C:158-191), at max_steps, or on an error,
+//      CP_State_GSpacePlane.C:158-191), at max_steps, or on an error (a
+//      step that could not be enqueued, or an event that reports anything
+//      but completion or "not ready"),
 //      releasing its memory so that queued requests can be admitted; a
 //      mixed-precision request that reaches fp64_below or stalls continues
 //      in an FP64 context.
@@ -29,6 +31,7 @@
 
 #include "oa_internal.h"
 #include "oa_serve.h"
+#include "oa_serve_poll.h"
 
 namespace {
 
@@ -43,9 +46,17 @@ struct Job {
   oa_request r{};
   std::vector<int> kx, ky, kz;
   std::vector<double> occ, psi0, vks;
+  // self-consistent request: the ions' local pseudopotential (copied)
+  bool scf = false;
+  oa_local_pseudo lp{};
+  std::vector<double> lp_pos, lp_vps, lp_vps_g0;
+  std::vector<int> lp_species;
   size_t bytes = 0;
   clk::time_point t_submit;
 };
+
+using oa_serve::classify_event;
+using oa_serve::StepPoll;
 
 struct Active {
   Job job;
@@ -118,12 +129,26 @@ oa_system job_system(Job& j) {
   return sys;
 }
 
+// V(r) of a new context: the fixed input, or the ions whose local
+// pseudopotential enters V_KS[rho] at every step
+oa_status load_potential(Active* a) {
+  if (a->job.scf) {
+    oa_local_pseudo lp = a->job.lp;
+    lp.pos = a->job.lp_pos.data();
+    lp.species = a->job.lp_species.data();
+    lp.vps = a->job.lp_vps.data();
+    lp.vps_g0 = a->job.lp_vps_g0.data();
+    return oa_local_pseudo_set(a->ctx, &lp);
+  }
+  return oa_vks_set(a->ctx, a->job.vks.data());
+}
+
 // create and load one admitted request's context; false: it finished with an error
 bool start(oa_server* s, Active* a) {
   oa_system sys = job_system(a->job);
   oa_status st = oa_create(s->device, &sys, &a->job.r.opt, &a->ctx);
   if (st == OA_OK) st = oa_psi_set(a->ctx, a->job.psi0.data());
-  if (st == OA_OK) st = oa_vks_set(a->ctx, a->job.vks.data());
+  if (st == OA_OK) st = load_potential(a);
   if (st == OA_OK && hipEventCreateWithFlags(&a->ev, hipEventDisableTiming) != hipSuccess)
     st = OA_EHIP;
   if (st != OA_OK) {
@@ -145,7 +170,7 @@ oa_status to_fp64(oa_server* s, Active* a) {
   oa_system sys = job_system(a->job);
   st = oa_create(s->device, &sys, &a->job.r.opt, &a->ctx);
   if (st == OA_OK) st = oa_psi_set(a->ctx, a->job.psi0.data());
-  if (st == OA_OK) st = oa_vks_set(a->ctx, a->job.vks.data());
+  if (st == OA_OK) st = load_potential(a);
   a->in_fp64 = true;
   return st;
 }
@@ -200,10 +225,15 @@ void scheduler(oa_server* s) {
       Active* a = *it;
       oa_status st = OA_OK;
       bool end = false;
-      if (a->issue_failed) {  // the step could not be enqueued
+      const StepPoll poll = a->issue_failed ? StepPoll::Failed
+                            : a->inflight   ? classify_event(hipEventQuery(a->ev))
+                                            : StepPoll::Running;
+      if (poll == StepPoll::Failed) {  // not enqueued, or the device reported an error
+        (void)hipGetLastError();
+        a->inflight = false;
         st = OA_EHIP;
         end = true;
-      } else if (a->inflight && hipEventQuery(a->ev) == hipSuccess) {
+      } else if (poll == StepPoll::Done) {
         a->inflight = false;
         a->steps++;
         progressed = true;
@@ -265,14 +295,31 @@ oa_status oa_server_create(int device, size_t hbm_budget_bytes, int max_active,
 }
 
 oa_status oa_server_submit(oa_server* s, const oa_request* req) {
-  if (!s || !req || !req->psi0 || !req->vks || req->max_steps < 1) return OA_EINVAL;
+  if (!s || !req || !req->psi0 || (!req->vks && !req->pseudo) || req->max_steps < 1)
+    return OA_EINVAL;
   Job j;
   j.r = *req;
   const oa_system& sys = req->sys;
+  if (!req->vks) {
+    const oa_local_pseudo& p = *req->pseudo;
+    if (p.natoms < 0 || p.nspecies < 1 || p.nspline < 1 || (p.natoms > 0 && (!p.pos || !p.species)) ||
+        !p.vps || !p.vps_g0)
+      return OA_EINVAL;
+    j.scf = true;
+    j.r.opt.vks_from_rho = 1;
+    j.lp = p;
+    j.lp_pos.assign(p.pos, p.pos + 3 * (size_t)p.natoms);
+    j.lp_species.assign(p.species, p.species + p.natoms);
+    j.lp_vps.assign(p.vps, p.vps + 4 * (size_t)p.nspecies * p.nspline);
+    j.lp_vps_g0.assign(p.vps_g0, p.vps_g0 + p.nspecies);
+  } else {
+    j.r.opt.vks_from_rho = 0;
+  }
+  j.r.pseudo = nullptr;
   (void)hipSetDevice(s->device);
-  j.bytes = oa_ctx_bytes(&sys, &req->opt);
+  j.bytes = oa_ctx_bytes(&sys, &j.r.opt);
   if (req->opt.precision == OA_MIXED && req->fp64_below > 0.0) {
-    oa_options o64 = req->opt;
+    oa_options o64 = j.r.opt;
     o64.precision = OA_FP64;
     j.bytes = std::max(j.bytes, oa_ctx_bytes(&sys, &o64));
   }
@@ -288,8 +335,10 @@ oa_status oa_server_submit(oa_server* s, const oa_request* req) {
   }
   if (sys.occ) j.occ.assign(sys.occ, sys.occ + sys.nstates);
   j.psi0.assign(req->psi0, req->psi0 + (size_t)2 * sys.nstates * npts);
-  const size_t N = (size_t)sys.grid[0] * sys.grid[1] * sys.grid[2];
-  j.vks.assign(req->vks, req->vks + N);
+  if (!j.scf) {
+    const size_t N = (size_t)sys.grid[0] * sys.grid[1] * sys.grid[2];
+    j.vks.assign(req->vks, req->vks + N);
+  }
   j.t_submit = clk::now();
   {
     std::lock_guard<std::mutex> lk(s->mu);

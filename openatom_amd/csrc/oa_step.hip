@@ -654,7 +654,7 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
     for (int kind = 4; kind >= 2; kind--)
       if (!c->dry && oa::ns_rows_possible(n, oa::ns_rows_split(kind), c->device))
         c->ns_kinds.push_back(kind);
-    if (oa::ns_coop_possible(n, c->device)) c->ns_kinds.push_back(1);
+    if (!c->dry && oa::ns_coop_possible(n, c->device)) c->ns_kinds.push_back(1);
     c->ns_kinds.push_back(0);
   } else if (nsm == 1) {
     c->ns_kinds.push_back(0);
@@ -867,8 +867,15 @@ oa_status st_r2g(oa_ctx* c, int ev0 = -1) {
   return st_energies(c);
 }
 
+// Split-BF16 products of a mixed-precision context (opt.gemm_split bits):
+// Lambda (gram and projection), S, Psi T.  The projection and the rotation
+// have a split kernel up to 128 states.
+constexpr int kSplitLambda = 1, kSplitS = 2, kSplitRot = 4;
+bool split_on(const oa_ctx* c, int bit) { return c->fp32 && (c->opt.gemm_split & bit) != 0; }
+bool split_rotate_ok(const oa_ctx* c) { return c->nst <= 128 && (2 * c->ld) % 64 == 0; }
+
 oa_status st_gram(oa_ctx* c, const double2* A, const double2* B, bool sym, double wlo, double whi,
-                  double* C) {
+                  double* C, bool split = false) {
   oa::GramArgs g;
   g.A = reinterpret_cast<const double*>(A);
   g.B = reinterpret_cast<const double*>(B);
@@ -882,6 +889,12 @@ oa_status st_gram(oa_ctx* c, const double2* A, const double2* B, bool sym, doubl
   g.nslice = sym ? c->nslice_sym : c->nslice_full;
   g.part = c->gpart;
   g.nb = c->nst;
+  if (split) {  // FP32 partials in the same buffer
+    float* pf = reinterpret_cast<float*>(c->gpart);
+    HIPCHK(oa::gram_bf16(g, pf, c->stream));
+    HIPCHK(oa::gram_reduce_f32(pf, g.nslice, c->nst, c->nst, g.sym, 1.0, C, c->stream));
+    return OA_OK;
+  }
   HIPCHK(oa::gram_f64(g, c->stream));
   HIPCHK(oa::gram_reduce(c->gpart, g.nslice, c->nst, c->nst, g.sym, 1.0, C, c->stream));
   return OA_OK;
@@ -889,8 +902,11 @@ oa_status st_gram(oa_ctx* c, const double2* A, const double2* B, bool sym, doubl
 
 // sd: also apply psi += dt/m F to X (= psi) in the same pass (n <= 128)
 // fnorm: also leave per-workgroup partials of sum Y^2 in fpart (n <= 128)
+// split: the split-BF16 kernel (no SD epilogue; split_rotate_ok);
+// update: Y = X + X (M - I) (split only)
 oa_status st_rotate(oa_ctx* c, const double2* X, const double* M, double2* Y, double beta,
-                    double slo, double shi, bool sd = false, bool fnorm = false) {
+                    double slo, double shi, bool sd = false, bool fnorm = false,
+                    bool split = false, bool update = false) {
   oa::RotArgs r{};
   r.X = reinterpret_cast<const double*>(X);
   r.M = M;
@@ -907,8 +923,23 @@ oa_status st_rotate(oa_ctx* c, const double2* X, const double* M, double2* Y, do
     r.dtm = c->dtm;
   }
   if (sd || fnorm) r.fpart = c->fpart;
+  if (split) {
+    if (sd) return fail(c, OA_EINVAL, "split rotate without SD epilogue");
+    if (update) {
+      r.Xadd = reinterpret_cast<const double*>(X);
+      r.m_minus_i = 1;
+    }
+    HIPCHK(oa::rotate_bf16(r, c->stream));
+    return OA_OK;
+  }
   HIPCHK(oa::rotate_f64(r, c->stream));
   return OA_OK;
+}
+
+// workgroups (|F|^2 partials) of the Lambda projection with fnorm
+int lambda_nwg(const oa_ctx* c) {
+  return split_on(c, kSplitLambda) && split_rotate_ok(c) ? oa::rotate_bf16_nwg(2 * c->npts)
+                                                           : oa::rotate_nwg(c->nst, 2 * c->npts);
 }
 
 // Lambda_ij = sum_stored Re conj(psi_i) F_j (alpha = 1, cpaimd.C:2527) and
@@ -916,12 +947,15 @@ oa_status st_rotate(oa_ctx* c, const double2* X, const double* M, double2* Y, do
 // (CP_State_GSpacePlane.C:2076-2086).  with_sd: the same pass also applies
 // the SD update and leaves the |F|^2 partials (then st_sd_finish).
 oa_status st_lambda(oa_ctx* c, bool with_sd = false) {
-  TRY(st_gram(c, c->psi, c->force, false, 1.0, 1.0, c->lam));
-  TRY(st_rotate(c, c->psi, c->lam, c->force, 1.0, -0.5, -1.0, with_sd));
+  const bool sp = split_on(c, kSplitLambda);
+  TRY(st_gram(c, c->psi, c->force, false, 1.0, 1.0, c->lam, sp));
+  const bool spr = sp && split_rotate_ok(c) && !with_sd;
+  TRY(st_rotate(c, c->psi, c->lam, c->force, 1.0, -0.5, -1.0, with_sd, false, spr));
   return OA_OK;
 }
 
-// |F|^2 from the partials of the fused rotate, then the redundancy sync
+// |F|^2 from the partials of the fused rotate (FP64 kernel), then the
+// redundancy sync
 oa_status st_sd_finish(oa_ctx* c) {
   HIPCHK(oa::reduce_to(c->fpart, oa::rotate_nwg(c->nst, 2 * c->npts), 1.0, &c->res->fnorm2,
                        c->stream));
@@ -933,11 +967,12 @@ oa_status st_sd_finish(oa_ctx* c) {
 // be complete before gamma, so the update is a pass of its own
 oa_status st_lambda_cg(oa_ctx* c) {
   const size_t N2 = (size_t)2 * c->nst * c->ld;  // reals in F, zero padding included
-  TRY(st_gram(c, c->psi, c->force, false, 1.0, 1.0, c->lam));
+  const bool sp = split_on(c, kSplitLambda);
+  TRY(st_gram(c, c->psi, c->force, false, 1.0, 1.0, c->lam, sp));
   if (c->nst <= 128) {
-    TRY(st_rotate(c, c->psi, c->lam, c->force, 1.0, -0.5, -1.0, false, true));
-    HIPCHK(oa::reduce_to(c->fpart, oa::rotate_nwg(c->nst, 2 * c->npts), 1.0, &c->res->fnorm2,
-                         c->stream));
+    TRY(st_rotate(c, c->psi, c->lam, c->force, 1.0, -0.5, -1.0, false, true,
+                  sp && split_rotate_ok(c)));
+    HIPCHK(oa::reduce_to(c->fpart, lambda_nwg(c), 1.0, &c->res->fnorm2, c->stream));
   } else {
     TRY(st_rotate(c, c->psi, c->lam, c->force, 1.0, -0.5, -1.0));
     const double* fr = reinterpret_cast<const double*>(c->force);
@@ -1008,7 +1043,7 @@ oa_status st_sd(oa_ctx* c) {
 
 oa_status st_overlap(oa_ctx* c) {
   // S = 2 sum (kx0/sqrt2)^2 ... = sum_p w_p Re conj(psi_i) psi_j, w = 1 on kx=0, 2 elsewhere
-  TRY(st_gram(c, c->psi, c->psi, true, 1.0, 2.0, c->S));
+  TRY(st_gram(c, c->psi, c->psi, true, 1.0, 2.0, c->S, split_on(c, kSplitS)));
   c->have_S = true;
   return OA_OK;
 }
@@ -1036,7 +1071,8 @@ oa_status st_ns(oa_ctx* c, double tol, int max_iter) {
 }
 
 oa_status st_rot_psi(oa_ctx* c) {
-  TRY(st_rotate(c, c->psi, c->T, c->psi2, 0.0, 1.0, 1.0));
+  const bool sp = split_on(c, kSplitRot) && split_rotate_ok(c);
+  TRY(st_rotate(c, c->psi, c->T, c->psi2, 0.0, 1.0, 1.0, false, false, sp, sp));
   std::swap(c->psi, c->psi2);
   return OA_OK;
 }
@@ -1185,11 +1221,16 @@ oa_status autotune(oa_ctx* c) {
   tune_knob(
       c, e0, e1, "S K slices", c->tune_nslice_sym, c->nslice_sym,
       [&](int v) { c->nslice_sym = v; },
-      [&]() { return st_gram(c, c->psi, c->psi, true, 1.0 / std::sqrt(2.0), 1.0, c->S); });
+      [&]() {
+        return st_gram(c, c->psi, c->psi, true, 1.0 / std::sqrt(2.0), 1.0, c->S,
+                       split_on(c, kSplitS));
+      });
   tune_knob(
       c, e0, e1, "Lambda K slices", c->tune_nslice_full, c->nslice_full,
       [&](int v) { c->nslice_full = v; },
-      [&]() { return st_gram(c, c->psi, c->force, false, 0.5, 1.0, c->lam); });
+      [&]() {
+        return st_gram(c, c->psi, c->force, false, 0.5, 1.0, c->lam, split_on(c, kSplitLambda));
+      });
   if (c->ns_kinds.size() > 1) {
     // S = 2I converges in one iteration: the cost of one iteration either way
     std::vector<double> two((size_t)c->nst * c->nst, 0.0);
@@ -1244,6 +1285,7 @@ void oa_default_options(oa_options* o) {
   o->ns_mode = 0;
   o->vks_from_rho = 0;
   o->autotune = 0;
+  o->gemm_split = 1;
 }
 
 int oa_kvecs(double ecut_ry, const double hmat[9], int* kx, int* ky, int* kz) {

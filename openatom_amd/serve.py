@@ -2,18 +2,20 @@
 fixed-ion electronic minimisations of different sizes on one GPU, admitted
 by HBM budget and advanced concurrently by the library's scheduler thread."""
 import ctypes
+import itertools
 
 import numpy as np
 
 from . import _lib
-from ._lib import OaError, OaOptions, OaSystem, ptr
+from ._lib import OaError, OaLocalPseudo, OaOptions, OaSystem, ptr
 
 
 class OaRequest(ctypes.Structure):
     _fields_ = [("sys", OaSystem), ("opt", OaOptions), ("psi0", ctypes.c_void_p),
                 ("vks", ctypes.c_void_p), ("f_tol", ctypes.c_double),
                 ("max_steps", ctypes.c_int), ("psi_out", ctypes.c_void_p),
-                ("tag", ctypes.c_longlong), ("fp64_below", ctypes.c_double)]
+                ("tag", ctypes.c_longlong), ("fp64_below", ctypes.c_double),
+                ("pseudo", ctypes.POINTER(OaLocalPseudo))]
 
 
 class OaResult(ctypes.Structure):
@@ -97,33 +99,51 @@ class Server:
         if st != _lib.OA_OK:
             raise OaError(st, "oa_server_create failed")
         self.h = h
-        self._pending = {}  # tag -> arrays kept alive until the result returns
+        # The native tag of a request is an id unique within this server; it
+        # maps back to the caller's tag and to the arrays the scheduler may
+        # still write (psi_out), which stay alive until the result returns.
+        self._ids = itertools.count(1)
+        self._pending = {}  # native id -> (caller's tag, arrays kept alive, psi_out)
 
     def submit(self, system, tag, f_tol=1e-5, max_steps=1000, precision="fp64", want_psi=False,
-               fp64_below=None, **opts):
+               fp64_below=None, scf=False, **opts):
         """fp64_below: mixed precision continues in FP64 once |F| reaches
-        it or stalls (default 30 f_tol for mixed; 0 never switches)."""
+        it or stalls (default 30 f_tol for mixed; 0 never switches).
+        scf: the request carries its ions (system["pos"], ["species"],
+        ["pseudo"]) and every step rebuilds V_KS from its own density;
+        otherwise system["vks"] is a fixed V(r).  Tags need not be unique."""
         keep = []
         req = OaRequest()
         req.sys = _system_struct(system, keep)
         req.opt = _options(precision, **opts)
         psi0 = np.ascontiguousarray(system["psi"], np.complex128)
-        vks = np.ascontiguousarray(system["vks"], np.float64)
-        keep += [psi0, vks]
+        keep.append(psi0)
         req.psi0 = ptr(psi0)
-        req.vks = ptr(vks)
+        if scf:
+            t = system["pseudo"]
+            lp, arrs = _lib.local_pseudo_struct(system["pos"], system["species"], t["gmin"],
+                                                t["dg"], t["vps"], t["vps_g0"])
+            keep += [lp, arrs]
+            req.vks = None
+            req.pseudo = ctypes.pointer(lp)
+        else:
+            vks = np.ascontiguousarray(system["vks"], np.float64)
+            keep.append(vks)
+            req.vks = ptr(vks)
+            req.pseudo = None
         req.f_tol = float(f_tol)
         req.max_steps = int(max_steps)
         out = np.zeros_like(psi0) if want_psi else None
         req.psi_out = ptr(out)
-        req.tag = int(tag)
+        nid = next(self._ids)
+        req.tag = nid
         if fp64_below is None:
             fp64_below = 30.0 * f_tol if precision == "mixed" else 0.0
         req.fp64_below = float(fp64_below)
         st = self._L.oa_server_submit(self.h, ctypes.byref(req))
         if st != _lib.OA_OK:
             raise OaError(st, "oa_server_submit failed")
-        self._pending[int(tag)] = (keep, out)
+        self._pending[nid] = (tag, keep, out)
 
     def poll(self, max_results=64):
         buf = (OaResult * max_results)()
@@ -131,7 +151,8 @@ class Server:
         out = []
         for i in range(n):
             d = buf[i].as_dict()
-            _, psi = self._pending.pop(d["tag"], (None, None))
+            tag, _, psi = self._pending.pop(d["tag"])
+            d["tag"] = tag
             d["psi"] = psi
             out.append(d)
         return out

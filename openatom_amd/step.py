@@ -132,13 +132,7 @@ class CPContext:
     def set_local_pseudo(self, pos, species, gmin, dg, vps, vps_g0):
         """Fixed ions of a vks_from_rho context: positions (bohr), species
         indices and per-species spline tables vps[nspecies][4][nspline]."""
-        keep = [np.ascontiguousarray(pos, np.float64).reshape(-1, 3),
-                np.ascontiguousarray(species, np.int32),
-                np.ascontiguousarray(vps, np.float64),
-                np.ascontiguousarray(vps_g0, np.float64)]
-        lp = _lib.OaLocalPseudo(keep[0].shape[0], keep[2].shape[0], keep[2].shape[2],
-                                ptr(keep[0]), ptr(keep[1]), float(gmin), float(dg),
-                                ptr(keep[2]), ptr(keep[3]))
+        lp, keep = _lib.local_pseudo_struct(pos, species, gmin, dg, vps, vps_g0)
         self._chk(self._L.oa_local_pseudo_set(self.h, ctypes.byref(lp)))
 
     def set_nonlocal(self, pos, species, gmin, dg, proj, proj_g0, vnorm):

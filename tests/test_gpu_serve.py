@@ -119,3 +119,79 @@ def test_mixed_request_finishes_in_fp64(gpu):
     assert abs(rmx["eelec"] - r64["eelec"]) / s["natoms"] < 1e-8
     need = max(serve.ctx_bytes(s, **OPTS), serve.ctx_bytes(s, precision="mixed", **OPTS))
     assert rmx["hbm_bytes"] == need
+
+
+def test_scf_request_equals_direct_steps(gpu):
+    """A self-consistent request (ions instead of a fixed V: V_KS rebuilt
+    from each step's rho) gives the result of stepping a vks_from_rho
+    context directly, alone and in a mix with fixed-V requests."""
+    ss = systems()
+    s = ss[0]
+    t = s["pseudo"]
+    with serve.Server(max_active=1) as srv:
+        srv.submit(s, "scf", f_tol=F_TOL, max_steps=MAX_STEPS, want_psi=True, scf=True, **OPTS)
+        alone = srv.drain()[0]
+    with serve.Server(max_active=4) as srv:
+        srv.submit(s, "scf", f_tol=F_TOL, max_steps=MAX_STEPS, want_psi=True, scf=True, **OPTS)
+        for k, o in enumerate(ss[1:]):
+            srv.submit(o, k, f_tol=F_TOL, max_steps=MAX_STEPS, **OPTS)
+        mix = {r["tag"]: r for r in srv.drain()}["scf"]
+    with CPContext(s["grid"], s["hmat"], s["ecut_ry"], s["nstates"], s["kx"], s["ky"], s["kz"],
+                   vks_from_rho=1, **OPTS) as c:
+        c.set_local_pseudo(s["pos"], s["species"], t["gmin"], t["dg"], t["vps"], t["vps_g0"])
+        c.set_psi(s["psi"])
+        for k in range(MAX_STEPS):
+            o = c.step()
+            if o["fnorm"] <= F_TOL:
+                break
+        psi = c.get_psi()
+    print(f"[serve scf] steps {alone['steps']} E {alone['eelec']:.10f} (direct {k + 1}, "
+          f"{o['eelec']:.10f})", flush=True)
+    # converged, or the same step cap reached everywhere
+    assert alone["status"] in (0, 4) and mix["status"] == alone["status"]
+    assert alone["steps"] == k + 1 == mix["steps"]
+    assert alone["eelec"] == o["eelec"] == mix["eelec"]
+    assert np.array_equal(alone["psi"], psi) and np.array_equal(mix["psi"], psi)
+
+
+def test_repeated_tag_results_stay_paired(gpu):
+    """Two requests under one tag: each result carries its own coefficients."""
+    ss = systems()
+    with serve.Server(max_active=2) as srv:
+        srv.submit(ss[0], "t", f_tol=F_TOL, max_steps=MAX_STEPS, want_psi=True, **OPTS)
+        srv.submit(ss[3], "t", f_tol=F_TOL, max_steps=MAX_STEPS, want_psi=True, **OPTS)
+        res = srv.drain()
+    assert sorted(r["psi"].shape for r in res) == sorted([ss[0]["psi"].shape, ss[3]["psi"].shape])
+    assert all(r["tag"] == "t" and r["status"] == 0 for r in res)
+
+
+def test_concurrent_sizing_and_creation(gpu):
+    """oa_ctx_bytes from several host threads while another thread creates
+    and steps contexts (the submit path beside the scheduler): every sizing
+    agrees and the contexts still step correctly."""
+    import threading
+    ss = systems()
+    want = [serve.ctx_bytes(s, **OPTS) for s in ss]
+    errors = []
+
+    def sizer():
+        for _ in range(20):
+            for s, w in zip(ss, want):
+                b = serve.ctx_bytes(s, **OPTS)
+                if b != w:
+                    errors.append((b, w))
+
+    def creator():
+        for s in ss:
+            with CPContext(s["grid"], s["hmat"], s["ecut_ry"], s["nstates"], s["kx"], s["ky"],
+                           s["kz"], **OPTS) as c:
+                c.set_psi(s["psi"])
+                c.set_vks(s["vks"])
+                c.step()
+
+    th = [threading.Thread(target=sizer) for _ in range(6)] + [threading.Thread(target=creator)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errors
