@@ -251,9 +251,13 @@ def kb_setup(pos, spec, ecut_ry, seed):
     return kb, nlt
 
 
-def build(workload="water32", seed=1234, nstates=None, kvec_source=None):
+def build(workload="water32", seed=1234, nstates=None, kvec_source=None, pseudo="gaussian"):
     """Build one seeded system.  kvec_source(ecut_ry, hmat) -> (kx, ky, kz)
-    defaults to the library's own enumeration (oa_kvecs)."""
+    defaults to the library's own enumeration (oa_kvecs).  pseudo: "gaussian"
+    for the seeded Gaussian local potential and KB projector, "db" for the
+    local and oxygen s-channel KB tables of the reference's pseudopotential
+    database (openatom_amd/pseudo_db.py; the fixed V(r) of the system, used
+    only by fixed-potential steps, stays the seeded Gaussian one)."""
     w = dict(WORKLOADS[workload]) if isinstance(workload, str) else dict(workload)
     nst = int(nstates or w["nstates"])
     L = w["box_ang"] * BOHR_PER_ANG
@@ -264,11 +268,21 @@ def build(workload="water32", seed=1234, nstates=None, kvec_source=None):
     kx, ky, kz = src(w["ecut_ry"], hmat)
     psi = initial_coeffs(kx, ky, kz, hmat, nst, seed)
     V, params = gaussian_vloc(pos, spec, hmat, grid, seed)
-    tab = local_pseudo_tables(params, w["ecut_ry"])
+    if pseudo == "db":
+        from . import pseudo_db
+        tab, nlt = pseudo_db.water_tables(hmat, w["ecut_ry"])
+        kb = None
+        nl_atoms = [i for i, sp in enumerate(spec) if sp in nlt["order"]]
+        nlt.update(atoms=np.array(nl_atoms, np.int32), pos=np.asarray(pos)[nl_atoms],
+                   species=np.array([nlt["order"].index(spec[i]) for i in nl_atoms], np.int32))
+    elif pseudo == "gaussian":
+        tab = local_pseudo_tables(params, w["ecut_ry"])
+        kb, nlt = kb_setup(pos, spec, w["ecut_ry"], seed)
+    else:
+        raise ValueError(f"pseudo must be 'gaussian' or 'db', not {pseudo!r}")
     species = np.array([tab["order"].index(s) for s in spec], np.int32)
-    kb, nlt = kb_setup(pos, spec, w["ecut_ry"], seed)
     return dict(name=workload if isinstance(workload, str) else "custom", seed=seed,
                 hmat=hmat, grid=grid, ecut_ry=float(w["ecut_ry"]), nstates=nst,
                 natoms=len(spec), nmol=w["nmol"], pos=pos, spec=spec, kx=kx, ky=ky, kz=kz,
                 psi=psi, vks=V, vparams=params, species=species, pseudo=tab,
-                kbparams=kb, kb=nlt)
+                kbparams=kb, kb=nlt, pseudo_source=pseudo)
