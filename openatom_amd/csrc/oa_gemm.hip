@@ -408,7 +408,11 @@ __global__ __launch_bounds__(1024) void k_gram_reduce_sym(const P* __restrict__ 
   const int x = threadIdx.x & 63, y = threadIdx.x >> 6;
   const int i = ti * 16 + (blockIdx.x & 3) * 4 + (x >> 4), j = tj * 16 + (x & 15);
   const size_t nn = (size_t)n * n;
-  const P* p = part + (size_t)min(i, n - 1) * n + min(j, n - 1);
+  // a diagonal tile takes both halves from the upper triangle's partials, so
+  // C is symmetric bit for bit also where the partials of (i, j) and (j, i)
+  // were summed in different orders (split-BF16 term products)
+  const int ir = ti == tj ? min(i, j) : i, jr = ti == tj ? max(i, j) : j;
+  const P* p = part + (size_t)min(ir, n - 1) * n + min(jr, n - 1);
   double s0 = 0.0, s1 = 0.0;
   for (int base = 0; base < nslice; base += 256) {
     double v[16];
@@ -1387,17 +1391,24 @@ bool ns_coop_possible(int n, int device) {
 
 hipError_t ns_run(const NsArgs& a, hipStream_t st) {
   if (!a.coop) return ns_run_launches(a, st);
-  if (a.coop >= 2) {
-    NsArgs args = a;
-    void* params[] = {&args};
-    return hipLaunchCooperativeKernel(nr_kernel(a.n), dim3((a.n + 15) / 16, ns_rows_split(a.coop)),
-                                      dim3(kNrWaves * 64), params, ns_rows_smem(a.n), st);
-  }
-  const int nt = (a.n + NT - 1) / NT;
   NsArgs args = a;
   void* params[] = {&args};
-  return hipLaunchCooperativeKernel((const void*)k_ns_coop, dim3(nt * nt), dim3(256), params,
-                                    kNsSmem, st);
+  hipError_t e;
+  if (a.coop >= 2) {
+    e = hipLaunchCooperativeKernel(nr_kernel(a.n), dim3((a.n + 15) / 16, ns_rows_split(a.coop)),
+                                   dim3(kNrWaves * 64), params, ns_rows_smem(a.n), st);
+  } else {
+    const int nt = (a.n + NT - 1) / NT;
+    e = hipLaunchCooperativeKernel((const void*)k_ns_coop, dim3(nt * nt), dim3(256), params,
+                                   kNsSmem, st);
+  }
+  if (e == hipSuccess) return e;
+  // The runtime refuses a cooperative launch when the device's grid-barrier
+  // resource is held by another process (hipErrorIllegalState); a refused
+  // launch enqueues nothing, so the same iteration runs as two launches per
+  // iteration instead (its error partials fit the same buffer).
+  (void)hipGetLastError();
+  return ns_run_launches(a, st);
 }
 
 }  // namespace oa
