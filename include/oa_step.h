@@ -258,6 +258,19 @@ oa_status oa_ortho_ns(oa_ctx *ctx, double tol, int max_iter, int *iters, double 
  * the last iterate.  Allocates its device work space per call. */
 oa_status oa_ortho_transform(double *T, const double *S, int n);
 
+/* oa_ortho_transform without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The work space, S and T live in a fixed array of the library
+ * (OA_ORTHO_NOALLOC_MAX_N = 512: 14.7 MB per device, held until the
+ * process ends), set up on the first call on each device.  OA_EINVAL for
+ * n > OA_ORTHO_NOALLOC_MAX_N.  Same kernels and result as
+ * oa_ortho_transform.  All calls on one device share that device's array
+ * and it keeps the last call's contents, so calls that overlapped would
+ * race: the library serialises them with a per-device lock. */
+#define OA_ORTHO_NOALLOC_MAX_N 512
+oa_status oa_ortho_transform_noalloc(double *T, const double *S, int n);
+
 /* load a T for oa_rotate (the PairCalculator's orthoT input) */
 oa_status oa_tmat_set(oa_ctx *ctx, const double *T);
 

@@ -23,7 +23,7 @@ EXPORTS = (
     "oa_psi_get", "oa_vks_set",
     "oa_fft_g2r", "oa_psir_get", "oa_density_get", "oa_vpsi_r2g", "oa_force_get",
     "oa_force_set", "oa_lambda", "oa_sd_update", "oa_overlap", "oa_ortho_ns",
-    "oa_tmat_set", "oa_ortho_transform", "oa_rotate", "oa_step", "oa_residual", "oa_sync", "oa_step_result", "oa_timings",
+    "oa_tmat_set", "oa_ortho_transform", "oa_ortho_transform_noalloc", "oa_rotate", "oa_step", "oa_residual", "oa_sync", "oa_step_result", "oa_timings",
     "oa_set_timing", "oa_local_pseudo_set", "oa_vks_build", "oa_vks_get", "oa_tuning_log",
     "oa_nonlocal_set", "oa_nonlocal_get", "oa_nonlocal_force",
     # state files (oa_state.h)
@@ -119,6 +119,7 @@ def load():
                        ("oa_ortho_ns", [vp, cd, ci, vp, vp, vp]),
                        ("oa_tmat_set", [vp, vp]), ("oa_rotate", [vp]),
                        ("oa_ortho_transform", [vp, vp, ci]),
+                       ("oa_ortho_transform_noalloc", [vp, vp, ci]),
                        ("oa_step", [vp, vp]), ("oa_residual", [vp, vp, vp]), ("oa_sync", [vp]),
                        ("oa_step_result", [vp, vp]), ("oa_timings", [vp, vp, ci]),
                        ("oa_set_timing", [vp, ci]),
@@ -168,6 +169,21 @@ def ortho_transform(S):
     st = load().oa_ortho_transform(ptr(T), ptr(S), n)
     if st not in (OA_OK, OA_ENOCONV):
         raise OaError(st, "oa_ortho_transform failed")
+    return T, st
+
+
+ORTHO_NOALLOC_MAX_N = 512
+
+
+def ortho_transform_noalloc(S):
+    """(T, status) as ortho_transform, through oa_ortho_transform_noalloc:
+    no device allocation, n <= ORTHO_NOALLOC_MAX_N."""
+    S = np.ascontiguousarray(S, np.float64)
+    n = S.shape[0]
+    T = np.zeros_like(S)
+    st = load().oa_ortho_transform_noalloc(ptr(T), ptr(S), n)
+    if st not in (OA_OK, OA_ENOCONV):
+        raise OaError(st, "oa_ortho_transform_noalloc failed")
     return T, st
 
 

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "oa_kernels.h"
 #include "oa_step.h"
@@ -78,15 +79,50 @@ oa_status ns_finish(const oa::StepDev* res, hipStream_t st, int* iters) {
   return it > 0 ? OA_OK : OA_ENOCONV;
 }
 
+// get_iter_Tmat's own limits (cp_lowdin.C:212)
+constexpr int kOrthoMaxIter = 10;
+constexpr double kOrthoTol = 1e-15;
+
+// ---- allocation-free variant: the work space, S and T live in a static
+// device array sized for n <= OA_ORTHO_NOALLOC_MAX_N (14.7 MB per device,
+// held until the process ends).  Nothing in it has to start at zero: the
+// iteration writes X0, T0, every error partial and the result record before
+// reading them.
+constexpr size_t kNoAllocMat =
+    align256((size_t)OA_ORTHO_NOALLOC_MAX_N * OA_ORTHO_NOALLOC_MAX_N * sizeof(double));
+constexpr size_t kScratchBytes =
+    ns_work(OA_ORTHO_NOALLOC_MAX_N, kOrthoMaxIter).total + 2 * kNoAllocMat;
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Per device: the address of that device's copy of `scratch` (the array's
+// name is a host-side stand-in, never passed to a kernel), a stream created
+// once, and a lock that serialises this process's calls on the device.
+struct ScratchDev {
+  std::once_flag once;
+  hipError_t rc = hipSuccess;
+  unsigned char* ptr = nullptr;
+  hipStream_t st = nullptr;
+  std::mutex mu;
+};
+constexpr int kMaxDevices = 64;
+ScratchDev g_scratch[kMaxDevices];
+
+// set-up, once per device with that device current
+void scratch_init(ScratchDev& d) {
+  d.rc = hipGetSymbolAddress(reinterpret_cast<void**>(&d.ptr), HIP_SYMBOL(scratch));
+  if (d.rc == hipSuccess && !d.ptr) d.rc = hipErrorInvalidSymbol;
+  if (d.rc == hipSuccess) d.rc = hipStreamCreateWithFlags(&d.st, hipStreamNonBlocking);
+}
+
 }  // namespace
 
 extern "C" {
 
 oa_status oa_ortho_transform(double* T, const double* S, int n) {
   if (!T || !S || n < 1) return OA_EINVAL;
-  // get_iter_Tmat's own limits (cp_lowdin.C:212)
-  const int max_iter = 10;
-  const double tol = 1e-15;
+  const int max_iter = kOrthoMaxIter;
+  const double tol = kOrthoTol;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return OA_EHIP;
   const NsWork w = ns_work(n, max_iter);
@@ -117,6 +153,31 @@ oa_status oa_ortho_transform(double* T, const double* S, int n) {
   (void)hipFree(work);
   (void)hipFree(dS);
   (void)hipFree(dT);
+  return s;
+}
+
+oa_status oa_ortho_transform_noalloc(double* T, const double* S, int n) {
+  if (!T || !S || n < 1 || n > OA_ORTHO_NOALLOC_MAX_N) return OA_EINVAL;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return OA_EHIP;
+  if (dev < 0 || dev >= kMaxDevices) return OA_EINVAL;
+  ScratchDev& d = g_scratch[dev];
+  std::call_once(d.once, scratch_init, std::ref(d));
+  if (d.rc != hipSuccess) return OA_EHIP;
+  std::lock_guard<std::mutex> lk(d.mu);
+  const NsWork w = ns_work(n, kOrthoMaxIter);
+  const size_t nn = (size_t)n * n * sizeof(double);
+  double* dS = reinterpret_cast<double*>(d.ptr + w.total);
+  double* dT = reinterpret_cast<double*>(d.ptr + w.total + kNoAllocMat);
+  if (hipMemcpyAsync(dS, S, nn, hipMemcpyHostToDevice, d.st) != hipSuccess) return OA_EHIP;
+  oa::NsArgs a = ns_args(dS, dT, d.ptr, n, kOrthoMaxIter, kOrthoTol);
+  a.coop = ns_kind(n, dev);
+  if (oa::ns_run(a, d.st) != hipSuccess) return OA_EHIP;
+  oa_status s = ns_finish(a.res, d.st, nullptr);
+  if ((s == OA_OK || s == OA_ENOCONV) &&
+      (hipMemcpyAsync(T, dT, nn, hipMemcpyDeviceToHost, d.st) != hipSuccess ||
+       hipStreamSynchronize(d.st) != hipSuccess))
+    s = OA_EHIP;
   return s;
 }
 
