@@ -211,6 +211,14 @@ oa_status oa_vks_set(oa_ctx *ctx, const double *v);
  * and the density reduction (CP_State_RealSpacePlane.C:349-355, 450-517;
  * CP_Rho_RealSpacePlane.C:398-447, 630-650). */
 oa_status oa_fft_g2r(oa_ctx *ctx);
+/* The same for the states [s0, s0 + ns) only, the granularity of the
+ * reference's per-state-plane calls doStpFFTGtoR_Gchare / _Rchare
+ * (fftCacheSlab.h:397-402) and of each state's contribution to the density
+ * reduction (CP_State_RealSpacePlane.C:450-517): rho = (first ? 0 : rho) +
+ * the range's share.  A caller that covers all states in ranges (first = 1
+ * on the first) holds the density of oa_fft_g2r; with vks_from_rho it then
+ * builds V with oa_vks_build.  Fused plane engine only (else OA_EINVAL). */
+oa_status oa_fft_g2r_states(oa_ctx *ctx, int s0, int ns, int first);
 /* psi_s(r) of one state (RealStateSlab plane data) */
 oa_status oa_psir_get(oa_ctx *ctx, int state, double *psir);
 /* rho(r) [Nz][Ny][Nx] */
@@ -222,6 +230,11 @@ oa_status oa_density_get(oa_ctx *ctx, double *rho);
  * GStateSlab::addForces (stateSlab.C:240-272) and CPNONLOCAL::CP_eke_calc
  * (cp_eke.C:30-140).  *eke may be NULL. */
 oa_status oa_vpsi_r2g(oa_ctx *ctx, double *eke);
+/* The same for the states [s0, s0 + ns) (doStpFFTRtoG_Rchare / _Gchare per
+ * state plane, fftCacheSlab.h:397-402), after oa_fft_g2r or
+ * oa_fft_g2r_states covered them; writes their forces, *eke is their share
+ * of the kinetic energy.  Fused plane engine only (else OA_EINVAL). */
+oa_status oa_vpsi_r2g_states(oa_ctx *ctx, int s0, int ns, double *eke);
 oa_status oa_force_get(oa_ctx *ctx, double *f);
 oa_status oa_force_set(oa_ctx *ctx, const double *f);
 

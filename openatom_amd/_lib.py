@@ -21,7 +21,8 @@ EXPORTS = (
     "oa_default_options", "oa_kvecs", "oa_create", "oa_destroy", "oa_last_error",
     "oa_layout_info", "oa_ctx_device_bytes", "oa_get_config", "oa_get_kvecs", "oa_psi_set",
     "oa_psi_get", "oa_vks_set",
-    "oa_fft_g2r", "oa_psir_get", "oa_density_get", "oa_vpsi_r2g", "oa_force_get",
+    "oa_fft_g2r", "oa_fft_g2r_states", "oa_psir_get", "oa_density_get", "oa_vpsi_r2g",
+    "oa_vpsi_r2g_states", "oa_force_get",
     "oa_force_set", "oa_lambda", "oa_sd_update", "oa_overlap", "oa_ortho_ns",
     "oa_tmat_set", "oa_ortho_transform", "oa_ortho_transform_noalloc", "oa_rotate", "oa_step", "oa_residual", "oa_sync", "oa_step_result", "oa_timings",
     "oa_set_timing", "oa_local_pseudo_set", "oa_vks_build", "oa_vks_get", "oa_tuning_log",
@@ -118,6 +119,7 @@ def load():
                        ("oa_overlap", [vp, vp]),
                        ("oa_ortho_ns", [vp, cd, ci, vp, vp, vp]),
                        ("oa_tmat_set", [vp, vp]), ("oa_rotate", [vp]),
+                       ("oa_fft_g2r_states", [vp, ci, ci, ci]), ("oa_vpsi_r2g_states", [vp, ci, ci, vp]),
                        ("oa_ortho_transform", [vp, vp, ci]),
                        ("oa_ortho_transform_noalloc", [vp, vp, ci]),
                        ("oa_step", [vp, vp]), ("oa_residual", [vp, vp, vp]), ("oa_sync", [vp]),

@@ -538,14 +538,14 @@ __global__ __launch_bounds__(NTH) void k_plane_bwd(Geom g, FftPlan px, FftPlan p
   }
 }
 
-// rho = (1/Omega) sum_chunks rho_part; per-workgroup partials of sum V rho.
-// rho = sum of the chunk partials / Omega, and the partials of sum V rho
-// when vks is set.  Two points per thread and round, all their loads issued
+// rho = sum of the chunk partials / Omega (added to rho when accumulate is
+// set), and the partials of sum V rho when vks is set.  Two points per thread and round, all their loads issued
 // before any is used (the partials are read once: latency, not bandwidth,
 // bounds a plain loop).
 __global__ void k_rho_reduce(const double* __restrict__ part, int nchunk, size_t n,
                              double inv_vol, const double* __restrict__ vks,
-                             double* __restrict__ rho, double* __restrict__ dpart) {
+                             double* __restrict__ rho, double* __restrict__ dpart,
+                             int accumulate) {
   __shared__ double red[16];
   double e = 0.0;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -572,6 +572,10 @@ __global__ void k_rho_reduce(const double* __restrict__ part, int nchunk, size_t
     }
     a0 *= inv_vol;
     a1 *= inv_vol;
+    if (accumulate) {
+      a0 += rho[r0];
+      a1 += rho[r1c];
+    }
     rho[r0] = a0;
     if (r1 < n) rho[r1] = a1;
     if (vks) {
@@ -648,10 +652,11 @@ hipError_t plane_bwd(const FftCtx& c, bool fp32, hipStream_t st) {
   return fp32 ? launch_plane_sel<float>(c, false, st) : launch_plane_sel<double>(c, false, st);
 }
 
-hipError_t rho_reduce(const FftCtx& c, double* dpart, int ndpart, hipStream_t st, bool with_v) {
+hipError_t rho_reduce(const FftCtx& c, double* dpart, int ndpart, hipStream_t st, bool with_v,
+                      bool accumulate) {
   const size_t n = (size_t)c.g.nx * c.g.ny * c.g.nz;
   hipLaunchKernelGGL(k_rho_reduce, dim3(ndpart), dim3(256), 0, st, c.rho_part, c.nchunk, n,
-                     c.inv_vol, with_v ? c.vks : nullptr, c.rho, dpart);
+                     c.inv_vol, with_v ? c.vks : nullptr, c.rho, dpart, accumulate ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -70,10 +70,10 @@ hipError_t plane3_info(int n, int nplane_x, bool fp32, int* rowparts, int* occ_g
 // fused y+x plane passes (one z-plane, a chunk of states per workgroup)
 hipError_t plane_fwd(const FftCtx& c, bool fp32, hipStream_t st);
 hipError_t plane_bwd(const FftCtx& c, bool fp32, hipStream_t st);
-// rho = sum of partials / Omega, and partial sums of V rho into dpart[ndpart]
-// rho = sum of the chunk partials / Omega; with_v: also partials of sum V rho
+// rho = sum of the chunk partials / Omega (accumulate: rho += that);
+// with_v: also partial sums of V rho into dpart[ndpart]
 hipError_t rho_reduce(const FftCtx& c, double* dpart, int ndpart, hipStream_t st,
-                      bool with_v = true);
+                      bool with_v = true, bool accumulate = false);
 size_t plane_smem(const FftCtx& c, bool fp32);
 int plane_config_for(int plane_points);  // -1 if the plane is too large
 int plane_threads(int cfg);

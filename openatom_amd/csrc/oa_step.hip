@@ -732,6 +732,34 @@ FftCtx& fctx(oa_ctx* c) {
   return c->fc;
 }
 
+// The plane engine's view of the states [s0, s0 + ns): every per-state array
+// offset to state s0, the state chunks of both plane passes recounted for ns
+// states (the rho partial buffer holds the full count's chunks).
+FftCtx range_ctx(oa_ctx* c, int s0, int ns) {
+  FftCtx f = fctx(c);
+  const size_t zst = (size_t)c->nz * c->nplane_x * c->ny * (c->fp32 ? 8 : 16);
+  f.g.nst = ns;
+  f.psi = c->psi + (size_t)s0 * c->ld;
+  f.force = c->force + (size_t)s0 * c->ld;
+  f.zbuf = static_cast<unsigned char*>(c->zbuf) + (size_t)s0 * zst;
+  f.zout = static_cast<unsigned char*>(c->zout) + (size_t)s0 * zst;
+  f.occ = c->occ + s0;
+  f.eke_part = c->eke_part + (size_t)s0 * c->nzblk;
+  f.nchunk = (ns + f.cs - 1) / f.cs;
+  f.nchunk_r = (ns + f.cs_r - 1) / f.cs_r;
+  return f;
+}
+
+oa_status check_range(oa_ctx* c, int s0, int ns) {
+  if (s0 < 0 || ns < 1 || s0 > c->nst - ns)
+    return fail(c, OA_EINVAL, "state range [s0, s0 + ns) outside [0, nstates)");
+  if (c->engine != kEngPlane)
+    return fail(c, OA_EINVAL,
+                "state ranges need the fused plane engine (fft_mode 0 or 4, square x-y planes "
+                "of a compiled length)");
+  return OA_OK;
+}
+
 // Timing marks inside oa_step: ev[0] start, ev[1..6] after each FFT stage,
 // ev[7] reductions + Lambda, ev[8] SD + sync, ev[9] S, ev[10] NS,
 // ev[11] Psi T, ev[12] residual (end).
@@ -1680,6 +1708,18 @@ oa_status oa_fft_g2r(oa_ctx* c) {
   return OA_OK;
 }
 
+oa_status oa_fft_g2r_states(oa_ctx* c, int s0, int ns, int first) {
+  if (!c) return OA_EINVAL;
+  TRY(check_range(c, s0, ns));
+  HIPCHK(hipSetDevice(c->device));
+  FftCtx f = range_ctx(c, s0, ns);
+  HIPCHK(oa::fft2_launch(oa::FFT_ZFWD, c->nz, c->fp32, f, c->stream));
+  HIPCHK(oa::plane3_launch(0, c->nx, c->fp32, f, c->stream));
+  HIPCHK(oa::rho_reduce(f, c->dpart, c->ndpart, c->stream, false, first == 0));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return OA_OK;
+}
+
 oa_status oa_psir_get(oa_ctx* c, int s, double* out) {
   if (!c || !out || s < 0 || s >= c->nst) return OA_EINVAL;
   HIPCHK(hipSetDevice(c->device));
@@ -1717,6 +1757,20 @@ oa_status oa_vpsi_r2g(oa_ctx* c, double* eke) {
   if (!c) return OA_EINVAL;
   HIPCHK(hipSetDevice(c->device));
   TRY(st_r2g(c));
+  TRY(fetch_result(c, nullptr));
+  if (eke) *eke = c->res_host->eke;
+  return OA_OK;
+}
+
+oa_status oa_vpsi_r2g_states(oa_ctx* c, int s0, int ns, double* eke) {
+  if (!c) return OA_EINVAL;
+  TRY(check_range(c, s0, ns));
+  HIPCHK(hipSetDevice(c->device));
+  FftCtx f = range_ctx(c, s0, ns);
+  HIPCHK(oa::plane3_launch(1, c->nx, c->fp32, f, c->stream));
+  f.zbuf = f.zout;  // the z pass back reads the R->G plane output
+  HIPCHK(oa::fft2_launch(oa::FFT_ZBWD, c->nz, c->fp32, f, c->stream));
+  HIPCHK(oa::reduce_to(f.eke_part, ns * c->nzblk, 1.0, &c->res->eke, c->stream));
   TRY(fetch_result(c, nullptr));
   if (eke) *eke = c->res_host->eke;
   return OA_OK;

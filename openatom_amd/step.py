@@ -207,6 +207,17 @@ class CPContext:
     def fft_g2r(self):
         self._chk(self._L.oa_fft_g2r(self.h))
 
+    def fft_g2r_states(self, s0, ns, first=True):
+        """G->R and the density share of the states [s0, s0 + ns); rho is
+        replaced (first) or added to."""
+        self._chk(self._L.oa_fft_g2r_states(self.h, int(s0), int(ns), 1 if first else 0))
+
+    def vpsi_r2g_states(self, s0, ns):
+        """V psi + R->G for the states [s0, s0 + ns); returns their EKE."""
+        e = ctypes.c_double(0.0)
+        self._chk(self._L.oa_vpsi_r2g_states(self.h, int(s0), int(ns), ctypes.byref(e)))
+        return e.value
+
     def vpsi_r2g(self):
         e = ctypes.c_double()
         self._chk(self._L.oa_vpsi_r2g(self.h, ctypes.byref(e)))
