@@ -21,7 +21,25 @@
 //
 // Template parameters: transform lengths Nx = X1*X2 and Ny = Y1*Y2 (the x
 // pass-2 writes X2 outputs per thread, so the smaller factor goes second),
-// NT threads.  P column stride CY = Ny | 1.
+// NT threads, and the LDS layout L (below).
+//
+// LDS banks.  gfx950 LDS has 32 banks of 4 bytes: an 8-byte access (complex
+// float) is served 16 lanes per cycle, a 16-byte access (complex double) 8
+// lanes per cycle, and lanes of one group that hit the same bank at different
+// addresses are serialised (measured: element strides 2^k cost min(2^k, 16)
+// cycles, odd strides one).  A transform's G = max(R1, R2) threads do not
+// fill a lane group, so a group mixes two or three transforms.  Three choices
+// keep the accesses apart:
+//   - the pass-2 twiddles W_N^{t j} come from tables indexed [t][j], so the
+//     lanes of one transform read consecutive entries (the old table indexed
+//     by t*j put them t entries apart);
+//   - each phase picks its lane map: j-minor (thread = transform * G + j)
+//     where its global loads or stores must run along y, transform-minor
+//     (thread = j * NTR + transform) where only LDS is touched, so that the
+//     lanes of a group are different transforms at the same j and differ by
+//     the row or column stride;
+//   - P columns and E rows have odd strides (in elements), and pass-1 output
+//     blocks may be padded.
 #pragma once
 
 #include "oa_fft2.h"
@@ -36,7 +54,24 @@ using oa2::twiddle;
 
 __device__ __forceinline__ int signed_k(int i, int n) { return (i <= (n - 1) / 2) ? i : i - n; }
 
-template <int X1, int X2, int Y1, int Y2, int NT>
+// LDS layout and lane maps of the plane passes.
+//   CYP : elements added to each P column beyond its padded length
+//   YA  : P columns hold YA empty elements after every Y1 rows
+//   EA  : exchange rows hold EA empty elements after every pass-1 output block
+//   REP : elements added to each exchange row beyond its padded length
+//   MAPS: transform-minor lane maps, bits: 1 y_fwd pass 2, 2 x c2r pass 1,
+//         4 x c2r pass 2 (and what follows it in registers: rho, V psi and
+//         the r2c pass 1), 8 r2c pass 2, 16 y_bwd pass 1
+//   TW2 : twiddle tables indexed [t][j] (else one table indexed t*j)
+template <int CYP_, int YA_, int EA_, int REP_, int MAPS_, bool TW2_>
+struct Lay {
+  static constexpr int CYP = CYP_, YA = YA_, EA = EA_, REP = REP_, MAPS = MAPS_;
+  static constexpr bool TW2 = TW2_;
+};
+
+enum : int { kMapYf2 = 1, kMapXc1 = 2, kMapXc2 = 4, kMapRx2 = 8, kMapYb1 = 16 };
+
+template <int X1, int X2, int Y1, int Y2, int NT, class L>
 struct PS {
   static constexpr int NX = X1 * X2, NY = Y1 * Y2;
   static constexpr int GX = X1 > X2 ? X1 : X2;  // threads per x transform
@@ -45,47 +80,94 @@ struct PS {
   static constexpr int NTRY = NT / GY;          // columns per y batch
   static constexpr int NPAIR = (NY + 1) / 2;
   static constexpr int ROUNDS = (NPAIR + NTRX - 1) / NTRX;
-  static constexpr int CY = NY | 1;
-  static constexpr int EX = NTRX * NX > NTRY * NY ? NTRX * NX : NTRY * NY;  // E slots
-  // pass-2 twiddles W_N^m, m = t j <= (X1 - 1)(X2 - 1), in LDS (planes are
-  // square, so one table serves the x and the y transforms)
-  static constexpr int NTW = (X1 - 1) * (X2 - 1) + 1;
-  static_assert(NX == NY, "the fused plane passes need square x-y planes");
+  static constexpr int XMIN = X1 < X2 ? X1 : X2;
+  // P column stride and exchange row strides, in elements
+  static constexpr int CY = NY + L::YA * ((NY - 1) / Y1) + L::CYP;
+  static constexpr int RE = NX + L::EA * ((NX - 1) / XMIN) + L::REP;
+  static constexpr int RY = NY + L::EA * ((NY - 1) / Y1) + L::REP;
+  static constexpr int EX = NTRX * RE > NTRY * RY ? NTRX * RE : NTRY * RY;  // E slots
+  // twiddles: [t][j] tables A (t < X1 = Y2, j < X2 = Y1) and B (t < X2,
+  // j < X1; the same table when X1 == X2), or one table by m = t j
+  static constexpr int NTWA = X1 * X2;
+  static constexpr int NTWB = X1 == X2 ? 0 : X1 * X2;
+  static constexpr int NTW1 = (X1 - 1) * (X2 - 1) + 1;
+  static constexpr int NTW = L::TW2 ? NTWA + NTWB : NTW1;
+  static_assert(NX == NY && X1 == Y2 && X2 == Y1,
+                "the fused plane passes need square x-y planes, y as R1 x R2, x as R2 x R1");
 };
 
-// pass-2 twiddle W_N^m from the LDS table, conjugated for S = +1
-template <typename T, int S>
-__device__ __forceinline__ cpx<T> twl(const cpx<T>* tw, int m) {
-  const cpx<T> w = tw[m];
+// P element of row y in a column (YA padding per Y1 rows)
+template <int Y1, class L>
+__device__ __forceinline__ int py(int y) {
+  return L::YA ? y + L::YA * (y / Y1) : y;
+}
+// exchange element of position p in a row whose pass-1 blocks are R long
+template <int R, class L>
+__device__ __forceinline__ int pe(int p) {
+  return L::EA ? p + L::EA * (p / R) : p;
+}
+
+// lane map: transform tr and thread j within it
+template <bool TMIN, int G, int NTR>
+__device__ __forceinline__ void lane_map(int tid, int& tr, int& j) {
+  if (TMIN) {
+    j = tid / NTR;
+    tr = tid - j * NTR;
+  } else {
+    tr = tid / G;
+    j = tid - tr * G;
+  }
+}
+
+// pass-2 twiddle W_N^{t j}, conjugated for S = +1.  Table A: t < X1, j < X2
+// (y passes, r2c pass 2); table B: t < X2, j < X1 (c2r pass 2).
+template <typename T, int S, int RJ, class L>
+__device__ __forceinline__ cpx<T> tw2(const cpx<T>* tw, int t, int j) {
+  const cpx<T> w = L::TW2 ? tw[t * RJ + j] : tw[t * j];
   return {w.x, S < 0 ? w.y : -w.y};
 }
 
-// copy the pass-2 twiddles and the per-column ky ranges into LDS (callers
+// copy the twiddles and the per-column ky ranges into LDS (callers
 // synchronise before use); the ranges are read for every state, and a
 // global load there would cost a memory latency per state
-template <typename T, int NTW, int NT>
+template <typename T, int X1, int X2, int NT, class L>
 __device__ __forceinline__ void load_twl(cpx<T>* tw, const double2* __restrict__ g, int2* ky,
                                          const FftCtx& c, int npx) {
-  for (int i = threadIdx.x; i < NTW; i += NT) tw[i] = {(T)g[i].x, (T)g[i].y};
+  if (L::TW2) {
+    for (int i = threadIdx.x; i < X1 * X2; i += NT) {
+      const int t = i / X2, j = i - t * X2;  // A[t][j], t < X1
+      tw[i] = {(T)g[t * j].x, (T)g[t * j].y};
+      if (X1 != X2) {
+        const int tb = i / X1, jb = i - tb * X1;  // B[t][j], t < X2
+        tw[X1 * X2 + i] = {(T)g[tb * jb].x, (T)g[tb * jb].y};
+      }
+    }
+  } else {
+    for (int i = threadIdx.x; i < (X1 - 1) * (X2 - 1) + 1; i += NT)
+      tw[i] = {(T)g[i].x, (T)g[i].y};
+  }
   for (int i = threadIdx.x; i < npx; i += NT) ky[i] = make_int2(c.kylo[i], c.kyhi[i]);
 }
 
 // LDS bytes of either kernel: P (nplane_x columns) + E + twiddles + ky ranges
-template <typename T, int X1, int X2, int Y1, int Y2, int NT>
-size_t plane3_smem(int npx) {
-  using S = PS<X1, X2, Y1, Y2, NT>;
+template <typename T, int X1, int X2, int Y1, int Y2, int NT, class L>
+size_t plane3_smem_l(int npx) {
+  using S = PS<X1, X2, Y1, Y2, NT, L>;
   return sizeof(cpx<T>) * ((size_t)npx * S::CY + S::EX + S::NTW) + sizeof(int2) * npx;
 }
 
 // y-FFT (FFTW_FORWARD) of the populated columns of one z-plane into P.
 // zs: zbuf plane [kx][NY]; only ky in KY[kx] = [kylo, kyhi] of each column
-// is read.
-template <typename T, int Y1, int Y2, int NT>
+// is read.  Pass 1 runs j-minor (loads along y), pass 2 in place.
+template <typename T, int Y1, int Y2, int NT, class L>
 __device__ __forceinline__ void y_fwd(const cpx<T>* __restrict__ zs, cpx<T>* P, const cpx<T>* TW,
                                       const int2* KY, int npx) {
-  constexpr int NY = Y1 * Y2, GY = Y1 > Y2 ? Y1 : Y2, NTRY = NT / GY, CY = NY | 1;
+  using S = PS<Y2, Y1, Y1, Y2, NT, L>;
+  constexpr int NY = S::NY, GY = S::GY, NTRY = S::NTRY, CY = S::CY;
   const int tid = threadIdx.x;
-  const int tr = tid / GY, j = tid - tr * GY;
+  int tr, j, tr2, j2;
+  lane_map<false, GY, NTRY>(tid, tr, j);
+  lane_map<(L::MAPS & kMapYf2) != 0, GY, NTRY>(tid, tr2, j2);
   for (int cb = 0; cb < npx; cb += NTRY) {
     const int kx = cb + tr;
     if (tr < NTRY && kx < npx && j < Y2) {
@@ -99,54 +181,64 @@ __device__ __forceinline__ void y_fwd(const cpx<T>* __restrict__ zs, cpx<T>* P, 
         v[t] = (kyv >= lo && kyv <= hi) ? src[y] : cpx<T>{(T)0, (T)0};
       }
       Dft<T, Y1, -1>::run(v);
+      cpx<T>* dst = P + kx * CY + j * (Y1 + L::YA);
 #pragma unroll
-      for (int t = 0; t < Y1; t++) P[kx * CY + j * Y1 + t] = v[t];
+      for (int t = 0; t < Y1; t++) dst[t] = v[t];
     }
   }
   __syncthreads();
-  // pass 2 in place: thread j reads and writes the slots j + t*Y1 of its column
+  // pass 2 in place: thread j reads and writes the rows j + t*Y1 of its column
   for (int cb = 0; cb < npx; cb += NTRY) {
-    const int kx = cb + tr;
-    if (tr < NTRY && kx < npx && j < Y1) {
+    const int kx = cb + tr2;
+    if (tr2 < NTRY && kx < npx && j2 < Y1) {
+      cpx<T>* col = P + kx * CY + j2;
       cpx<T> u[Y2];
 #pragma unroll
-      for (int t = 0; t < Y2; t++) u[t] = P[kx * CY + j + t * Y1];
+      for (int t = 0; t < Y2; t++) u[t] = col[t * (Y1 + L::YA)];
 #pragma unroll
-      for (int t = 1; t < Y2; t++) u[t] = cm(u[t], twl<T, -1>(TW, t * j));
+      for (int t = 1; t < Y2; t++) u[t] = cm(u[t], tw2<T, -1, Y1, L>(TW, t, j2));
       Dft<T, Y2, -1>::run(u);
 #pragma unroll
-      for (int t = 0; t < Y2; t++) P[kx * CY + j + t * Y1] = u[t];
+      for (int t = 0; t < Y2; t++) col[t * (Y1 + L::YA)] = u[t];
     }
   }
   __syncthreads();
 }
 
 // y-FFT (FFTW_BACKWARD) of the nplane_x columns of P into the zout plane,
-// keeping ky in KY[kx]; E is the exchange buffer.
-template <typename T, int Y1, int Y2, int NT>
+// keeping ky in KY[kx]; E is the exchange buffer.  Pass 2 runs j-minor
+// (stores along y).
+template <typename T, int Y1, int Y2, int NT, class L>
 __device__ __forceinline__ void y_bwd(const cpx<T>* P, cpx<T>* E, const cpx<T>* TW,
                                       cpx<T>* __restrict__ zo, const int2* KY, int npx) {
-  constexpr int NY = Y1 * Y2, GY = Y1 > Y2 ? Y1 : Y2, NTRY = NT / GY, CY = NY | 1;
+  using S = PS<Y2, Y1, Y1, Y2, NT, L>;
+  constexpr int NY = S::NY, GY = S::GY, NTRY = S::NTRY, CY = S::CY, RY = S::RY;
   const int tid = threadIdx.x;
-  const int tr = tid / GY, j = tid - tr * GY;
+  int tr1, j1, tr, j;
+  lane_map<(L::MAPS & kMapYb1) != 0, GY, NTRY>(tid, tr1, j1);
+  lane_map<false, GY, NTRY>(tid, tr, j);
   for (int cb = 0; cb < npx; cb += NTRY) {
-    const int kx = cb + tr;
-    const bool act = tr < NTRY && kx < npx;
-    if (act && j < Y2) {
-      cpx<T> v[Y1];
+    {
+      const int kx = cb + tr1;
+      if (tr1 < NTRY && kx < npx && j1 < Y2) {
+        cpx<T> v[Y1];
 #pragma unroll
-      for (int t = 0; t < Y1; t++) v[t] = P[kx * CY + j + t * Y2];
-      Dft<T, Y1, +1>::run(v);
+        for (int t = 0; t < Y1; t++) v[t] = P[kx * CY + py<Y1, L>(j1 + t * Y2)];
+        Dft<T, Y1, +1>::run(v);
+        cpx<T>* dst = E + tr1 * RY + j1 * (Y1 + L::EA);
 #pragma unroll
-      for (int t = 0; t < Y1; t++) E[tr * NY + j * Y1 + t] = v[t];
+        for (int t = 0; t < Y1; t++) dst[t] = v[t];
+      }
     }
     __syncthreads();
-    if (act && j < Y1) {
+    const int kx = cb + tr;
+    if (tr < NTRY && kx < npx && j < Y1) {
+      const cpx<T>* src = E + tr * RY + j;
       cpx<T> u[Y2];
 #pragma unroll
-      for (int t = 0; t < Y2; t++) u[t] = E[tr * NY + j + t * Y1];
+      for (int t = 0; t < Y2; t++) u[t] = src[t * (Y1 + L::EA)];
 #pragma unroll
-      for (int t = 1; t < Y2; t++) u[t] = cm(u[t], twl<T, +1>(TW, t * j));
+      for (int t = 1; t < Y2; t++) u[t] = cm(u[t], tw2<T, +1, Y1, L>(TW, t, j));
       Dft<T, Y2, +1>::run(u);
       const int lo = KY[kx].x, hi = KY[kx].y;
       cpx<T>* dst = zo + (size_t)kx * NY;
@@ -161,22 +253,44 @@ __device__ __forceinline__ void y_bwd(const cpx<T>* P, cpx<T>* E, const cpx<T>* 
   }
 }
 
+// Lane maps of one x round: pass 1 of the c2r (thread j < X2 of pair-row q)
+// and pass 2 with everything after it in registers (thread j < X1).
+struct XLanes {
+  int tr1, j1, q1;
+  bool act1;
+  int tr, j, q;
+  bool act;
+};
+
+template <int X1, int X2, int Y1, int Y2, int NT, class L>
+__device__ __forceinline__ XLanes x_lanes(int r) {
+  using S = PS<X1, X2, Y1, Y2, NT, L>;
+  XLanes x;
+  lane_map<(L::MAPS & kMapXc1) != 0, S::GX, S::NTRX>(threadIdx.x, x.tr1, x.j1);
+  lane_map<(L::MAPS & kMapXc2) != 0, S::GX, S::NTRX>(threadIdx.x, x.tr, x.j);
+  x.q1 = r * S::NTRX + x.tr1;
+  x.q = r * S::NTRX + x.tr;
+  x.act1 = x.tr1 < S::NTRX && x.q1 < S::NPAIR;
+  x.act = x.tr < S::NTRX && x.q < S::NPAIR;
+  return x;
+}
+
 // One round of Hermitian x-FFTs (c2r by the pair-row trick) from the rows of
 // P: pair q carries rows y1 = 2q (real part) and y2 = 2q+1 (imaginary part).
-// On return thread (tr, j < X1) holds psi(r) at x = j + t*X1 in u[t].
+// On return thread (x.tr, x.j < X1) holds psi(r) at x = j + t*X1 in u[t].
 // Ends after the pass-2 reads of E (callers sync before reusing E).
-template <typename T, int X1, int X2, int NT>
-__device__ __forceinline__ void x_c2r(const cpx<T>* P, cpx<T>* E, const cpx<T>* TW,
-                                      const FftCtx& c, int npx, int ny, int q, int tr, int j,
-                                      bool act, cpx<T> (&u)[X2]) {
-  constexpr int NX = X1 * X2;
-  const int CY = ny | 1;
-  const int y1 = 2 * q, y2 = y1 + 1;
-  if (act && j < X2) {
+template <typename T, int X1, int X2, int Y1, int Y2, int NT, class L>
+__device__ __forceinline__ void x_c2r(const cpx<T>* P, cpx<T>* E, const cpx<T>* TW, int npx,
+                                      const XLanes& x, cpx<T> (&u)[X2]) {
+  using S = PS<X1, X2, Y1, Y2, NT, L>;
+  constexpr int NX = S::NX, NY = S::NY, CY = S::CY, RE = S::RE;
+  if (x.act1 && x.j1 < X2) {
+    const int y1 = 2 * x.q1, y2 = y1 + 1;
+    const int o1 = py<Y1, L>(y1), o2 = py<Y1, L>(y2);
     cpx<T> v[X1];
 #pragma unroll
     for (int t = 0; t < X1; t++) {
-      const int k = j + t * X2;
+      const int k = x.j1 + t * X2;
       cpx<T> Z = {(T)0, (T)0};
       int kk = -1;
       bool mirror = false;
@@ -187,8 +301,8 @@ __device__ __forceinline__ void x_c2r(const cpx<T>* P, cpx<T>* E, const cpx<T>* 
         mirror = true;
       }
       if (kk >= 0) {
-        const cpx<T> A = P[kk * CY + y1];
-        const cpx<T> B = (y2 < ny) ? P[kk * CY + y2] : cpx<T>{(T)0, (T)0};
+        const cpx<T> A = P[kk * CY + o1];
+        const cpx<T> B = (y2 < NY) ? P[kk * CY + o2] : cpx<T>{(T)0, (T)0};
         if (kk == 0) Z = {A.x, B.x};
         else if (!mirror) Z = {A.x - B.y, A.y + B.x};
         else Z = {A.x + B.y, B.x - A.y};
@@ -196,15 +310,17 @@ __device__ __forceinline__ void x_c2r(const cpx<T>* P, cpx<T>* E, const cpx<T>* 
       v[t] = Z;
     }
     Dft<T, X1, -1>::run(v);
+    cpx<T>* dst = E + x.tr1 * RE + x.j1 * (X1 + L::EA);
 #pragma unroll
-    for (int t = 0; t < X1; t++) E[tr * NX + j * X1 + t] = v[t];
+    for (int t = 0; t < X1; t++) dst[t] = v[t];
   }
   __syncthreads();
-  if (act && j < X1) {
+  if (x.act && x.j < X1) {
+    const cpx<T>* src = E + x.tr * RE + x.j;
 #pragma unroll
-    for (int t = 0; t < X2; t++) u[t] = E[tr * NX + j + t * X1];
+    for (int t = 0; t < X2; t++) u[t] = src[t * (X1 + L::EA)];
 #pragma unroll
-    for (int t = 1; t < X2; t++) u[t] = cm(u[t], twl<T, -1>(TW, t * j));
+    for (int t = 1; t < X2; t++) u[t] = cm(u[t], tw2<T, -1, X1, L>(TW + S::NTWA * (X1 != X2 && L::TW2), t, x.j));
     Dft<T, X2, -1>::run(u);
   }
 }
@@ -212,10 +328,10 @@ __device__ __forceinline__ void x_c2r(const cpx<T>* P, cpx<T>* E, const cpx<T>* 
 // G->R plane pass.  Grid (nz, row parts, state chunks).  Row part rp covers
 // x rounds [rp*RPW, rp*RPW + RPW).  PSIR: write psi(r) of state
 // c.psir_state into c.psir ([Nz][Ny][Nx] of T) instead of rho partials.
-template <typename T, int X1, int X2, int Y1, int Y2, int NT, int RPW, bool PSIR>
+template <typename T, int X1, int X2, int Y1, int Y2, int NT, int RPW, bool PSIR, class L>
 __global__ __launch_bounds__(NT) void k3_g2r(FftCtx c) {
-  using S = PS<X1, X2, Y1, Y2, NT>;
-  constexpr int NX = S::NX, NY = S::NY, GX = S::GX, NTRX = S::NTRX;
+  using S = PS<X1, X2, Y1, Y2, NT, L>;
+  constexpr int NX = S::NX, NY = S::NY;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int npx = c.g.nplane_x;
   cpx<T>* P = reinterpret_cast<cpx<T>*>(smem_raw);
@@ -232,11 +348,9 @@ __global__ __launch_bounds__(NT) void k3_g2r(FftCtx c) {
     st0 = chunk * c.cs;
     st1 = min(c.g.nst, st0 + c.cs);
   }
-  const int tid = threadIdx.x;
-  const int tr = tid / GX, j = tid - tr * GX;
   const size_t plane = (size_t)npx * NY;
   const size_t NP = (size_t)NX * NY;
-  load_twl<T, S::NTW, NT>(TW, c.twx, KY, c, npx);
+  load_twl<T, X1, X2, NT, L>(TW, c.twx, KY, c, npx);
   __syncthreads();
   double acc[RPW][2 * X2];
 #pragma unroll
@@ -245,24 +359,23 @@ __global__ __launch_bounds__(NT) void k3_g2r(FftCtx c) {
     for (int b = 0; b < 2 * X2; b++) acc[a][b] = 0.0;
   for (int s = st0; s < st1; s++) {
     const cpx<T>* zs = (const cpx<T>*)c.zbuf + ((size_t)s * c.g.nz + z) * plane;
-    y_fwd<T, Y1, Y2, NT>(zs, P, TW, KY, npx);
+    y_fwd<T, Y1, Y2, NT, L>(zs, P, TW, KY, npx);
     const double oc = c.occ[s];
 #pragma unroll
     for (int rr = 0; rr < RPW; rr++) {
       const int r = r0 + rr;
       if (r < S::ROUNDS) {
-        const int q = r * NTRX + tr;
-        const bool act = tr < NTRX && q < S::NPAIR;
+        const XLanes x = x_lanes<X1, X2, Y1, Y2, NT, L>(r);
         cpx<T> u[X2];
-        x_c2r<T, X1, X2, NT>(P, E, TW, c, npx, NY, q, tr, j, act, u);
-        if (act && j < X1) {
-          const int y1 = 2 * q, y2 = y1 + 1;
+        x_c2r<T, X1, X2, Y1, Y2, NT, L>(P, E, TW, npx, x, u);
+        if (x.act && x.j < X1) {
+          const int y1 = 2 * x.q, y2 = y1 + 1;
           if (PSIR) {
             T* pr = (T*)c.psir + (size_t)z * NP;
 #pragma unroll
             for (int t = 0; t < X2; t++) {
-              pr[(size_t)y1 * NX + j + t * X1] = u[t].x;
-              if (y2 < NY) pr[(size_t)y2 * NX + j + t * X1] = u[t].y;
+              pr[(size_t)y1 * NX + x.j + t * X1] = u[t].x;
+              if (y2 < NY) pr[(size_t)y2 * NX + x.j + t * X1] = u[t].y;
             }
           } else {
 #pragma unroll
@@ -281,13 +394,13 @@ __global__ __launch_bounds__(NT) void k3_g2r(FftCtx c) {
 #pragma unroll
   for (int rr = 0; rr < RPW; rr++) {
     const int r = r0 + rr;
-    const int q = r * NTRX + tr;
-    if (r < S::ROUNDS && tr < NTRX && q < S::NPAIR && j < X1) {
-      const int y1 = 2 * q, y2 = y1 + 1;
+    const XLanes x = x_lanes<X1, X2, Y1, Y2, NT, L>(r);
+    if (r < S::ROUNDS && x.act && x.j < X1) {
+      const int y1 = 2 * x.q, y2 = y1 + 1;
 #pragma unroll
       for (int t = 0; t < X2; t++) {
-        rp_out[(size_t)y1 * NX + j + t * X1] = acc[rr][2 * t];
-        if (y2 < NY) rp_out[(size_t)y2 * NX + j + t * X1] = acc[rr][2 * t + 1];
+        rp_out[(size_t)y1 * NX + x.j + t * X1] = acc[rr][2 * t];
+        if (y2 < NY) rp_out[(size_t)y2 * NX + x.j + t * X1] = acc[rr][2 * t + 1];
       }
     }
   }
@@ -299,12 +412,12 @@ __global__ __launch_bounds__(NT) void k3_g2r(FftCtx c) {
 // faster than two; larger planes would spill).  RHO: also accumulate rho
 // partials of the chunk from the recomputed psi(r), for a V(r) that does
 // not depend on this step's rho (then no separate G->R pass is needed).
-template <typename T, int X1, int X2, int Y1, int Y2, int NT, bool VREG, bool RHO>
+template <typename T, int X1, int X2, int Y1, int Y2, int NT, bool VREG, bool RHO, class L>
 __global__ __launch_bounds__(NT, (sizeof(T) == 4 && !RHO) ? (X1 * X2 <= 100 ? 3 : 2) : 1) void
 k3_r2g(FftCtx c) {
-  using S = PS<X1, X2, Y1, Y2, NT>;
-  constexpr int NX = S::NX, NY = S::NY, GX = S::GX, NTRX = S::NTRX, R = S::ROUNDS;
-  constexpr int CY = S::CY;
+  using S = PS<X1, X2, Y1, Y2, NT, L>;
+  constexpr int NX = S::NX, NY = S::NY, NTRX = S::NTRX, R = S::ROUNDS;
+  constexpr int CY = S::CY, RE = S::RE;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int npx = c.g.nplane_x;
   cpx<T>* P = reinterpret_cast<cpx<T>*>(smem_raw);
@@ -314,11 +427,10 @@ k3_r2g(FftCtx c) {
   const int z = blockIdx.x, chunk = blockIdx.y;
   const int st0 = chunk * c.cs_r, st1 = min(c.g.nst, st0 + c.cs_r);
   const int tid = threadIdx.x;
-  const int tr = tid / GX, j = tid - tr * GX;
   const size_t plane = (size_t)npx * NY;
   const size_t NP = (size_t)NX * NY;
   const double* vz = c.vks + (size_t)z * NP;
-  load_twl<T, S::NTW, NT>(TW, c.twx, KY, c, npx);
+  load_twl<T, X1, X2, NT, L>(TW, c.twx, KY, c, npx);
   __syncthreads();
   T vr[VREG ? R : 1][VREG ? 2 * X2 : 1];
   double acc[RHO ? R : 1][RHO ? 2 * X2 : 1];
@@ -334,42 +446,43 @@ k3_r2g(FftCtx c) {
   if (VREG) {
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      const int q = r * NTRX + tr;
-      const bool act = tr < NTRX && q < S::NPAIR && j < X1;
-      const int ya = 2 * min(q, S::NPAIR - 1), yb = min(ya + 1, NY - 1);
+      const XLanes x = x_lanes<X1, X2, Y1, Y2, NT, L>(r);
+      const bool act = x.act && x.j < X1;
+      const int ya = 2 * min(x.q, S::NPAIR - 1), yb = min(ya + 1, NY - 1);
 #pragma unroll
       for (int t = 0; t < X2; t++) {
-        const int x = min(j + t * X1, NX - 1);
-        const T a = (T)vz[(size_t)ya * NX + x], b = (T)vz[(size_t)yb * NX + x];
+        const int xx = min(x.j + t * X1, NX - 1);
+        const T a = (T)vz[(size_t)ya * NX + xx], b = (T)vz[(size_t)yb * NX + xx];
         vr[r][2 * t] = act ? a : (T)0;
-        vr[r][2 * t + 1] = (act && 2 * q + 1 < NY) ? b : (T)0;
+        vr[r][2 * t + 1] = (act && 2 * x.q + 1 < NY) ? b : (T)0;
       }
     }
   }
+  int trf, jf;  // r2c pass-2 lanes
+  lane_map<(L::MAPS & kMapRx2) != 0, S::GX, NTRX>(tid, trf, jf);
   for (int s = st0; s < st1; s++) {
     const cpx<T>* zs = (const cpx<T>*)c.zbuf + ((size_t)s * c.g.nz + z) * plane;
-    y_fwd<T, Y1, Y2, NT>(zs, P, TW, KY, npx);
+    y_fwd<T, Y1, Y2, NT, L>(zs, P, TW, KY, npx);
     const T oc = (T)c.occ[s];
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      const int q = r * NTRX + tr;
-      const bool act = tr < NTRX && q < S::NPAIR;
+      const XLanes x = x_lanes<X1, X2, Y1, Y2, NT, L>(r);
       // without VREG: this round's V, loaded (branch-free) ahead of the transform
       T vv[VREG ? 1 : 2 * X2];
       if (!VREG) {
-        const int ya = 2 * min(q, S::NPAIR - 1), yb = min(ya + 1, NY - 1);
+        const int ya = 2 * min(x.q, S::NPAIR - 1), yb = min(ya + 1, NY - 1);
 #pragma unroll
         for (int t = 0; t < X2; t++) {
-          const int x = min(j + t * X1, NX - 1);
-          vv[VREG ? 0 : 2 * t] = (T)vz[(size_t)ya * NX + x];
-          vv[VREG ? 0 : 2 * t + 1] = (T)vz[(size_t)yb * NX + x];
+          const int xx = min(x.j + t * X1, NX - 1);
+          vv[VREG ? 0 : 2 * t] = (T)vz[(size_t)ya * NX + xx];
+          vv[VREG ? 0 : 2 * t + 1] = (T)vz[(size_t)yb * NX + xx];
         }
       }
       cpx<T> u[X2];
-      x_c2r<T, X1, X2, NT>(P, E, TW, c, npx, NY, q, tr, j, act, u);
+      x_c2r<T, X1, X2, Y1, Y2, NT, L>(P, E, TW, npx, x, u);
       __syncthreads();  // E is rewritten below
-      if (act && j < X1) {
-        const int y1 = 2 * q, y2 = y1 + 1;
+      if (x.act && x.j < X1) {
+        const int y2 = 2 * x.q + 1;
         if (RHO) {
           const double od = c.occ[s];
 #pragma unroll
@@ -394,19 +507,22 @@ k3_r2g(FftCtx c) {
         // forward transform of the pair, N = X2 * X1: thread j holds
         // x = j + t*X1 (t < X2), so pass 1 is a radix-X2 DFT in registers
         Dft<T, X2, -1>::run(w);
+        cpx<T>* dst = E + x.tr * RE + x.j * (X2 + L::EA);
 #pragma unroll
-        for (int t = 0; t < X2; t++) E[tr * NX + j * X2 + t] = w[t];
+        for (int t = 0; t < X2; t++) dst[t] = w[t];
       }
       __syncthreads();
-      if (act && j < X2) {
+      const int qf = r * NTRX + trf;
+      if (trf < NTRX && qf < S::NPAIR && jf < X2) {
+        cpx<T>* row = E + trf * RE + jf;
         cpx<T> a[X1];
 #pragma unroll
-        for (int t = 0; t < X1; t++) a[t] = E[tr * NX + j + t * X2];
+        for (int t = 0; t < X1; t++) a[t] = row[t * (X2 + L::EA)];
 #pragma unroll
-        for (int t = 1; t < X1; t++) a[t] = cm(a[t], twl<T, -1>(TW, t * j));
+        for (int t = 1; t < X1; t++) a[t] = cm(a[t], tw2<T, -1, X2, L>(TW, t, jf));
         Dft<T, X1, -1>::run(a);
 #pragma unroll
-        for (int t = 0; t < X1; t++) E[tr * NX + j + t * X2] = a[t];
+        for (int t = 0; t < X1; t++) row[t * (X2 + L::EA)] = a[t];
       }
       __syncthreads();
       // split the two real rows, keep kx < nplane_x, conjugated, into P
@@ -414,28 +530,28 @@ k3_r2g(FftCtx c) {
       for (int idx = tid; idx < nq * npx; idx += NT) {
         const int qq = idx / npx, k = idx - qq * npx;
         const int r1 = 2 * (r * NTRX + qq), r2 = r1 + 1;
-        const cpx<T> Zk = E[qq * NX + k];
-        const cpx<T> Zm = E[qq * NX + (k == 0 ? 0 : NX - k)];
-        P[k * CY + r1] = {(T)0.5 * (Zk.x + Zm.x), -(T)0.5 * (Zk.y - Zm.y)};
-        if (r2 < NY) P[k * CY + r2] = {(T)0.5 * (Zk.y + Zm.y), (T)0.5 * (Zk.x - Zm.x)};
+        const cpx<T> Zk = E[qq * RE + pe<X2, L>(k)];
+        const cpx<T> Zm = E[qq * RE + pe<X2, L>(k == 0 ? 0 : NX - k)];
+        P[k * CY + py<Y1, L>(r1)] = {(T)0.5 * (Zk.x + Zm.x), -(T)0.5 * (Zk.y - Zm.y)};
+        if (r2 < NY) P[k * CY + py<Y1, L>(r2)] = {(T)0.5 * (Zk.y + Zm.y), (T)0.5 * (Zk.x - Zm.x)};
       }
       __syncthreads();
     }
     cpx<T>* zo = (cpx<T>*)c.zout + ((size_t)s * c.g.nz + z) * plane;
-    y_bwd<T, Y1, Y2, NT>(P, E, TW, zo, KY, npx);
+    y_bwd<T, Y1, Y2, NT, L>(P, E, TW, zo, KY, npx);
   }
   if (!RHO) return;
   // rho partial of this chunk (same layout and per-element order as k3_g2r)
   double* rp_out = c.rho_part + (size_t)chunk * NP * c.g.nz + (size_t)z * NP;
 #pragma unroll
   for (int r = 0; r < R; r++) {
-    const int q = r * NTRX + tr;
-    if (tr < NTRX && q < S::NPAIR && j < X1) {
-      const int y1 = 2 * q, y2 = y1 + 1;
+    const XLanes x = x_lanes<X1, X2, Y1, Y2, NT, L>(r);
+    if (x.act && x.j < X1) {
+      const int y1 = 2 * x.q, y2 = y1 + 1;
 #pragma unroll
       for (int t = 0; t < X2; t++) {
-        rp_out[(size_t)y1 * NX + j + t * X1] = acc[r][2 * t];
-        if (y2 < NY) rp_out[(size_t)y2 * NX + j + t * X1] = acc[r][2 * t + 1];
+        rp_out[(size_t)y1 * NX + x.j + t * X1] = acc[r][2 * t];
+        if (y2 < NY) rp_out[(size_t)y2 * NX + x.j + t * X1] = acc[r][2 * t + 1];
       }
     }
   }
@@ -445,7 +561,8 @@ k3_r2g(FftCtx c) {
 // radix-10 pair rows
 template <int X1, int X2, int NT>
 constexpr bool r2g_rho_ok() {
-  return PS<X1, X2, X2, X1, NT>::ROUNDS * X2 <= 20;
+  return (((X1 * X2 + 1) / 2 + NT / (X1 > X2 ? X1 : X2) - 1) / (NT / (X1 > X2 ? X1 : X2))) * X2 <=
+         20;
 }
 
 // ------------------------------------------------------------------ launch
@@ -453,6 +570,19 @@ constexpr bool r2g_rho_ok() {
 enum PlaneKind { G2R = 0, R2G = 1, G2R_PSIR = 2, R2G_RHO = 3 };
 
 constexpr int kPlaneThreads = 256;
+
+// Layouts.  OldLay is the layout before the bank-conflict work (one
+// twiddle table by t*j, j-minor lanes everywhere, P columns NY|1, unpadded
+// exchange rows); PadLay has the [t][j] twiddle tables, transform-minor
+// lanes wherever only LDS is touched and odd P column and exchange row
+// strides.  Measured on the 100^3 / 128-state R->G pass (FP32): 286 -> 256
+// us, identical results.  Padding grows the LDS a little, so a launch uses
+// PadLay only where that keeps the workgroups per CU of OldLay (FP64 at
+// 100^2 planes sits 8 bytes under half a CU's LDS with OldLay).
+template <int N>
+using OldLay = Lay<(N % 2 == 0) ? 1 : 0, 0, 0, 0, 0, false>;
+template <int N>
+using PadLay = Lay<(N % 2 == 0) ? 5 : 4, 0, 0, (N % 2 == 0) ? 5 : 4, 31, true>;
 
 // x rounds per k3_g2r workgroup: the rho accumulators are 2*X2 doubles per
 // round and thread; FP64 keeps one round, FP32 two.  When the LDS of one
@@ -466,24 +596,39 @@ constexpr int row_rounds() {
 
 constexpr size_t kCuLds = 160 * 1024;
 
-template <typename T, int X1, int X2, int Y1, int Y2, int NT>
+template <typename T, int X1, int X2, int Y1, int Y2, int NT, class L>
 bool g2r_wide(int npx) {
-  return 2 * plane3_smem<T, X1, X2, Y1, Y2, NT>(npx) > kCuLds;
+  return 2 * plane3_smem_l<T, X1, X2, Y1, Y2, NT, L>(npx) > kCuLds;
+}
+
+// PadLay where it keeps OldLay's workgroups per CU (by LDS)
+template <typename T, int X1, int X2, int Y1, int Y2, int NT>
+bool use_pad(int npx) {
+  const size_t so = plane3_smem_l<T, X1, X2, Y1, Y2, NT, OldLay<X1 * X2>>(npx);
+  const size_t sp = plane3_smem_l<T, X1, X2, Y1, Y2, NT, PadLay<X1 * X2>>(npx);
+  return kCuLds / sp >= kCuLds / so;
+}
+
+template <typename T, int X1, int X2, int Y1, int Y2, int NT>
+size_t plane3_smem(int npx) {
+  return use_pad<T, X1, X2, Y1, Y2, NT>(npx)
+             ? plane3_smem_l<T, X1, X2, Y1, Y2, NT, PadLay<X1 * X2>>(npx)
+             : plane3_smem_l<T, X1, X2, Y1, Y2, NT, OldLay<X1 * X2>>(npx);
 }
 
 // Launch facts of one size for the host's chunk choice: row parts of the
 // G->R grid and resident workgroups per CU of both kernels.
-template <typename T, int X1, int X2, int Y1, int Y2>
-hipError_t plane3_info(int npx, int* rowparts, int* occ_g2r, int* occ_r2g, int* rho_ok) {
+template <typename T, int X1, int X2, int Y1, int Y2, class L>
+hipError_t plane3_info_l(int npx, int* rowparts, int* occ_g2r, int* occ_r2g, int* rho_ok) {
   constexpr int NT = kPlaneThreads;
-  using S = PS<X1, X2, Y1, Y2, NT>;
+  using S = PS<X1, X2, Y1, Y2, NT, L>;
   constexpr bool VREG = sizeof(T) == 4 && S::ROUNDS * 2 * X2 <= 48;
-  const size_t smem = plane3_smem<T, X1, X2, Y1, Y2, NT>(npx);
-  const bool wide = g2r_wide<T, X1, X2, Y1, Y2, NT>(npx);
+  const size_t smem = plane3_smem_l<T, X1, X2, Y1, Y2, NT, L>(npx);
+  const bool wide = g2r_wide<T, X1, X2, Y1, Y2, NT, L>(npx);
   *rowparts = wide ? 1 : (S::ROUNDS + row_rounds<T>() - 1) / row_rounds<T>();
-  const void* g = wide ? (const void*)k3_g2r<T, X1, X2, Y1, Y2, NT, S::ROUNDS, false>
-                       : (const void*)k3_g2r<T, X1, X2, Y1, Y2, NT, row_rounds<T>(), false>;
-  const void* r = (const void*)k3_r2g<T, X1, X2, Y1, Y2, NT, VREG, false>;
+  const void* g = wide ? (const void*)k3_g2r<T, X1, X2, Y1, Y2, NT, S::ROUNDS, false, L>
+                       : (const void*)k3_g2r<T, X1, X2, Y1, Y2, NT, row_rounds<T>(), false, L>;
+  const void* r = (const void*)k3_r2g<T, X1, X2, Y1, Y2, NT, VREG, false, L>;
   *rho_ok = r2g_rho_ok<X1, X2, NT>() ? 1 : 0;
   hipError_t e;
   if ((e = oa::grant_lds(g, smem)) != hipSuccess || (e = oa::grant_lds(r, smem)) != hipSuccess)
@@ -493,46 +638,46 @@ hipError_t plane3_info(int npx, int* rowparts, int* occ_g2r, int* occ_r2g, int* 
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ_r2g, r, NT, smem);
 }
 
-template <typename T, int X1, int X2, int Y1, int Y2>
-hipError_t launch_plane3(int kind, const FftCtx& c, hipStream_t st) {
+template <typename T, int X1, int X2, int Y1, int Y2, class L>
+hipError_t launch_plane3_l(int kind, const FftCtx& c, hipStream_t st) {
   constexpr int NT = kPlaneThreads;
-  using S = PS<X1, X2, Y1, Y2, NT>;
+  using S = PS<X1, X2, Y1, Y2, NT, L>;
   constexpr int kRowRounds = row_rounds<T>();
   constexpr int RPn = (S::ROUNDS + kRowRounds - 1) / kRowRounds;
   constexpr bool VREG = sizeof(T) == 4 && S::ROUNDS * 2 * X2 <= 48;
-  const size_t smem = plane3_smem<T, X1, X2, Y1, Y2, NT>(c.g.nplane_x);
+  const size_t smem = plane3_smem_l<T, X1, X2, Y1, Y2, NT, L>(c.g.nplane_x);
   if (c.g.nx != S::NX || c.g.ny != S::NY) return hipErrorInvalidValue;
   // raise the dynamic LDS limit of an instantiation (once per device)
   auto grant = [&](const void* fn) -> hipError_t { return oa::grant_lds(fn, smem); };
   hipError_t e = hipSuccess;
   switch (kind) {
     case G2R: {
-      if (g2r_wide<T, X1, X2, Y1, Y2, NT>(c.g.nplane_x)) {
-        auto fn = k3_g2r<T, X1, X2, Y1, Y2, NT, S::ROUNDS, false>;
+      if (g2r_wide<T, X1, X2, Y1, Y2, NT, L>(c.g.nplane_x)) {
+        auto fn = k3_g2r<T, X1, X2, Y1, Y2, NT, S::ROUNDS, false, L>;
         if ((e = grant((const void*)fn)) != hipSuccess) return e;
         hipLaunchKernelGGL(fn, dim3(c.g.nz, 1, c.nchunk), dim3(NT), smem, st, c);
       } else {
-        auto fn = k3_g2r<T, X1, X2, Y1, Y2, NT, kRowRounds, false>;
+        auto fn = k3_g2r<T, X1, X2, Y1, Y2, NT, kRowRounds, false, L>;
         if ((e = grant((const void*)fn)) != hipSuccess) return e;
         hipLaunchKernelGGL(fn, dim3(c.g.nz, RPn, c.nchunk), dim3(NT), smem, st, c);
       }
       break;
     }
     case G2R_PSIR: {
-      auto fn = k3_g2r<T, X1, X2, Y1, Y2, NT, kRowRounds, true>;
+      auto fn = k3_g2r<T, X1, X2, Y1, Y2, NT, kRowRounds, true, L>;
       if ((e = grant((const void*)fn)) != hipSuccess) return e;
       hipLaunchKernelGGL(fn, dim3(c.g.nz, RPn, 1), dim3(NT), smem, st, c);
       break;
     }
     case R2G: {
-      auto fn = k3_r2g<T, X1, X2, Y1, Y2, NT, VREG, false>;
+      auto fn = k3_r2g<T, X1, X2, Y1, Y2, NT, VREG, false, L>;
       if ((e = grant((const void*)fn)) != hipSuccess) return e;
       hipLaunchKernelGGL(fn, dim3(c.g.nz, c.nchunk_r), dim3(NT), smem, st, c);
       break;
     }
     case R2G_RHO: {
       if (!r2g_rho_ok<X1, X2, NT>()) return hipErrorInvalidValue;
-      auto fn = k3_r2g<T, X1, X2, Y1, Y2, NT, VREG, r2g_rho_ok<X1, X2, NT>()>;
+      auto fn = k3_r2g<T, X1, X2, Y1, Y2, NT, VREG, r2g_rho_ok<X1, X2, NT>(), L>;
       if ((e = grant((const void*)fn)) != hipSuccess) return e;
       hipLaunchKernelGGL(fn, dim3(c.g.nz, c.nchunk_r), dim3(NT), smem, st, c);
       break;
@@ -541,6 +686,22 @@ hipError_t launch_plane3(int kind, const FftCtx& c, hipStream_t st) {
       return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <typename T, int X1, int X2, int Y1, int Y2>
+hipError_t plane3_info(int npx, int* rowparts, int* occ_g2r, int* occ_r2g, int* rho_ok) {
+  return use_pad<T, X1, X2, Y1, Y2, kPlaneThreads>(npx)
+             ? plane3_info_l<T, X1, X2, Y1, Y2, PadLay<X1 * X2>>(npx, rowparts, occ_g2r, occ_r2g,
+                                                                 rho_ok)
+             : plane3_info_l<T, X1, X2, Y1, Y2, OldLay<X1 * X2>>(npx, rowparts, occ_g2r, occ_r2g,
+                                                                 rho_ok);
+}
+
+template <typename T, int X1, int X2, int Y1, int Y2>
+hipError_t launch_plane3(int kind, const FftCtx& c, hipStream_t st) {
+  return use_pad<T, X1, X2, Y1, Y2, kPlaneThreads>(c.g.nplane_x)
+             ? launch_plane3_l<T, X1, X2, Y1, Y2, PadLay<X1 * X2>>(kind, c, st)
+             : launch_plane3_l<T, X1, X2, Y1, Y2, OldLay<X1 * X2>>(kind, c, st);
 }
 
 }  // namespace oa3
