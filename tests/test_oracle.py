@@ -153,6 +153,77 @@ def test_eke_force_closed_form(small):
     assert abs(eke - ref_e) < 1e-12 * abs(ref_e)
 
 
+def test_kinetic_force_is_minus_the_energy_gradient(small):
+    """oo_eke_force: EKE is quadratic in the coefficients, so central
+    differences give its gradient up to rounding.  Reference convention (the
+    same as the non-local force's): F = -(dE/dRe c + i dE/dIm c) for kx > 0;
+    on the kx = 0 plane, where c and its stored partner -G move together,
+    half of that; at G = 0 (real) the whole derivative.  A dropped kx > 0
+    doubling, a w on the wrong plane or a missing 1/2 fails it."""
+    s, o = small
+    kx, ky, kz = s["kx"], s["ky"], s["kz"]
+    f = np.zeros_like(s["psi"])
+    o.eke_force(s["psi"], f)
+    key = {(a, b, c): i for i, (a, b, c) in enumerate(zip(kx, ky, kz))}
+    g0 = key[(0, 0, 0)]
+    uniq0 = [i for i in range(len(kx)) if kx[i] == 0 and (ky[i] > 0 or (ky[i] == 0 and kz[i] > 0))]
+    pos_kx = [i for i in range(len(kx)) if kx[i] > 0]
+    rng = np.random.default_rng(5)
+    h = 1e-3
+
+    def eke(psi):
+        return o.eke_force(psi, np.zeros_like(psi))
+    for st in (0, s["nstates"] - 1):
+        for p in [g0] + list(rng.choice(uniq0, 4)) + list(rng.choice(pos_kx, 4)):
+            partner = key[(0, -ky[p], -kz[p])] if kx[p] == 0 else None
+            grad = []
+            for d in ((1.0, 0.0), (0.0, 1.0)) if p != g0 else ((1.0, 0.0),):
+                e = []
+                for sgn in (1.0, -1.0):
+                    psi = s["psi"].copy()
+                    dz = sgn * h * complex(*d)
+                    psi[st, p] += dz
+                    if partner is not None and partner != p:
+                        psi[st, partner] += np.conj(dz)
+                    e.append(eke(psi))
+                grad.append((e[0] - e[1]) / (2 * h))
+            g = complex(grad[0], grad[1] if len(grad) > 1 else 0.0)
+            scale = 1.0 if (kx[p] > 0 or p == g0) else 0.5
+            expect = -scale * g
+            assert abs(f[st, p] - expect) < 1e-8 * max(abs(expect), 1.0), (st, p, f[st, p], expect)
+
+
+def test_fictitious_mass_preconditions_the_kinetic_force():
+    """oo_cmass (CP_create_mass): the two branches meet at g^2 = ecut_mass, and
+    above it an SD step with the kinetic force alone scales every coefficient
+    by the same factor 1 - dt ecut_mass / cmass0 (the mass cancels w g^2);
+    cmass0 = 2 tau^2 CP_EMAGIC with tau in atomic units of time, checked
+    against scipy's value of that unit."""
+    from scipy import constants
+    s = small_system(ecut=40.0)
+    o = oracle.System(s["grid"], s["hmat"], s["ecut_ry"], s["nstates"], s["kx"], s["ky"], s["kz"])
+    tau_fs, ecut_mass, dt = 25.0, 2.0, 5.0
+    cm = o.cmass(tau_fs, ecut_mass)
+    g2 = system.gvec_sq(s["kx"], s["ky"], s["kz"], s["hmat"])
+    w = system.gamma_weights(s["kx"])
+    au_fs = constants.hbar / constants.physical_constants["Hartree energy"][0] * 1e15
+    cmass0 = 2.0 * (tau_fs / au_fs) ** 2 * 0.0005
+    low = g2 <= ecut_mass
+    assert low.any() and (~low).any()
+    assert np.allclose(cm[low], w[low] * cmass0, rtol=1e-5, atol=0.0)
+    # continuity: the branch above, evaluated at the cutoff, is the branch below
+    hi_at_cut = w * ecut_mass * (cm[~low] / (w[~low] * g2[~low])).mean()
+    assert np.allclose(hi_at_cut, w * cmass0, rtol=1e-5)
+    # the preconditioned kinetic step is uniform above ecut_mass
+    psi = s["psi"].copy()
+    f = np.zeros_like(psi)
+    o.eke_force(psi, f)
+    o.sd_update(cm, dt, f, psi)
+    ratio = psi[:, ~low] / s["psi"][:, ~low]
+    factor = 1.0 - dt * ecut_mass / cmass0
+    assert np.allclose(ratio, factor, rtol=1e-5, atol=0.0)
+
+
 def test_overlap_and_lambda(small):
     """oo_overlap = full-sphere overlap (2I for norm-2 orthonormal input);
     oo_lambda = Re sum_stored conj(psi_i) F_j; oo_lambda_apply weights 1/2, 1."""
