@@ -76,7 +76,7 @@ def plane_bytes(stage, geo, csz, nst, launch):
 
 
 def run_precision(sysd, precision, steps, warmup, device=0, use_graph=0, vks=True, autotune=1,
-                  nonlocal_kb=False, prof_steps=None):
+                  nonlocal_kb=False, prof_steps=None, gemm_split=None):
     """vks: V(r) = V_xc + V_H + V_loc rebuilt from each step's rho on the
     device (the density chares' work inside the step); else the seeded
     local potential as a fixed input.  nonlocal_kb: add the l = 0 KB
@@ -84,11 +84,12 @@ def run_precision(sysd, precision, steps, warmup, device=0, use_graph=0, vks=Tru
     from openatom_amd.step import CPContext
     # The reference step has no ||Psi^H Psi - I|| evaluation (Ortho checks the
     # T iterates); it is measured once after the timed steps instead.
+    extra = {} if gemm_split is None else {"gemm_split": int(gemm_split)}
     c = CPContext(sysd["grid"], sysd["hmat"], sysd["ecut_ry"], sysd["nstates"], sysd["kx"],
                   sysd["ky"], sysd["kz"], precision=precision, device=device, dt=5.0,
                   ns_tol=1e-15 if precision == "fp64" else 1e-14, ns_max_iter=30,
                   want_resid=0, use_graph=use_graph, vks_from_rho=int(vks),
-                  autotune=int(autotune))
+                  autotune=int(autotune), **extra)
     c.set_psi(sysd["psi"])
     if vks:
         t = sysd["pseudo"]
@@ -240,6 +241,7 @@ def serving_run(systems, arrive, precision, f_tol, max_steps, budget, max_active
     with serve.Server(hbm_budget=budget, max_active=max_active) as srv:
         t0 = time.perf_counter()
         i = 0
+        t_log = 0.0
         while len(res) < n:
             now = time.perf_counter() - t0
             while i < n and arrive[i] <= now:
@@ -249,6 +251,11 @@ def serving_run(systems, arrive, precision, f_tol, max_steps, budget, max_active
                 i += 1
             for r in srv.poll():
                 res[r["tag"]] = r
+            if now - t_log >= 30.0:  # progress of a long run
+                t_log = now
+                st = srv.stats()
+                log(f"[serve] {precision} t={now:.0f}s submitted {i}/{n} done {len(res)} "
+                    f"active {int(st['active'])} queued {int(st['queued'])}")
             time.sleep(5e-4)
         wall = time.perf_counter() - t0
         st = srv.stats()
@@ -265,6 +272,36 @@ def serving_run(systems, arrive, precision, f_tol, max_steps, budget, max_active
         "steps_mean": round(float(steps.mean()), 1), "steps_max": int(steps.max()),
         "wall_s": round(wall, 3), "peak_active": int(st["peak_active"]),
         "peak_hbm_gib": round(st["peak_hbm_bytes"] / 2**30, 3)})
+
+
+def serving_leg(args, mix, precisions):
+    """The serving workload in each precision, with the largest energy
+    difference per atom between precisions over requests both converged."""
+    import numpy as np
+    systems, arrive = mix
+    sopts = dict(dt=20.0, use_cg=1, ns_mode=1)
+    serving = {"workload": f"{len(systems)} independent minimisations of water clusters "
+                           f"(4-{args.serve_max_mol} H2O log-uniform, 4 states/molecule, "
+                           f"Ecut in {{50,60,70}} Ry), Poisson arrivals at "
+                           f"{args.serve_rate:g}/s, conjugate gradient to |F| <= "
+                           f"{args.serve_ftol:g}",
+               "nmol_hist": {str(k): int(v) for k, v in zip(*np.unique(
+                   [int(s["nmol"]) for s in systems], return_counts=True))},
+               "max_active": args.serve_active, "hbm_budget_gib": args.serve_budget_gib}
+    runs_s = {}
+    for prec in precisions:
+        log(f"[bench] serving ({prec}): {len(systems)} requests")
+        r = serving_run(systems, arrive, prec, args.serve_ftol, args.serve_max_steps,
+                        int(args.serve_budget_gib * 2**30), args.serve_active, sopts)
+        log(f"[bench] serving {prec}: {r['summary']}")
+        serving[prec] = r["summary"]
+        runs_s[prec] = r["res"]
+    if "mixed" in runs_s and "fp64" in runs_s:
+        de = [abs(runs_s["mixed"][k]["eelec"] - runs_s["fp64"][k]["eelec"]) /
+              (3 * systems[k]["nmol"]) for k in range(len(systems))
+              if runs_s["mixed"][k]["status"] == 0 and runs_s["fp64"][k]["status"] == 0]
+        serving["max_dE_per_atom_mixed_vs_fp64"] = max(de) if de else None
+    return serving
 
 
 def cpu_baseline(sysd, nsteps, warmup=3):
@@ -326,6 +363,10 @@ def main():
     ap.add_argument("--serve-max-steps", type=int, default=1500)
     ap.add_argument("--serve-active", type=int, default=8, help="requests on the GPU at once")
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the serving workload (--serve-requests requests, each "
+                         "precision of --serve-precisions) and print its JSON line")
+    ap.add_argument("--serve-precisions", default="fp64,mixed")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the step on a single "
@@ -339,6 +380,12 @@ def main():
     if args.serve_requests > 0:
         log(f"[bench] building the serving mix: {args.serve_requests} requests")
         mix = serving_mix(args.serve_requests, args.serve_rate, args.seed, args.serve_max_mol)
+    if args.serve_only:
+        if mix is None:
+            sys.exit("--serve-only needs --serve-requests > 0")
+        print(json.dumps({"serving": serving_leg(args, mix,
+                                                 args.serve_precisions.split(","))}), flush=True)
+        return
 
     if args.copy_calib_mb > 0:
         copy_calibration(args.copy_calib_mb)
@@ -384,6 +431,18 @@ def main():
                                  r["timings"].get("nonlocal+reduce_lambda", 0.0), 4),
                              "lambda_only_ms": round(
                                  runs[(prec, 0)]["timings"].get("reduce_lambda", 0.0), 4)}
+    # the mixed step with the Lambda products in FP64 (gemm_split 0), the
+    # default split-BF16 gram (1), and gram and projection split (9)
+    lam_forms = None
+    if vks and not args.no_graph:
+        lam_forms = {}
+        for gs in (0, 1, 9):
+            r = run_precision(sysd, "mixed", args.steps, args.warmup, vks=True,
+                              autotune=not args.no_autotune, gemm_split=gs)
+            log(f"[bench] mixed gemm_split={gs} {r['ms']:.3f} ms/step")
+            lam_forms[str(gs)] = {"ms_per_step": round(r["ms"], 4),
+                                  "lambda_ms": round(r["timings"].get("reduce_lambda", 0.0), 4),
+                                  "sd_ms": round(r["timings"].get("sd_sync", 0.0), 4)}
     qt = None
     if vks and args.quality_steps > 0:
         log(f"[bench] quality trace: mixed vs FP64 in lockstep over {args.quality_steps} steps")
@@ -410,28 +469,8 @@ def main():
 
     serving = None
     if mix is not None:
-        systems, arrive = mix
-        sopts = dict(dt=20.0, use_cg=1, ns_mode=1)
-        serving = {"workload": f"{len(systems)} independent minimisations of water clusters "
-                               f"(4-{args.serve_max_mol} H2O log-uniform, 4 states/molecule, "
-                               f"Ecut in {{50,60,70}} Ry), Poisson arrivals at "
-                               f"{args.serve_rate:g}/s, conjugate gradient to |F| <= "
-                               f"{args.serve_ftol:g}",
-                   "nmol": [int(s["nmol"]) for s in systems],
-                   "max_active": args.serve_active, "hbm_budget_gib": args.serve_budget_gib}
-        runs_s = {}
-        for prec in ("fp64", "mixed"):
-            log(f"[bench] serving sample ({prec})")
-            r = serving_run(systems, arrive, prec, args.serve_ftol, args.serve_max_steps,
-                            int(args.serve_budget_gib * 2**30), args.serve_active, sopts)
-            log(f"[bench] serving {prec}: {r['summary']}")
-            serving[prec] = r["summary"]
-            runs_s[prec] = r["res"]
-        de = [abs(runs_s["mixed"][k]["eelec"] - runs_s["fp64"][k]["eelec"]) /
-              (3 * systems[k]["nmol"]) for k in range(len(systems))
-              if runs_s["mixed"][k]["status"] == 0 and runs_s["fp64"][k]["status"] == 0]
-        serving["max_dE_per_atom_mixed_vs_fp64"] = max(de) if de else None
-        del mix, systems
+        serving = serving_leg(args, mix, ("fp64", "mixed"))
+        del mix
 
     cpu = None
     if not args.no_cpu:
@@ -472,6 +511,7 @@ def main():
         "graph_replay_steps_per_s": graph,
         "fixed_v_steps": fixed,
         "with_nonlocal_kb": with_nl,
+        "mixed_lambda_forms": lam_forms,
         "fp64": {"steps_per_s": round(r64v["steps_per_s"], 3), "ms_per_step": round(r64v["ms"], 4),
                  "ns_iters": r64["last"]["ns_iters"], "resid_max": r64["last"]["resid_max"],
                  "roofline": rl64, "timings_ms": r64["timings"],

@@ -99,14 +99,20 @@ typedef struct {
                           or two launches per iteration.  Chunking changes the order of the
                           rho sums (rounding only).  0 (default): fixed heuristics */
   int gemm_split;      /* mixed precision only: products run as split-BF16 matrix-core
-                          products (each FP64 operand split into three BF16 terms, the six
-                          term products with i + j <= 2 summed in FP32; K-slice partials
-                          summed in FP64), bit 0 (default on): Lambda = Re Psi^H F and
-                          F -= w Psi Lambda; bit 1: S = Psi^H Psi; bit 2: Psi T, in the
-                          update form Psi + Psi (T - I).  Bits 0 and 2 apply the split to the
-                          projection and the rotation up to 128 states (FP64 above).  The
-                          ||Psi^H Psi/2 - I|| residual, Newton-Schulz and the energies stay
-                          FP64.  Ignored in FP64 contexts. */
+                          products (each FP64 operand rounded to FP32 and split exactly into
+                          three BF16 terms, the six term products with i + j <= 2 summed in
+                          FP32; K-slice partials summed in FP64), bit 0 (default on): the
+                          Lambda gram Re Psi^H F; bit 1: S = Psi^H Psi; bit 2: Psi T, in the
+                          update form Psi + Psi (T - I); bit 3: the Lambda projection
+                          F -= w Psi Lambda.  Bits 2 and 3 apply up to 128 states (FP64
+                          above).  The ||Psi^H Psi/2 - I|| residual, Newton-Schulz and the
+                          energies stay FP64.  Ignored in FP64 contexts. */
+  int ns_fp32;         /* mixed precision only: Newton-Schulz (S/2)^{-1/2} runs its first
+                          iterations in FP32 on the matrix cores, until the rms change of T
+                          is <= 1e-4 (at most ns_max_iter - 1 of them), then FP64 iterations
+                          from that iterate to ns_tol, ns_max_iter iterations in all (two
+                          launches per iteration; oa_step_out.ns_iters counts both).  T meets
+                          the same tolerance as in FP64.  0 (default): FP64 throughout */
 } oa_options;
 
 typedef struct {

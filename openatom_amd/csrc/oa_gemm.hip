@@ -996,14 +996,21 @@ __global__ void k_ns_init(const double* __restrict__ S, double* __restrict__ X0,
   }
 }
 
-// True when iteration `it` must not run (converged before it).  Thread 0
-// decides and broadcasts, so the whole workgroup takes the same path.
+// FP64 iterations this launch sequence may run: max_iter, less the FP32
+// iterations before it in the mixed-precision form
+__device__ __forceinline__ int ns_limit(int max_iter, int mixed, const StepDev* res) {
+  return mixed ? max_iter - *(volatile const int*)&res->ns32_iter : max_iter;
+}
+
+// True when iteration `it` must not run (converged before it, or past the
+// limit).  Thread 0 decides and broadcasts, so the whole workgroup takes the
+// same path.
 __device__ bool ns_skip(const double* __restrict__ errpart, int it, int nwg, int n, double tol,
-                        StepDev* res, bool record) {
+                        StepDev* res, bool record, int lim) {
   __shared__ int skip;
   if (threadIdx.x == 0) {
     int f = 0;
-    if (*(volatile int*)&res->ns_iter != 0) {
+    if (it > lim || *(volatile int*)&res->ns_iter != 0) {
       f = 1;
     } else if (it > 1) {
       double s = 0.0;
@@ -1028,9 +1035,9 @@ __global__ __launch_bounds__(256) void k_ns_step1(const double* __restrict__ T,
                                                   const double* __restrict__ X,
                                                   double* __restrict__ C, int n, int it,
                                                   double tol, const double* errpart,
-                                                  StepDev* res) {
+                                                  StepDev* res, int max_iter, int mixed) {
   const int nt = (n + NT - 1) / NT, nwg = nt * nt;
-  if (ns_skip(errpart, it, nwg, n, tol, res, true)) return;
+  if (ns_skip(errpart, it, nwg, n, tol, res, true, ns_limit(max_iter, mixed, res))) return;
   int row0, col0;
   blk_coords(nt, row0, col0);
   blk_store(C, blk_gemm(T, X, n, row0, col0), n, row0, col0, -1.0, 3.0, nullptr);
@@ -1041,9 +1048,10 @@ __global__ __launch_bounds__(256) void k_ns_step23(const double* __restrict__ Xo
                                                    const double* __restrict__ C,
                                                    double* __restrict__ Xn,
                                                    double* __restrict__ Tn, int n, int it,
-                                                   double tol, double* errpart, StepDev* res) {
+                                                   double tol, double* errpart, StepDev* res,
+                                                   int max_iter, int mixed) {
   const int nt = (n + NT - 1) / NT, nwg = nt * nt;
-  if (ns_skip(errpart, it, nwg, n, tol, res, false)) return;
+  if (ns_skip(errpart, it, nwg, n, tol, res, false, ns_limit(max_iter, mixed, res))) return;
   int row0, col0;
   blk_coords(nt, row0, col0);
   blk_store(Xn, blk_gemm(Xo, C, n, row0, col0), n, row0, col0, 0.5, 0.0, nullptr);
@@ -1053,15 +1061,16 @@ __global__ __launch_bounds__(256) void k_ns_step23(const double* __restrict__ Xo
 }
 
 __global__ void k_ns_final(const double* __restrict__ errpart, int nwg, int n, int max_iter,
-                           double tol, StepDev* res) {
+                           double tol, StepDev* res, int mixed) {
   if (threadIdx.x != 0) return;
   if (res->ns_iter != 0) return;
+  const int lim = ns_limit(max_iter, mixed, res);
   double s = 0.0;
-  const double* ep = errpart + (size_t)max_iter * nwg;
+  const double* ep = errpart + (size_t)lim * nwg;
   for (int w = 0; w < nwg; w++) s += ep[w];
   const double err = sqrt(s / ((double)n * n));
   res->ns_err = err;
-  res->ns_iter = (err <= tol) ? max_iter : -max_iter;
+  res->ns_iter = (err <= tol) ? lim : -lim;
 }
 
 __global__ void k_ns_copy(const double* __restrict__ T0, const double* __restrict__ T1,
@@ -1074,27 +1083,215 @@ __global__ void k_ns_copy(const double* __restrict__ T0, const double* __restric
     Tout[e] = src[e];
 }
 
-static hipError_t ns_run_launches(const NsArgs& a, hipStream_t st) {
+// X = (S/2) T in FP64, before each FP64 iteration of the mixed-precision
+// form.  The coupled iteration keeps X = (S/2) T and sees S only through X,
+// so it cannot remove an error of T that does not commute with S (an FP32
+// T's): it converges to within that error (measured 1e-10 .. 1e-7).  With X
+// rebuilt from the exact S, the iteration is T' = (3I - T (S/2) T) T / 2,
+// which corrects such errors quadratically.
+__global__ __launch_bounds__(256) void k_ns_x_from_t(const double* __restrict__ S,
+                                                     const double* __restrict__ T,
+                                                     double* __restrict__ X, int n, int it,
+                                                     double tol, const double* errpart,
+                                                     StepDev* res, int max_iter) {
+  const int nt = (n + NT - 1) / NT, nwg = nt * nt;
+  if (ns_skip(errpart, it, nwg, n, tol, res, false, ns_limit(max_iter, 1, res))) return;
+  int row0, col0;
+  blk_coords(nt, row0, col0);
+  blk_store(X, blk_gemm(S, T, n, row0, col0), n, row0, col0, 0.5, 0.0, nullptr);
+}
+
+// init: X0 = S/2, T0 = I; else X0 and T0 are already set (the FP64 phase of
+// the mixed-precision iteration)
+static hipError_t ns_run_launches(const NsArgs& a, hipStream_t st, bool init = true,
+                                  int mixed = 0) {
   const int n = a.n;
   const int nt = (n + NT - 1) / NT;
   const int nwg = nt * nt;
   const size_t nn = (size_t)n * n;
   int eb = (int)((nn + 255) / 256);
   if (eb > 512) eb = 512;
-  hipLaunchKernelGGL(k_ns_init, dim3(eb), dim3(256), 0, st, a.S, a.X0, a.T0, n, a.res);
+  if (init) hipLaunchKernelGGL(k_ns_init, dim3(eb), dim3(256), 0, st, a.S, a.X0, a.T0, n, a.res);
   double* X[2] = {a.X0, a.X1};
   double* T[2] = {a.T0, a.T1};
   for (int it = 1; it <= a.max_iter; it++) {
     const int o = (it - 1) & 1, w = it & 1;
+    if (mixed)
+      hipLaunchKernelGGL(k_ns_x_from_t, dim3(nwg), dim3(256), 0, st, a.S, T[o], X[o], n, it,
+                         a.tol, a.errpart, a.res, a.max_iter);
     hipLaunchKernelGGL(k_ns_step1, dim3(nwg), dim3(256), 0, st, T[o], X[o], a.C, n, it, a.tol,
-                       a.errpart, a.res);
+                       a.errpart, a.res, a.max_iter, mixed);
     hipLaunchKernelGGL(k_ns_step23, dim3(nwg), dim3(256), 0, st, X[o], T[o], a.C, X[w], T[w], n,
-                       it, a.tol, a.errpart, a.res);
+                       it, a.tol, a.errpart, a.res, a.max_iter, mixed);
   }
   hipLaunchKernelGGL(k_ns_final, dim3(1), dim3(64), 0, st, a.errpart, nwg, n, a.max_iter, a.tol,
-                     a.res);
+                     a.res, mixed);
   hipLaunchKernelGGL(k_ns_copy, dim3(eb), dim3(256), 0, st, a.T0, a.T1, a.Tout, n, a.res);
   return hipGetLastError();
+}
+
+// ---- FP32 iterations of the mixed-precision Newton-Schulz
+//
+// The recurrence of get_iter_Tmat (cp_lowdin.C:201-241) on FP32 X, T, C with
+// v_mfma_f32_16x16x4f32, two launches per iteration as above, until the rms
+// change of T is <= kNs32Switch; the last FP32 T is then widened into the
+// FP64 T0, and FP64 iterations with X = (S/2) T rebuilt from the exact S
+// continue to the step's tolerance (k_ns_x_from_t): they remove the FP32
+// error quadratically, and T meets the same tolerance as an FP64 run.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4f mfma16f(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// 16x16 block (row0, col0) of A B in FP32.  A lane holds A[r][k] and B[k][c]
+// with r, c = lane % 16 and k = lane / 16, as in FP64; the FP32 accumulator
+// element v of lane l is D[4 (l/16) + v][l % 16] (FP64: D[l/16 + 4 v]).
+__device__ __forceinline__ v4f blk_gemm32(const float* __restrict__ A,
+                                          const float* __restrict__ B, int n, int row0,
+                                          int col0) {
+  const int lane = threadIdx.x & 63, l16 = lane & 15, g = lane >> 4;
+  const int r = row0 + l16, cc = col0 + l16;
+  v4f acc = {0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < n; k0 += 4) {
+    const int k = k0 + g;
+    const float av = (r < n && k < n) ? A[(size_t)r * n + k] : 0.f;
+    const float bv = (k < n && cc < n) ? B[(size_t)k * n + cc] : 0.f;
+    acc = mfma16f(av, bv, acc);
+  }
+  return acc;
+}
+
+// C = alpha acc + gamma I on the block; returns sum (Told - C)^2 if Told
+__device__ __forceinline__ double blk_store32(float* __restrict__ C, const v4f& acc, int n,
+                                              int row0, int col0, float alpha, float gamma,
+                                              const float* __restrict__ Told) {
+  const int lane = threadIdx.x & 63, l16 = lane & 15, g = lane >> 4;
+  double e = 0.0;
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int row = row0 + 4 * g + v, col = col0 + l16;
+    if (row < n && col < n) {
+      const float val = alpha * acc[v] + (row == col ? gamma : 0.f);
+      C[(size_t)row * n + col] = val;
+      if (Told) {
+        const double d = (double)Told[(size_t)row * n + col] - (double)val;
+        e += d * d;
+      }
+    }
+  }
+  return e;
+}
+
+__global__ void k_ns32_init(const double* __restrict__ S, float* __restrict__ X0,
+                            float* __restrict__ T0, int n, StepDev* res) {
+  const size_t nn = (size_t)n * n;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nn;
+       e += (size_t)gridDim.x * blockDim.x) {
+    X0[e] = (float)(S[e] * 0.5);
+    const size_t i = e / n, j = e - i * n;
+    T0[e] = (i == j) ? 1.f : 0.f;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) res->ns32_iter = 0;
+}
+
+// True when FP32 iteration `it` must not run: the switch test passed before
+// it (recorded in ns32_iter by the first workgroup of step 1).
+__device__ bool ns32_skip(const double* __restrict__ errpart, int it, int nwg, int n,
+                          StepDev* res, bool record) {
+  __shared__ int skip;
+  if (threadIdx.x == 0) {
+    int f = 0;
+    if (*(volatile int*)&res->ns32_iter != 0) {
+      f = 1;
+    } else if (it > 1) {
+      double s = 0.0;
+      const double* ep = errpart + (size_t)(it - 1) * nwg;
+      for (int w = 0; w < nwg; w++) s += ep[w];
+      if (sqrt(s / ((double)n * n)) <= kNs32Switch) {
+        f = 1;
+        if (record && blockIdx.x == 0) res->ns32_iter = it - 1;
+      }
+    }
+    skip = f;
+  }
+  __syncthreads();
+  return skip != 0;
+}
+
+__global__ __launch_bounds__(256) void k_ns32_step1(const float* __restrict__ T,
+                                                    const float* __restrict__ X,
+                                                    float* __restrict__ C, int n, int it,
+                                                    const double* errpart, StepDev* res) {
+  const int nt = (n + NT - 1) / NT, nwg = nt * nt;
+  if (ns32_skip(errpart, it, nwg, n, res, true)) return;
+  int row0, col0;
+  blk_coords(nt, row0, col0);
+  blk_store32(C, blk_gemm32(T, X, n, row0, col0), n, row0, col0, -1.f, 3.f, nullptr);
+}
+
+__global__ __launch_bounds__(256) void k_ns32_step23(const float* __restrict__ Xo,
+                                                     const float* __restrict__ To,
+                                                     const float* __restrict__ C,
+                                                     float* __restrict__ Xn,
+                                                     float* __restrict__ Tn, int n, int it,
+                                                     double* errpart, StepDev* res) {
+  const int nt = (n + NT - 1) / NT, nwg = nt * nt;
+  if (ns32_skip(errpart, it, nwg, n, res, false)) return;
+  int row0, col0;
+  blk_coords(nt, row0, col0);
+  blk_store32(Xn, blk_gemm32(Xo, C, n, row0, col0), n, row0, col0, 0.5f, 0.f, nullptr);
+  const double e =
+      blk_store32(Tn, blk_gemm32(C, To, n, row0, col0), n, row0, col0, 0.5f, 0.f, To);
+  const double es = wg_sum(e);
+  if (threadIdx.x == 0) errpart[(size_t)it * nwg + blockIdx.x] = es;
+}
+
+// T of the last FP32 iterate (all max32 ran when none passed the switch
+// test) widened into the FP64 T0
+__global__ void k_ns32_widen(const float* __restrict__ T0f, const float* __restrict__ T1f,
+                             double* __restrict__ T0, int n, int max_iter, StepDev* res) {
+  int k = *(volatile int*)&res->ns32_iter;
+  if (k == 0) k = max_iter;
+  const float* T = (k & 1) ? T1f : T0f;
+  const size_t nn = (size_t)n * n;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nn;
+       e += (size_t)gridDim.x * blockDim.x)
+    T0[e] = (double)T[e];
+}
+
+
+__global__ void k_ns32_final(int max_iter, StepDev* res) {
+  if (threadIdx.x == 0 && res->ns32_iter == 0) res->ns32_iter = max_iter;
+  if (threadIdx.x == 0) {
+    res->ns_iter = 0;
+    res->ns_err = 0.0;
+  }
+}
+
+hipError_t ns_run_mixed(const NsArgs& a, const Ns32Bufs& b, hipStream_t st) {
+  const int n = a.n;
+  const int nt = (n + NT - 1) / NT;
+  const int nwg = nt * nt;
+  const size_t nn = (size_t)n * n;
+  int eb = (int)((nn + 255) / 256);
+  if (eb > 512) eb = 512;
+  hipLaunchKernelGGL(k_ns32_init, dim3(eb), dim3(256), 0, st, a.S, b.X0, b.T0, n, a.res);
+  float* X[2] = {b.X0, b.X1};
+  float* T[2] = {b.T0, b.T1};
+  const int max32 = a.max_iter > 1 ? a.max_iter - 1 : 1;  // leaves at least one FP64 iteration
+  for (int it = 1; it <= max32; it++) {
+    const int o = (it - 1) & 1, w = it & 1;
+    hipLaunchKernelGGL(k_ns32_step1, dim3(nwg), dim3(256), 0, st, T[o], X[o], b.C, n, it,
+                       a.errpart, a.res);
+    hipLaunchKernelGGL(k_ns32_step23, dim3(nwg), dim3(256), 0, st, X[o], T[o], b.C, X[w], T[w],
+                       n, it, a.errpart, a.res);
+  }
+  // the widening reads ns32_iter before k_ns32_final sets "all ran"
+  hipLaunchKernelGGL(k_ns32_widen, dim3(eb), dim3(256), 0, st, b.T0, b.T1, a.T0, n, max32,
+                     a.res);
+  hipLaunchKernelGGL(k_ns32_final, dim3(1), dim3(64), 0, st, max32, a.res);
+  return ns_run_launches(a, st, false, 1);
 }
 
 // ---- block-row variant: one grid barrier per iteration

@@ -1,15 +1,19 @@
 // oa_gemm_bf16.hip -- split-BF16 contractions of the mixed-precision step on
 // CDNA4 matrix cores (v_mfma_f32_32x32x16_bf16).
 //
-// Every FP64 operand x is split into three BF16 terms, x = h + m + l + r
-// with |r| <= ~2^-26 |x| (h = bf16(x), m = bf16(x - h), l = bf16(x - h - m),
-// each residual exact in FP64).  A product keeps the six term products with
-// i + j <= 2 (h*h, h*m, m*h, h*l, m*m, l*h): each BF16 x BF16 product is
-// exact in FP32, the dropped ones are ~2^-27 relative, and the MFMA sums
-// into FP32 accumulators.  So the result carries FP32 accumulation error
-// over the products of one K slice (gram) or of the inner dimension
-// (rotate), no more.  K-slice partial sums are written in FP32 and summed
-// in FP64.
+// Every FP64 operand x is rounded to FP32, f = fl32(x) (2^-24 relative),
+// and f is split exactly into three BF16 terms, f = h + m + l (h = bf16(f),
+// m = bf16(f - h), l = f - h - m, each step exact in FP32 with round to
+// nearest, |m| <= 2^-8 |f|, |l| <= 2^-16 |f|).  A product keeps the six term
+// products with i + j <= 2 (h*h, h*m, m*h, h*l, m*m, l*h): each BF16 x BF16
+// product is exact in FP32, the dropped ones are below 2^-24 relative, and
+// the MFMA sums into FP32 accumulators.  So the result carries the FP32
+// rounding of its operands and FP32 accumulation error over the products
+// of one K slice (gram) or of the inner dimension (rotate), no more.
+// K-slice partial sums are written in FP32 and summed in FP64.  The split
+// costs three packed conversions, two widenings and two FP32 subtractions
+// per pair of values (splitting the FP64 residuals instead took twice the
+// vector instructions, which bound the gram).
 //
 //   gram_bf16   : C = sum_k w(k) A[i][k] B[j][k] (Lambda = Re Psi^H F, and S on
 //                 request): one workgroup per 128 x 128 output block and K
@@ -51,24 +55,26 @@ __device__ __forceinline__ f16v fzero16() {
   return z;
 }
 
-// Three BF16 terms of eight doubles.
+// Three BF16 terms of eight doubles (of their FP32 roundings).
 struct Split {
   bf8 h, m, l;
 };
 
 __device__ __forceinline__ Split split3(const double (&x)[8]) {
   Split s;
+  float r[8];
 #pragma unroll
-  for (int e = 0; e < 8; e++) {
-    const double v = x[e];
-    const __bf16 a = (__bf16)(float)v;
-    const double r1 = v - (double)(float)a;
-    const __bf16 b = (__bf16)(float)r1;
-    const double r2 = r1 - (double)(float)b;
-    s.h[e] = a;
-    s.m[e] = b;
-    s.l[e] = (__bf16)(float)r2;
-  }
+  for (int e = 0; e < 8; e++) r[e] = (float)x[e];
+#pragma unroll
+  for (int e = 0; e < 8; e++) s.h[e] = (__bf16)r[e];
+#pragma unroll
+  for (int e = 0; e < 8; e++) r[e] -= (float)s.h[e];
+#pragma unroll
+  for (int e = 0; e < 8; e++) s.m[e] = (__bf16)r[e];
+#pragma unroll
+  for (int e = 0; e < 8; e++) r[e] -= (float)s.m[e];
+#pragma unroll
+  for (int e = 0; e < 8; e++) s.l[e] = (__bf16)r[e];
   return s;
 }
 

@@ -89,7 +89,7 @@ struct StepDev {
   double eke, eloc, fnorm2;
   double ns_err;
   int ns_iter;      // >0 converged at this iteration, <0 not converged
-  int pad;
+  int ns32_iter;    // FP32 iterations run before the FP64 ones (mixed Newton-Schulz)
   double resid_max, resid_rms;
   double ehart, exc, muxc;  // V_KS from rho: Hartree, exchange-correlation, sum V_xc rho dV
   double enl;               // non-local KB energy
@@ -232,6 +232,21 @@ struct NsArgs {
              // tiles (one barrier per iteration)
 };
 hipError_t ns_run(const NsArgs& a, hipStream_t st);
+// FP32 work space of the mixed-precision Newton-Schulz: X and T (double
+// buffered) and C, n x n floats each
+struct Ns32Bufs {
+  float* X0; float* X1;
+  float* T0; float* T1;
+  float* C;
+};
+// Newton-Schulz with its first iterations in FP32 on the matrix cores: FP32
+// iterations until the rms change of T is <= kNs32Switch (at most
+// a.max_iter - 1 of them), then FP64 iterations from that iterate to a.tol,
+// a.max_iter iterations in all; two launches per iteration.
+// res->ns32_iter holds the FP32 count, res->ns_iter the FP64 one (negative
+// when the tolerance was not met).
+constexpr double kNs32Switch = 1e-4;
+hipError_t ns_run_mixed(const NsArgs& a, const Ns32Bufs& b, hipStream_t st);
 bool ns_coop_possible(int n, int device);
 bool ns_rows_possible(int n, int split, int device);
 inline int ns_rows_split(int coop) { return 1 << (coop - 2); }  // coop 2, 3, 4

@@ -154,6 +154,7 @@ struct oa_ctx {
   int nslice_sym = 0, nslice_full = 0;  // gram K slices (symmetric S, full Lambda)
   double *nsX0 = nullptr, *nsX1 = nullptr, *nsT0 = nullptr, *nsT1 = nullptr, *nsC = nullptr,
          *errpart = nullptr;
+  oa::Ns32Bufs ns32{};  // FP32 Newton-Schulz work space (mixed contexts with ns_fp32)
   double *fpart = nullptr, *dpart = nullptr;
   int nfpart = 0, ndpart = 0;
   StepDev* res = nullptr;
@@ -642,6 +643,13 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
   TRY(dalloc(c, &c->nsT0, nn * sizeof(double)));
   TRY(dalloc(c, &c->nsT1, nn * sizeof(double)));
   TRY(dalloc(c, &c->nsC, nn * sizeof(double)));
+  if (c->fp32 && c->opt.ns_fp32) {
+    TRY(dalloc(c, &c->ns32.X0, nn * sizeof(float)));
+    TRY(dalloc(c, &c->ns32.X1, nn * sizeof(float)));
+    TRY(dalloc(c, &c->ns32.T0, nn * sizeof(float)));
+    TRY(dalloc(c, &c->ns32.T1, nn * sizeof(float)));
+    TRY(dalloc(c, &c->ns32.C, nn * sizeof(float)));
+  }
   const int nt = (n + 31) / 32;
   // per-workgroup error partials of either cooperative kind
   TRY(dalloc(c, &c->errpart,
@@ -896,9 +904,9 @@ oa_status st_r2g(oa_ctx* c, int ev0 = -1) {
 }
 
 // Split-BF16 products of a mixed-precision context (opt.gemm_split bits):
-// Lambda (gram and projection), S, Psi T.  The projection and the rotation
-// have a split kernel up to 128 states.
-constexpr int kSplitLambda = 1, kSplitS = 2, kSplitRot = 4;
+// the Lambda gram, S, Psi T, the Lambda projection F -= w Psi Lambda.  The
+// projection and the rotation have a split kernel up to 128 states.
+constexpr int kSplitLambda = 1, kSplitS = 2, kSplitRot = 4, kSplitProj = 8;
 bool split_on(const oa_ctx* c, int bit) { return c->fp32 && (c->opt.gemm_split & bit) != 0; }
 bool split_rotate_ok(const oa_ctx* c) { return c->nst <= 128 && (2 * c->ld) % 64 == 0; }
 
@@ -966,7 +974,7 @@ oa_status st_rotate(oa_ctx* c, const double2* X, const double* M, double2* Y, do
 
 // workgroups (|F|^2 partials) of the Lambda projection with fnorm
 int lambda_nwg(const oa_ctx* c) {
-  return split_on(c, kSplitLambda) && split_rotate_ok(c) ? oa::rotate_bf16_nwg(2 * c->npts)
+  return split_on(c, kSplitProj) && split_rotate_ok(c) ? oa::rotate_bf16_nwg(2 * c->npts)
                                                            : oa::rotate_nwg(c->nst, 2 * c->npts);
 }
 
@@ -975,9 +983,8 @@ int lambda_nwg(const oa_ctx* c) {
 // (CP_State_GSpacePlane.C:2076-2086).  with_sd: the same pass also applies
 // the SD update and leaves the |F|^2 partials (then st_sd_finish).
 oa_status st_lambda(oa_ctx* c, bool with_sd = false) {
-  const bool sp = split_on(c, kSplitLambda);
-  TRY(st_gram(c, c->psi, c->force, false, 1.0, 1.0, c->lam, sp));
-  const bool spr = sp && split_rotate_ok(c) && !with_sd;
+  TRY(st_gram(c, c->psi, c->force, false, 1.0, 1.0, c->lam, split_on(c, kSplitLambda)));
+  const bool spr = split_on(c, kSplitProj) && split_rotate_ok(c) && !with_sd;
   TRY(st_rotate(c, c->psi, c->lam, c->force, 1.0, -0.5, -1.0, with_sd, false, spr));
   return OA_OK;
 }
@@ -995,11 +1002,10 @@ oa_status st_sd_finish(oa_ctx* c) {
 // be complete before gamma, so the update is a pass of its own
 oa_status st_lambda_cg(oa_ctx* c) {
   const size_t N2 = (size_t)2 * c->nst * c->ld;  // reals in F, zero padding included
-  const bool sp = split_on(c, kSplitLambda);
-  TRY(st_gram(c, c->psi, c->force, false, 1.0, 1.0, c->lam, sp));
+  TRY(st_gram(c, c->psi, c->force, false, 1.0, 1.0, c->lam, split_on(c, kSplitLambda)));
   if (c->nst <= 128) {
     TRY(st_rotate(c, c->psi, c->lam, c->force, 1.0, -0.5, -1.0, false, true,
-                  sp && split_rotate_ok(c)));
+                  split_on(c, kSplitProj) && split_rotate_ok(c)));
     HIPCHK(oa::reduce_to(c->fpart, lambda_nwg(c), 1.0, &c->res->fnorm2, c->stream));
   } else {
     TRY(st_rotate(c, c->psi, c->lam, c->force, 1.0, -0.5, -1.0));
@@ -1093,7 +1099,10 @@ oa_status st_ns(oa_ctx* c, double tol, int max_iter) {
   a.max_iter = max_iter;
   a.tol = tol;
   a.coop = c->ns_kind;
-  HIPCHK(oa::ns_run(a, c->stream));
+  if (c->fp32 && c->opt.ns_fp32)
+    HIPCHK(oa::ns_run_mixed(a, c->ns32, c->stream));
+  else
+    HIPCHK(oa::ns_run(a, c->stream));
   c->have_T = true;
   return OA_OK;
 }
@@ -1126,7 +1135,8 @@ oa_status fetch_result(oa_ctx* c, oa_step_out* out) {
     out->enl = c->nl_natm > 0 ? r.enl : 0.0;
     out->eelec = r.eke + r.eloc + out->ehart + out->exc + out->enl;
     out->fnorm = std::sqrt(r.fnorm2);
-    out->ns_iters = r.ns_iter < 0 ? -r.ns_iter : r.ns_iter;
+    out->ns_iters = (r.ns_iter < 0 ? -r.ns_iter : r.ns_iter) +
+                    (c->fp32 && c->opt.ns_fp32 ? r.ns32_iter : 0);
     out->ns_err = r.ns_err;
     out->resid_max = r.resid_max;
     out->resid_rms = r.resid_rms;
@@ -1314,6 +1324,7 @@ void oa_default_options(oa_options* o) {
   o->vks_from_rho = 0;
   o->autotune = 0;
   o->gemm_split = 1;
+  o->ns_fp32 = 0;
 }
 
 int oa_kvecs(double ecut_ry, const double hmat[9], int* kx, int* ky, int* kz) {

@@ -1,9 +1,10 @@
 """Error model of the split-BF16 products of the mixed-precision step, on the
 CPU (numpy emulation of the device arithmetic):
 
-- the three-term split x = h + m + l (each a BF16 value, the residuals exact
-  in FP64) represents x to about 2^-26 relative, and keeping the six term
-  products with i + j <= 2 loses about 2^-26 of a product;
+- each operand is rounded to FP32, f = fl32(x) (2^-24 relative), and f is
+  split exactly into three BF16 terms f = h + m + l (round to nearest at
+  each level, the residuals exact in FP32); keeping the six term products
+  with i + j <= 2 loses less than 2^-24 of a product;
 - the matrix cores sum those products in FP32 within a K slice, so a gram
   entry carries the FP32 rounding of its slice sums.  Emulated on seeded
   water8 coefficients: S = Psi^H Psi in split BF16 lands at about 1e-8 to
@@ -24,13 +25,15 @@ def bf16(x32):
 
 
 def split3(x):
-    x = np.asarray(x, np.float64)
-    h = bf16(x.astype(np.float32)).astype(np.float64)
-    r1 = x - h
-    m = bf16(r1.astype(np.float32)).astype(np.float64)
+    """The device split: f = fl32(x), h = bf16(f), m = bf16(f - h),
+    l = bf16(f - h - m); every subtraction exact in FP32."""
+    f = np.asarray(x, np.float64).astype(np.float32)
+    h = bf16(f)
+    r1 = f - h
+    m = bf16(r1)
     r2 = r1 - m
-    lo = bf16(r2.astype(np.float32)).astype(np.float64)
-    return h, m, lo
+    lo = bf16(r2)
+    return h.astype(np.float64), m.astype(np.float64), lo.astype(np.float64)
 
 
 def split_gram(A, B, slice_len=256, chunk=16):
@@ -53,30 +56,34 @@ def split_gram(A, B, slice_len=256, chunk=16):
     return C
 
 
-def test_split_represents_operands_to_2_pow_minus_26():
+def test_split_is_exact_for_the_fp32_rounding():
     rng = np.random.default_rng(1)
     x = rng.standard_normal(200000) * np.exp(rng.uniform(-20, 20, 200000))
     h, m, lo = split3(x)
+    f = x.astype(np.float32).astype(np.float64)
+    assert np.array_equal(h + m + lo, f)  # three BF16 terms hold an FP32 value exactly
     rel = np.abs((h + m + lo) - x) / np.abs(x)
-    assert rel.max() < 2.0 ** -25
+    assert rel.max() <= 2.0 ** -24
+    assert np.all(np.abs(m) <= 2.0 ** -8 * np.abs(f)) and np.all(np.abs(lo) <= 2.0 ** -16 * np.abs(f))
     # each term is a BF16 value: 8 significant bits
     for t in (h, m, lo):
         t32 = t.astype(np.float32)
         assert np.array_equal(bf16(t32), t32)
 
 
-def test_six_products_keep_a_product_to_2_pow_minus_24():
+def test_six_products_keep_the_product_of_the_fp32_roundings():
     rng = np.random.default_rng(2)
     a = rng.standard_normal(100000)
     b = rng.standard_normal(100000)
     ah, am, al = split3(a)
     bh, bm, bl = split3(b)
     p = al * bh + am * bm + ah * bl + am * bh + ah * bm + ah * bh
-    rel = np.abs(p - a * b) / np.abs(a * b)
-    assert rel.max() < 2.0 ** -24
-    # dropping the three smallest products is what bounds it, not the split
-    full = sum(x * y for x in (ah, am, al) for y in (bh, bm, bl))
-    assert np.abs(full - a * b).max() <= np.abs(p - a * b).max()
+    fa, fb = a.astype(np.float32).astype(np.float64), b.astype(np.float32).astype(np.float64)
+    # the three dropped products (m l, l m, l l) are below 2^-24 of the product
+    # of the roundings
+    assert (np.abs(p - fa * fb) / np.abs(fa * fb)).max() < 2.0 ** -24
+    # against the FP64 product the FP32 rounding of the operands dominates
+    assert (np.abs(p - a * b) / np.abs(a * b)).max() < 2.0 ** -22
 
 
 def _gamma_rows(psi, kx):
