@@ -12,7 +12,10 @@
  * and the slot count allow, advances every admitted request one CP step at a
  * time on its own HIP stream (so their kernels overlap on the device), and
  * retires a request when |F| <= f_tol (the tol_cp_min exit of
- * CP_State_GSpacePlane.C:158-191) or after max_steps.  A request's result
+ * CP_State_GSpacePlane.C:158-191) or after max_steps.  A finished request's
+ * context stays resident (within the HBM budget) for the next request of
+ * the same shape: cell, grid, cutoff, states, k-vectors, occupations and
+ * options; it is released when another shape needs the memory.  A request's result
  * does not depend on what else runs: each context owns its buffers and
  * stream, and every kernel is deterministic.
  *
@@ -63,6 +66,8 @@ typedef struct {
   double total_ms;      /* submit -> done */
   size_t hbm_bytes;     /* device memory the request held */
   int fp64_steps;       /* steps run in FP64 after a mixed-precision switch */
+  int reused;           /* 1: the request ran on a pooled context of its shape (no
+                           device allocation at admission) */
 } oa_result;
 
 /* HBM one context of this system needs (what admission charges) */

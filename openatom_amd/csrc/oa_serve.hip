@@ -15,6 +15,12 @@
 //      releasing its memory so that queued requests can be admitted; a
 //      mixed-precision request that reaches fp64_below or stalls continues
 //      in an FP64 context.
+// A finished request's context stays resident in a pool, keyed by
+// everything oa_create consumed (cell, grid, cutoff, states, k-vectors,
+// occupations, options): the next request of the same shape takes it over
+// with oa_psi_set and its own potential, without device allocation or host
+// layout work.  Pooled contexts count against the HBM budget and are
+// released, least recently pooled first, when a new shape needs the room.
 // Steps of different requests overlap on the device; no request waits for
 // another one to finish.
 #include <hip/hip_runtime.h>
@@ -41,6 +47,19 @@ double ms_since(clk::time_point a, clk::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
+// What oa_create consumed for a request: two contexts with equal keys hold
+// the same layout and options.
+struct ShapeKey {
+  std::vector<double> scalars;  // cell, cutoff, options (integers exactly as doubles)
+  std::vector<int> ints;        // grid, states, points
+  std::vector<int> kx, ky, kz;
+  std::vector<double> occ;
+  bool operator==(const ShapeKey& o) const {
+    return ints == o.ints && scalars == o.scalars && kx == o.kx && ky == o.ky && kz == o.kz &&
+           occ == o.occ;
+  }
+};
+
 // A request with its own copies of every input.
 struct Job {
   oa_request r{};
@@ -53,7 +72,40 @@ struct Job {
   std::vector<int> lp_species;
   size_t bytes = 0;
   clk::time_point t_submit;
+  ShapeKey key;  // of the requested precision (set at submit)
 };
+
+ShapeKey shape_key(const Job& j, int precision) {
+  ShapeKey k;
+  const oa_system& sys = j.r.sys;
+  const oa_options& o = j.r.opt;
+  k.ints = {sys.grid[0], sys.grid[1], sys.grid[2], sys.nstates, sys.npts, precision,
+            o.ns_max_iter, o.use_lambda, o.use_cg, o.want_resid, o.fft_mode, o.use_graph,
+            o.ns_mode, o.vks_from_rho, o.autotune, o.gemm_split, o.ns_fp32};
+  k.scalars.assign(sys.hmat, sys.hmat + 9);
+  for (double v : {sys.ecut_ry, o.ns_tol, o.dt, o.tau_mass, o.ecut_mass}) k.scalars.push_back(v);
+  k.kx = j.kx;
+  k.ky = j.ky;
+  k.kz = j.kz;
+  k.occ = j.occ;
+  return k;
+}
+
+
+// the key of the same request's context in another precision
+ShapeKey with_precision(ShapeKey k, int precision) {
+  k.ints[5] = precision;
+  return k;
+}
+
+// A finished request's context, resident for the next request of its shape.
+struct Pooled {
+  oa_ctx* ctx = nullptr;
+  size_t bytes = 0;
+  ShapeKey key;
+};
+
+constexpr size_t kPoolMax = 64;  // pooled contexts at most
 
 using oa_serve::classify_event;
 using oa_serve::StepPoll;
@@ -61,6 +113,7 @@ using oa_serve::StepPoll;
 struct Active {
   Job job;
   oa_ctx* ctx = nullptr;
+  bool from_pool = false;  // ctx was taken over from the pool at admission
   hipEvent_t ev = nullptr;
   bool inflight = false;
   bool issue_failed = false;
@@ -91,6 +144,9 @@ struct oa_server {
   long long steps_run = 0, submitted = 0, finished = 0;
   bool stop = false;
   std::thread th;
+  // owned by the scheduler thread; its bytes are part of `used`
+  std::list<Pooled> pool;
+  long long pool_hits = 0;
 };
 
 namespace {
@@ -110,14 +166,51 @@ void finish(oa_server* s, Active* a, oa_status st) {
   res.total_ms = ms_since(a->job.t_submit, now);
   res.hbm_bytes = a->job.bytes;
   res.fp64_steps = a->fp64_steps;
+  res.reused = a->from_pool ? 1 : 0;
   if (a->ev) (void)hipEventDestroy(a->ev);
-  if (a->ctx) oa_destroy(a->ctx);
-  std::lock_guard<std::mutex> lk(s->mu);
-  s->used -= a->job.bytes;
-  s->n_active--;
-  s->finished++;
-  s->done.push_back(res);
-  s->cv_done.notify_all();
+  // a context that ran without a device error goes back to the pool
+  const bool keep = a->ctx && (st == OA_OK || st == OA_ENOCONV);
+  Pooled p;
+  std::vector<oa_ctx*> release;
+  if (keep) {
+    p.ctx = a->ctx;
+    p.bytes = oa_ctx_device_bytes(a->ctx);
+    p.key = a->in_fp64 ? with_precision(a->job.key, OA_FP64) : a->job.key;
+  } else if (a->ctx) {
+    release.push_back(a->ctx);
+  }
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->used -= a->job.bytes;
+    if (keep) {
+      s->used += p.bytes;
+      s->pool.push_back(std::move(p));
+      while (s->pool.size() > kPoolMax) {
+        s->used -= s->pool.front().bytes;
+        release.push_back(s->pool.front().ctx);
+        s->pool.pop_front();
+      }
+    }
+    s->n_active--;
+    s->finished++;
+    s->done.push_back(res);
+    s->cv_done.notify_all();
+  }
+  for (oa_ctx* c : release) oa_destroy(c);
+}
+
+// Take a pooled context of this shape (caller holds s->mu), or nullptr.
+oa_ctx* pool_take(oa_server* s, const ShapeKey& key) {
+  for (auto it = s->pool.begin(); it != s->pool.end(); ++it) {
+    if (it->key == key) {
+      oa_ctx* c = it->ctx;
+      s->used -= it->bytes;
+      s->pool.erase(it);
+      s->pool_hits++;
+      return c;
+    }
+  }
+  return nullptr;
 }
 
 oa_system job_system(Job& j) {
@@ -146,7 +239,7 @@ oa_status load_potential(Active* a) {
 // create and load one admitted request's context; false: it finished with an error
 bool start(oa_server* s, Active* a) {
   oa_system sys = job_system(a->job);
-  oa_status st = oa_create(s->device, &sys, &a->job.r.opt, &a->ctx);
+  oa_status st = a->ctx ? OA_OK : oa_create(s->device, &sys, &a->job.r.opt, &a->ctx);
   if (st == OA_OK) st = oa_psi_set(a->ctx, a->job.psi0.data());
   if (st == OA_OK) st = load_potential(a);
   if (st == OA_OK && hipEventCreateWithFlags(&a->ev, hipEventDisableTiming) != hipSuccess)
@@ -167,8 +260,14 @@ oa_status to_fp64(oa_server* s, Active* a) {
   oa_destroy(a->ctx);
   a->ctx = nullptr;
   a->job.r.opt.precision = OA_FP64;
+  {
+    // a pooled FP64 context of this shape: the request's reservation already
+    // covers one (admission charged the larger of the two precisions)
+    std::lock_guard<std::mutex> lk(s->mu);
+    a->ctx = pool_take(s, with_precision(a->job.key, OA_FP64));
+  }
   oa_system sys = job_system(a->job);
-  st = oa_create(s->device, &sys, &a->job.r.opt, &a->ctx);
+  st = a->ctx ? OA_OK : oa_create(s->device, &sys, &a->job.r.opt, &a->ctx);
   if (st == OA_OK) st = oa_psi_set(a->ctx, a->job.psi0.data());
   if (st == OA_OK) st = load_potential(a);
   a->in_fp64 = true;
@@ -178,6 +277,7 @@ oa_status to_fp64(oa_server* s, Active* a) {
 void scheduler(oa_server* s) {
   (void)hipSetDevice(s->device);
   std::list<Active*> active;
+  std::vector<oa_ctx*> release;  // pooled contexts given up for room
   for (;;) {
     // 1. admission (arrival order, skipping requests that do not fit yet)
     std::vector<Active*> admitted;
@@ -188,9 +288,19 @@ void scheduler(oa_server* s) {
         if (s->stop && s->queue.empty()) break;
       }
       for (auto it = s->queue.begin(); it != s->queue.end() && s->n_active < s->max_active;) {
-        if (s->used + it->bytes <= s->budget) {
+        // a pooled context of the same shape, else room made by releasing
+        // pooled contexts of other shapes, least recently pooled first
+        oa_ctx* reuse = pool_take(s, it->key);
+        while (!reuse && s->used + it->bytes > s->budget && !s->pool.empty()) {
+          s->used -= s->pool.front().bytes;
+          release.push_back(s->pool.front().ctx);
+          s->pool.pop_front();
+        }
+        if (reuse || s->used + it->bytes <= s->budget) {
           Active* a = new Active();
           a->job = std::move(*it);
+          a->ctx = reuse;
+          a->from_pool = reuse != nullptr;
           it = s->queue.erase(it);
           s->used += a->job.bytes;
           s->n_active++;
@@ -202,6 +312,8 @@ void scheduler(oa_server* s) {
         }
       }
     }
+    for (oa_ctx* c : release) oa_destroy(c);
+    release.clear();
     for (Active* a : admitted) {
       a->t_admit = clk::now();
       if (start(s, a))
@@ -273,6 +385,13 @@ void scheduler(oa_server* s) {
     }
     if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
   }
+  std::list<Pooled> pool;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    pool.swap(s->pool);
+    for (const Pooled& p : pool) s->used -= p.bytes;
+  }
+  for (Pooled& p : pool) oa_destroy(p.ctx);
 }
 
 }  // namespace
@@ -339,6 +458,7 @@ oa_status oa_server_submit(oa_server* s, const oa_request* req) {
     const size_t N = (size_t)sys.grid[0] * sys.grid[1] * sys.grid[2];
     j.vks.assign(req->vks, req->vks + N);
   }
+  j.key = shape_key(j, j.r.opt.precision);
   j.t_submit = clk::now();
   {
     std::lock_guard<std::mutex> lk(s->mu);

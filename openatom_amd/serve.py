@@ -23,7 +23,7 @@ class OaResult(ctypes.Structure):
                 ("eelec", ctypes.c_double), ("fnorm", ctypes.c_double),
                 ("queue_ms", ctypes.c_double), ("run_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("hbm_bytes", ctypes.c_size_t),
-                ("fp64_steps", ctypes.c_int)]
+                ("fp64_steps", ctypes.c_int), ("reused", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -195,3 +195,25 @@ def test_concurrent_sizing_and_creation(gpu):
     for t in th:
         t.join()
     assert not errors
+
+
+def test_pooled_context_equals_fresh(gpu):
+    """A request that takes over the pooled context of an earlier request of
+    the same shape (other coefficients, left in the pool when it finished)
+    gives the result of a fresh context, bit for bit; a request of another
+    shape does not take it."""
+    ss = systems()
+    s = ss[3]
+    warm = dict(s, psi=np.ascontiguousarray(s["psi"][::-1]))  # same shape, states reversed
+    with serve.Server(max_active=1) as srv:
+        fresh = run(srv, [("x", s)])["x"]
+    with serve.Server(max_active=1) as srv:
+        res = run(srv, [("warm", warm), ("other", ss[1]), ("x", s)])
+    print(f"[serve pool] fresh: {fresh['steps']} steps E {fresh['eelec']:.10f}; pooled: "
+          f"{res['x']['steps']} steps E {res['x']['eelec']:.10f}, reused "
+          f"{[res[k]['reused'] for k in ('warm', 'other', 'x')]}", flush=True)
+    assert fresh["reused"] == 0 and res["warm"]["reused"] == 0 and res["other"]["reused"] == 0
+    assert res["x"]["reused"] == 1
+    assert res["x"]["status"] == 0 and res["x"]["steps"] == fresh["steps"]
+    assert res["x"]["eelec"] == fresh["eelec"]
+    assert np.array_equal(res["x"]["psi"], fresh["psi"])
