@@ -211,42 +211,75 @@ def _serving_system(args):
     return system.build(system.serving_workload(nmol, ecut), seed=seed)
 
 
-def serving_mix(nreq, rate, seed, max_mol, workers=8):
+def _noop(i):
+    return i
+
+
+def _one_blas_thread():
+    """Worker initializer: one BLAS thread per builder process (the workers
+    run side by side on the host cores)."""
+    try:
+        from threadpoolctl import threadpool_limits
+        threadpool_limits(1)
+    except ImportError:
+        pass
+
+
+class ServingMix:
     """The request mix of the serving workload: water clusters of 4..max_mol
     molecules (log-uniform), 4 states per molecule, Ecut uniform in
-    {50, 60, 70} Ry, Poisson arrivals at `rate` per second.  Systems are
-    built in worker processes before any GPU work."""
-    import numpy as np
-    from concurrent.futures import ProcessPoolExecutor
-    import multiprocessing as mp
-    rng = np.random.default_rng(seed)
-    nmol = np.floor(np.exp(rng.uniform(np.log(4), np.log(max_mol + 1), nreq))).astype(int)
-    nmol = np.clip(nmol, 4, max_mol)
-    ecut = rng.choice([50.0, 60.0, 70.0], nreq)
-    arrive = np.cumsum(rng.exponential(1.0 / rate, nreq))
-    jobs = [(int(n), float(e), int(seed * 1000 + i)) for i, (n, e) in enumerate(zip(nmol, ecut))]
-    with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("fork")) as ex:
-        systems = list(ex.map(_serving_system, jobs))
-    return systems, arrive
+    {50, 60, 70} Ry, Poisson arrivals at `rate` per second.  Each request's
+    system is built when it is needed, in worker processes forked before any
+    GPU work, and dropped once submitted (the server copies its inputs): a
+    2,000-request mix would hold tens of GB of coefficients at once."""
+
+    def __init__(self, nreq, rate, seed, max_mol, workers=12):
+        import numpy as np
+        from concurrent.futures import ProcessPoolExecutor
+        import multiprocessing as mp
+        rng = np.random.default_rng(seed)
+        nmol = np.floor(np.exp(rng.uniform(np.log(4), np.log(max_mol + 1), nreq))).astype(int)
+        self.nmol = [int(n) for n in np.clip(nmol, 4, max_mol)]
+        ecut = rng.choice([50.0, 60.0, 70.0], nreq)
+        self.arrive = np.cumsum(rng.exponential(1.0 / rate, nreq))
+        self.jobs = [(n, float(e), int(seed * 1000 + i))
+                     for i, (n, e) in enumerate(zip(self.nmol, ecut))]
+        self.ex = ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("fork"),
+                                      initializer=_one_blas_thread)
+        list(self.ex.map(_noop, range(4 * workers)))  # every worker forked now, before the GPU
+
+    def __len__(self):
+        return len(self.jobs)
+
+    def build(self, i):
+        return self.ex.submit(_serving_system, self.jobs[i])
+
+    def close(self):
+        self.ex.shutdown()
 
 
-def serving_run(systems, arrive, precision, f_tol, max_steps, budget, max_active, opts):
-    """Submit each request at its arrival time, collect results as they
-    finish.  Latency = finish - arrival (queueing included)."""
+def serving_run(mix, arrive, precision, f_tol, max_steps, budget, max_active, opts, ahead=64):
+    """Submit each request at its arrival time (or as soon as its system is
+    built, when that is later), collect results as they finish.  Latency =
+    finish - arrival (queueing and any build delay included)."""
     import numpy as np
     from openatom_amd import serve
-    n = len(systems)
+    n = len(mix)
     res = {}
     lag = np.zeros(n)
+    futs = {}
     with serve.Server(hbm_budget=budget, max_active=max_active) as srv:
         t0 = time.perf_counter()
         i = 0
         t_log = 0.0
         while len(res) < n:
+            for k in range(i, min(n, i + ahead)):
+                if k not in futs:
+                    futs[k] = mix.build(k)
             now = time.perf_counter() - t0
-            while i < n and arrive[i] <= now:
-                srv.submit(systems[i], i, f_tol=f_tol, max_steps=max_steps, precision=precision,
-                           **opts)
+            while i < n and arrive[i] <= now and futs[i].done():
+                srv.submit(futs.pop(i).result(), i, f_tol=f_tol, max_steps=max_steps,
+                           precision=precision, **opts)
                 lag[i] = time.perf_counter() - t0 - arrive[i]
                 i += 1
             for r in srv.poll():
@@ -255,14 +288,18 @@ def serving_run(systems, arrive, precision, f_tol, max_steps, budget, max_active
                 t_log = now
                 st = srv.stats()
                 log(f"[serve] {precision} t={now:.0f}s submitted {i}/{n} done {len(res)} "
-                    f"active {int(st['active'])} queued {int(st['queued'])}")
+                    f"active {int(st['active'])} queued {int(st['queued'])} "
+                    f"device {st['hbm_bytes'] / 2**30:.1f} GiB")
             time.sleep(5e-4)
         wall = time.perf_counter() - t0
         st = srv.stats()
     lat = np.array([res[k]["total_ms"] + lag[k] * 1e3 for k in range(n)])
     steps = np.array([res[k]["steps"] for k in range(n)])
     ok = [k for k in range(n) if res[k]["status"] == 0]
+    for r in res.values():
+        r.pop("psi", None)
     return dict(res=res, summary={
+        "reused_contexts": int(sum(r["reused"] for r in res.values())),
         "solves_per_s": round(len(ok) / wall, 3),
         "p50_ms": round(float(np.percentile(lat, 50)), 2),
         "p99_ms": round(float(np.percentile(lat, 99)), 2),
@@ -278,27 +315,28 @@ def serving_leg(args, mix, precisions):
     """The serving workload in each precision, with the largest energy
     difference per atom between precisions over requests both converged."""
     import numpy as np
-    systems, arrive = mix
-    sopts = dict(dt=20.0, use_cg=1, ns_mode=1)
-    serving = {"workload": f"{len(systems)} independent minimisations of water clusters "
+    arrive = mix.arrive
+    # fixed ions: V_KS = V_xc + V_H + V_loc rebuilt from every step's rho
+    sopts = dict(dt=20.0, use_cg=1, ns_mode=1, scf=True)
+    serving = {"workload": f"{len(mix)} independent fixed-ion minimisations of water clusters "
                            f"(4-{args.serve_max_mol} H2O log-uniform, 4 states/molecule, "
-                           f"Ecut in {{50,60,70}} Ry), Poisson arrivals at "
-                           f"{args.serve_rate:g}/s, conjugate gradient to |F| <= "
+                           f"Ecut in {{50,60,70}} Ry, V_KS from each step's rho), Poisson "
+                           f"arrivals at {args.serve_rate:g}/s, conjugate gradient to |F| <= "
                            f"{args.serve_ftol:g}",
                "nmol_hist": {str(k): int(v) for k, v in zip(*np.unique(
-                   [int(s["nmol"]) for s in systems], return_counts=True))},
+                   mix.nmol, return_counts=True))},
                "max_active": args.serve_active, "hbm_budget_gib": args.serve_budget_gib}
     runs_s = {}
     for prec in precisions:
-        log(f"[bench] serving ({prec}): {len(systems)} requests")
-        r = serving_run(systems, arrive, prec, args.serve_ftol, args.serve_max_steps,
+        log(f"[bench] serving ({prec}): {len(mix)} requests")
+        r = serving_run(mix, arrive, prec, args.serve_ftol, args.serve_max_steps,
                         int(args.serve_budget_gib * 2**30), args.serve_active, sopts)
         log(f"[bench] serving {prec}: {r['summary']}")
         serving[prec] = r["summary"]
         runs_s[prec] = r["res"]
     if "mixed" in runs_s and "fp64" in runs_s:
         de = [abs(runs_s["mixed"][k]["eelec"] - runs_s["fp64"][k]["eelec"]) /
-              (3 * systems[k]["nmol"]) for k in range(len(systems))
+              (3 * mix.nmol[k]) for k in range(len(mix))
               if runs_s["mixed"][k]["status"] == 0 and runs_s["fp64"][k]["status"] == 0]
         serving["max_dE_per_atom_mixed_vs_fp64"] = max(de) if de else None
     return serving
@@ -379,12 +417,13 @@ def main():
     mix = None
     if args.serve_requests > 0:
         log(f"[bench] building the serving mix: {args.serve_requests} requests")
-        mix = serving_mix(args.serve_requests, args.serve_rate, args.seed, args.serve_max_mol)
+        mix = ServingMix(args.serve_requests, args.serve_rate, args.seed, args.serve_max_mol)
     if args.serve_only:
         if mix is None:
             sys.exit("--serve-only needs --serve-requests > 0")
-        print(json.dumps({"serving": serving_leg(args, mix,
-                                                 args.serve_precisions.split(","))}), flush=True)
+        out = serving_leg(args, mix, args.serve_precisions.split(","))
+        mix.close()
+        print(json.dumps({"serving": out}), flush=True)
         return
 
     if args.copy_calib_mb > 0:
@@ -470,6 +509,7 @@ def main():
     serving = None
     if mix is not None:
         serving = serving_leg(args, mix, ("fp64", "mixed"))
+        mix.close()
         del mix
 
     cpu = None

@@ -117,10 +117,9 @@ def gamma_weights(kx):
 
 def gamma_overlap(psi, kx):
     """S_ij = sum_p w_p Re(conj psi_i psi_j): the full-sphere overlap."""
-    w = gamma_weights(kx)
-    A = psi * np.sqrt(w)[None, :]
-    Ar = np.concatenate([A.real, A.imag], axis=1)
-    return Ar @ Ar.T
+    w = np.sqrt(gamma_weights(kx))[None, :]
+    Ar, Ai = psi.real * w, psi.imag * w
+    return Ar @ Ar.T + Ai @ Ai.T
 
 
 def initial_coeffs(kx, ky, kz, hmat, nstates, seed, norm=2.0):
@@ -136,10 +135,14 @@ def initial_coeffs(kx, ky, kz, hmat, nstates, seed, norm=2.0):
     psi[:, red] = np.conj(psi[:, par])
     g0 = np.where((np.asarray(kx) == 0) & (np.asarray(ky) == 0) & (np.asarray(kz) == 0))[0]
     psi[:, g0] = psi[:, g0].real
+    from scipy.linalg import solve_triangular
     for _ in range(2):
         S = gamma_overlap(psi, kx)
         Lc = np.linalg.cholesky(S)
-        psi = np.linalg.solve(Lc, psi)
+        # Lc is real: two real triangular solves instead of a complex LU solve
+        re = solve_triangular(Lc, psi.real, lower=True, check_finite=False)
+        im = solve_triangular(Lc, psi.imag, lower=True, check_finite=False)
+        psi = re + 1j * im
     return psi * np.sqrt(norm)
 
 
