@@ -142,7 +142,7 @@ def initial_coeffs(kx, ky, kz, hmat, nstates, seed, norm=2.0):
         # Lc is real: two real triangular solves instead of a complex LU solve
         re = solve_triangular(Lc, psi.real, lower=True, check_finite=False)
         im = solve_triangular(Lc, psi.imag, lower=True, check_finite=False)
-        psi = re + 1j * im
+        psi = np.ascontiguousarray(re + 1j * im)  # the solves return Fortran order
     return psi * np.sqrt(norm)
 
 
