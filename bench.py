@@ -405,11 +405,22 @@ def main():
                     help="run only the serving workload (--serve-requests requests, each "
                          "precision of --serve-precisions) and print its JSON line")
     ap.add_argument("--serve-precisions", default="fp64,mixed")
+    ap.add_argument("--traffic-from", nargs="+", default=None,
+                    help="rocprofv3 --pmc output directories (request-size counters): write "
+                         "profiles/traffic_<workload>.json from them and exit (no GPU work)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the step on a single "
               "MI355X (run with --gpus 1).", file=sys.stderr)
         sys.exit(2)
+
+    if args.traffic_from:
+        def prec(name):
+            return "mixed" if "<float" in name else ("fp64" if "<double" in name else None)
+        t = traffic_from_counters(args.traffic_from, os.path.join(
+            ROOT, "profiles", f"traffic_{args.workload}.json"), prec)
+        print(json.dumps({k: t[k] for k in ("fp64", "mixed")}), flush=True)
+        return
 
     import numpy as np
     from openatom_amd import system
@@ -580,6 +591,85 @@ def main():
         "serving": serving,
     }
     print(json.dumps(out), flush=True)
+
+
+def traffic_from_counters(dirs, out_path, precision_of):
+    """HBM bytes per launch of each kernel from rocprofv3 request-size
+    counters, no calibration: reads = 32 RDREQ_32B + 64 RDREQ_64B + 128
+    RDREQ_128B, writes = 64 WRREQ_64B + 32 (WRREQ - WRREQ_64B) (TCC/EA
+    interface, summed over the TCC instances).  Per kernel: the median of its
+    last three dispatches (the timed steps, after the autotuner's trials).
+    The runtime's copy kernel of a known size checks the byte counts.
+    precision_of(kernel name) -> "fp64" | "mixed" | None."""
+    import csv
+    import numpy as np
+    per = {}
+    for d in dirs:
+        with open(os.path.join(d, "run_counter_collection.csv")) as f:
+            for r in csv.DictReader(f):
+                k = (r["Kernel_Name"], int(r["Dispatch_Id"]))
+                per.setdefault(k, {})[r["Counter_Name"]] = float(r["Counter_Value"])
+    # reads and writes come from separate passes: match dispatches of a kernel by order
+    by_kernel = {}
+    for (name, did), c in sorted(per.items(), key=lambda x: x[0][1]):
+        by_kernel.setdefault(name, []).append(c)
+    out = {"unit": "bytes per launch (HBM, TCC/EA request sizes: reads 32/64/128 B, "
+                   "writes 32/64 B; no calibration factor)",
+           "commands": ["cd /tmp && export TMPDIR=/tmp"] + [
+               f"timeout -s KILL 120 rocprofv3 --pmc {c} --output-format csv -d <out>/{d} -o run "
+               "-- python3 <repo>/bench.py --steps 2 --warmup 1 --no-cpu --no-graph "
+               "--serve-requests 0 --copy-calib-mb 256"
+               for c, d in (("TCC_EA0_RDREQ_sum TCC_EA0_RDREQ_32B_sum TCC_EA0_RDREQ_64B_sum "
+                             "TCC_EA0_RDREQ_128B_sum", "pmc_rd"),
+                            ("TCC_EA0_WRREQ_sum TCC_EA0_WRREQ_64B_sum", "pmc_wr"))] + [
+               "python3 <repo>/bench.py --traffic-from <out>/pmc_rd <out>/pmc_wr"],
+           "per_launch": "median of each kernel's last three dispatches (the timed steps, "
+                         "after the autotuner's trials)",
+           "fp64": {}, "mixed": {}, "detail": {}}
+    for name, cs in by_kernel.items():
+        rd = [c for c in cs if "TCC_EA0_RDREQ_sum" in c]
+        wr = [c for c in cs if "TCC_EA0_WRREQ_sum" in c]
+        rb = [32 * c["TCC_EA0_RDREQ_32B_sum"] + 64 * c["TCC_EA0_RDREQ_64B_sum"] +
+              128 * c["TCC_EA0_RDREQ_128B_sum"] for c in rd[-3:]]
+        other = [c["TCC_EA0_RDREQ_sum"] - c["TCC_EA0_RDREQ_32B_sum"] - c["TCC_EA0_RDREQ_64B_sum"]
+                 - c["TCC_EA0_RDREQ_128B_sum"] for c in rd[-3:]]
+        wb = [64 * c["TCC_EA0_WRREQ_64B_sum"] + 32 * (c["TCC_EA0_WRREQ_sum"] -
+                                                       c["TCC_EA0_WRREQ_64B_sum"])
+              for c in wr[-3:]]
+        if not rb or not wb:
+            continue
+        det = {"read_bytes": float(np.median(rb)), "write_bytes": float(np.median(wb)),
+               "reads_of_other_sizes": float(np.median(other)), "launches": len(rd)}
+        out["detail"][name] = det
+        prec = precision_of(name)
+        key = STAGE_OF_KERNEL(name)
+        if prec and key:
+            out[prec][key] = int(det["read_bytes"] + det["write_bytes"])
+        if "copyBuffer" in name:  # the known-size check: the largest runtime copy
+            big = max(range(len(rd)), key=lambda i: rd[i]["TCC_EA0_RDREQ_sum"])
+            c, w = rd[big], wr[big] if big < len(wr) else wr[-1]
+            out["copy_check"] = {
+                "read_bytes": 32 * c["TCC_EA0_RDREQ_32B_sum"] + 64 * c["TCC_EA0_RDREQ_64B_sum"]
+                + 128 * c["TCC_EA0_RDREQ_128B_sum"],
+                "write_bytes": 64 * w["TCC_EA0_WRREQ_64B_sum"] + 32 * (
+                    w["TCC_EA0_WRREQ_sum"] - w["TCC_EA0_WRREQ_64B_sum"]),
+                "copy_bytes": 256 << 20}
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+    return out
+
+
+def STAGE_OF_KERNEL(name):
+    """The bench stage a kernel's bytes belong to (the roofline's names)."""
+    if "k2_zfwd" in name:
+        return "k_zfwd"
+    if "k2_zbwd" in name:
+        return "k_zbwd"
+    if "k3_g2r" in name:
+        return "k_plane_g2r"
+    if "k3_r2g" in name:
+        return "k_plane_r2g"
+    return None
 
 
 def copy_calibration(mib):

@@ -35,12 +35,13 @@ def serving_workload(nmol, ecut_ry):
     """Water box of the request mix: nmol molecules at the density of the
     32-molecule box (side 9.86 (nmol/32)^(1/3) A), four doubly occupied states
     per molecule, and the smallest compiled cubic grid that holds the density
-    sphere (N >= 4 kmax + 1, kmax the largest |k| component of the
-    wavefunction sphere)."""
+    sphere |G|^2 <= 4 Ecut: N >= 2 kr + 1, kr the largest integer component
+    inside it, floor(2 sqrt(Ecut) L / 2 pi) in a cube of side L (twice the
+    wavefunction sphere's kmax can fall one short of it)."""
     box = 9.86 * (nmol / 32.0) ** (1.0 / 3.0)
     L = box * BOHR_PER_ANG
-    kmax = int(np.floor(np.sqrt(ecut_ry) * L / (2.0 * np.pi)))
-    need = 4 * kmax + 1
+    kr = int(np.floor(2.0 * np.sqrt(ecut_ry) * L / (2.0 * np.pi)))
+    need = 2 * kr + 1
     grid = next((n for n in FFT_SIZES if n >= need), FFT_SIZES[-1])
     return dict(nmol=int(nmol), box_ang=box, nstates=4 * int(nmol), ecut_ry=float(ecut_ry),
                 grid=grid)

@@ -71,3 +71,20 @@ def test_duplicate_tags_keep_their_own_arrays():
     vals = sorted(float(r["psi"].real.flat[0]) for r in res)
     assert vals == [1.0, 2.0]
     assert not srv._pending
+
+
+def test_serving_grids_hold_the_density_sphere():
+    """Every shape of the serving mix (4-64 molecules, 50/60/70 Ry) gets a
+    grid with N >= 2 kr + 1, kr the largest component of the density sphere
+    |G|^2 <= 4 Ecut found by enumerating it (the library refuses smaller
+    grids), and the smallest compiled size that does."""
+    from openatom_amd import _lib, system
+    for nmol in range(4, 65):
+        for e in (50.0, 60.0, 70.0):
+            w = system.serving_workload(nmol, e)
+            L = w["box_ang"] * system.BOHR_PER_ANG
+            kx, ky, kz = _lib.kvecs(4.0 * e, np.diag([L, L, L]))
+            need = 2 * int(max(kx.max(), np.abs(ky).max(), np.abs(kz).max())) + 1
+            assert w["grid"] >= need, (nmol, e, w["grid"], need)
+            smaller = [n for n in system.FFT_SIZES if need <= n < w["grid"]]
+            assert not smaller, (nmol, e, w["grid"], smaller)
