@@ -75,6 +75,10 @@ def plane_bytes(stage, geo, csz, nst, launch):
     return 2 * nst * lines + launch["pcp"] * n * 8
 
 
+# options every benchmark context takes (set from the command line)
+RUN_OPTS = {}
+
+
 def run_precision(sysd, precision, steps, warmup, device=0, use_graph=0, vks=True, autotune=1,
                   nonlocal_kb=False, prof_steps=None, gemm_split=None):
     """vks: V(r) = V_xc + V_H + V_loc rebuilt from each step's rho on the
@@ -84,7 +88,9 @@ def run_precision(sysd, precision, steps, warmup, device=0, use_graph=0, vks=Tru
     from openatom_amd.step import CPContext
     # The reference step has no ||Psi^H Psi - I|| evaluation (Ortho checks the
     # T iterates); it is measured once after the timed steps instead.
-    extra = {} if gemm_split is None else {"gemm_split": int(gemm_split)}
+    extra = dict(RUN_OPTS)
+    if gemm_split is not None:
+        extra["gemm_split"] = int(gemm_split)
     c = CPContext(sysd["grid"], sysd["hmat"], sysd["ecut_ry"], sysd["nstates"], sysd["kx"],
                   sysd["ky"], sysd["kz"], precision=precision, device=device, dt=5.0,
                   ns_tol=1e-15 if precision == "fp64" else 1e-14, ns_max_iter=30,
@@ -405,6 +411,9 @@ def main():
                     help="run only the serving workload (--serve-requests requests, each "
                          "precision of --serve-precisions) and print its JSON line")
     ap.add_argument("--serve-precisions", default="fp64,mixed")
+    ap.add_argument("--ns-mode", type=int, default=None,
+                    help="force the Newton-Schulz kernel (ns_mode option; default: autotuned). "
+                         "1 = one launch per product, no cooperative launch")
     ap.add_argument("--traffic-from", nargs="+", default=None,
                     help="rocprofv3 --pmc output directories (request-size counters): write "
                          "profiles/traffic_<workload>.json from them and exit (no GPU work)")
@@ -414,6 +423,8 @@ def main():
               "MI355X (run with --gpus 1).", file=sys.stderr)
         sys.exit(2)
 
+    if args.ns_mode is not None:
+        RUN_OPTS["ns_mode"] = args.ns_mode
     if args.traffic_from:
         def prec(name):
             return "mixed" if "<float" in name else ("fp64" if "<double" in name else None)
@@ -660,7 +671,9 @@ def traffic_from_counters(dirs, out_path, precision_of):
 
 
 def STAGE_OF_KERNEL(name):
-    """The bench stage a kernel's bytes belong to (the roofline's names)."""
+    """The bench stage a kernel's bytes belong to (the roofline's names).
+    k3_r2g's eighth template argument is RHO: true in the fixed-V step's
+    R->G pass, which also accumulates the density."""
     if "k2_zfwd" in name:
         return "k_zfwd"
     if "k2_zbwd" in name:
@@ -668,7 +681,8 @@ def STAGE_OF_KERNEL(name):
     if "k3_g2r" in name:
         return "k_plane_g2r"
     if "k3_r2g" in name:
-        return "k_plane_r2g"
+        targs = name[name.index("<") + 1:].split(", ")
+        return "k_plane_r2g_rho" if targs[7] == "true" else "k_plane_r2g"
     return None
 
 

@@ -13,8 +13,8 @@ namespace oa3 {
 #define OA_DECL(n, r1, r2)                                                                  \
   hipError_t plane3_n##n(int kind, bool fp32, const oa::FftCtx& c, hipStream_t st);        \
   size_t plane3_smem_n##n(int npx, bool fp32);                                              \
-  hipError_t plane3_info_n##n(int npx, bool fp32, int* rowparts, int* occ_g2r, int* occ_r2g,  \
-                              int* rho_ok);
+  hipError_t plane3_info_n##n(int npx, bool fp32, bool prune, int* rowparts, int* occ_g2r,    \
+                              int* occ_r2g, int* rho_ok);
 OA_FFT2_SIZES(OA_DECL)
 #undef OA_DECL
 }  // namespace oa3
@@ -57,12 +57,12 @@ size_t plane3_smem(int n, int npx, bool fp32) {
   }
 }
 
-hipError_t plane3_info(int n, int npx, bool fp32, int* rowparts, int* occ_g2r, int* occ_r2g,
-                       int* rho_ok) {
+hipError_t plane3_info(int n, int npx, bool fp32, bool prune, int* rowparts, int* occ_g2r,
+                       int* occ_r2g, int* rho_ok) {
   switch (n) {
 #define OA_CASE(nn, r1, r2) \
   case nn:                  \
-    return oa3::plane3_info_n##nn(npx, fp32, rowparts, occ_g2r, occ_r2g, rho_ok);
+    return oa3::plane3_info_n##nn(npx, fp32, prune, rowparts, occ_g2r, occ_r2g, rho_ok);
     OA_FFT2_SIZES(OA_CASE)
 #undef OA_CASE
     default:

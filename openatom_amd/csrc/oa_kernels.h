@@ -49,6 +49,7 @@ struct FftCtx {
   int nchunk_r;           // state chunks of the R->G plane pass ...
   int cs_r;               // ... and states per chunk
   int psir_state;         // state whose psi(r) the G->R plane pass writes to psir
+  int prune;              // plane passes pruned to |kx|, |ky| <= (N - 1) / 4
 };
 
 // two-factor register FFT engine (oa_fft2_*.hip): kind 0 z fwd, 1 y fwd,
@@ -64,7 +65,7 @@ int fft2_ntr(int n);  // transforms per workgroup for length n, 0 if not compile
 hipError_t plane3_launch(int kind, int n, bool fp32, const FftCtx& c, hipStream_t st);
 size_t plane3_smem(int n, int nplane_x, bool fp32);  // 0 if n is not compiled
 // row parts of the G->R grid and resident workgroups per CU of both passes
-hipError_t plane3_info(int n, int nplane_x, bool fp32, int* rowparts, int* occ_g2r,
+hipError_t plane3_info(int n, int nplane_x, bool fp32, bool prune, int* rowparts, int* occ_g2r,
                        int* occ_r2g, int* rho_ok);
 
 // fused y+x plane passes (one z-plane, a chunk of states per workgroup)

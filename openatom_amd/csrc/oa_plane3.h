@@ -63,10 +63,32 @@ __device__ __forceinline__ int signed_k(int i, int n) { return (i <= (n - 1) / 2
 //         4 x c2r pass 2 (and what follows it in registers: rho, V psi and
 //         the r2c pass 1), 8 r2c pass 2, 16 y_bwd pass 1
 //   TW2 : twiddle tables indexed [t][j] (else one table indexed t*j)
-template <int CYP_, int YA_, int EA_, int REP_, int MAPS_, bool TW2_>
+//   PR  : prune the plane transforms to the sphere (Sph below): every
+//         populated kx and ky satisfies |k| <= (N - 1) / 4
+template <int CYP_, int YA_, int EA_, int REP_, int MAPS_, bool TW2_, bool PR_ = false>
 struct Lay {
   static constexpr int CYP = CYP_, YA = YA_, EA = EA_, REP = REP_, MAPS = MAPS_;
-  static constexpr bool TW2 = TW2_;
+  static constexpr bool TW2 = TW2_, PR = PR_;
+};
+
+// Pruning to the sphere.  A grid that holds the density sphere |G|^2 <= 4
+// Ecut has N >= 2 kr + 1 with kr >= 2 kmax, so the wavefunction components
+// satisfy |k| <= KM = (N - 1) / 4.  A radix pass reading (or writing) index
+// i = j + t*STRIDE, j < STRIDE, touches only zeros (or unneeded outputs) for
+// every j when KM < t*STRIDE and t*STRIDE + STRIDE - 1 < N - KM, i.e. for t in
+// [T0, T1].  Those inputs are set to zero at compile time and those outputs
+// are not stored, so the compiler drops their arithmetic (the plane
+// translation units are built with -fno-signed-zeros, which lets x + 0 fold
+// to x; only the sign of zero results can differ).
+template <int N, int STRIDE>
+struct Sph {
+  static constexpr int KM = (N - 1) / 4;
+  static constexpr int T0 = KM / STRIDE + 1;
+  static constexpr int T1 = (N - KM - STRIDE) / STRIDE;
+  template <class L>
+  static __device__ __forceinline__ constexpr bool cut(int t) {
+    return L::PR && t >= T0 && t <= T1;
+  }
 };
 
 enum : int { kMapYf2 = 1, kMapXc1 = 2, kMapXc2 = 4, kMapRx2 = 8, kMapYb1 = 16 };
@@ -178,7 +200,8 @@ __device__ __forceinline__ void y_fwd(const cpx<T>* __restrict__ zs, cpx<T>* P, 
       for (int t = 0; t < Y1; t++) {
         const int y = j + t * Y2;
         const int kyv = signed_k(y, NY);
-        v[t] = (kyv >= lo && kyv <= hi) ? src[y] : cpx<T>{(T)0, (T)0};
+        v[t] = (!Sph<NY, Y2>::template cut<L>(t) && kyv >= lo && kyv <= hi) ? src[y]
+                                                                          : cpx<T>{(T)0, (T)0};
       }
       Dft<T, Y1, -1>::run(v);
       cpx<T>* dst = P + kx * CY + j * (Y1 + L::YA);
@@ -246,7 +269,7 @@ __device__ __forceinline__ void y_bwd(const cpx<T>* P, cpx<T>* E, const cpx<T>* 
       for (int t = 0; t < Y2; t++) {
         const int y = j + t * Y1;
         const int kyv = signed_k(y, NY);
-        if (kyv >= lo && kyv <= hi) dst[y] = u[t];
+        if (!Sph<NY, Y1>::template cut<L>(t) && kyv >= lo && kyv <= hi) dst[y] = u[t];
       }
     }
     __syncthreads();
@@ -294,7 +317,8 @@ __device__ __forceinline__ void x_c2r(const cpx<T>* P, cpx<T>* E, const cpx<T>* 
       cpx<T> Z = {(T)0, (T)0};
       int kk = -1;
       bool mirror = false;
-      if (k < npx) {
+      if (Sph<NX, X2>::template cut<L>(t)) {
+      } else if (k < npx) {
         kk = k;
       } else if (k > NX - npx) {
         kk = NX - k;
@@ -408,8 +432,8 @@ __global__ __launch_bounds__(NT) void k3_g2r(FftCtx c) {
 
 // R->G plane pass.  Grid (nz, state chunks).  VREG: V(r) of the thread's
 // pass-2 positions for all rounds is held in registers across the chunk.
-// FP32 up to N = 100: at most 168 VGPRs, three waves per SIMD (measured
-// faster than two; larger planes would spill).  RHO: also accumulate rho
+// FP32 up to N = 100: at most 168 VGPRs (154 at N = 100, 144 pruned), three
+// waves per SIMD (measured faster than two; larger planes would spill).  RHO: also accumulate rho
 // partials of the chunk from the recomputed psi(r), for a V(r) that does
 // not depend on this step's rho (then no separate G->R pass is needed).
 template <typename T, int X1, int X2, int Y1, int Y2, int NT, bool VREG, bool RHO, class L>
@@ -522,7 +546,8 @@ k3_r2g(FftCtx c) {
         for (int t = 1; t < X1; t++) a[t] = cm(a[t], tw2<T, -1, X2, L>(TW, t, jf));
         Dft<T, X1, -1>::run(a);
 #pragma unroll
-        for (int t = 0; t < X1; t++) row[t * (X2 + L::EA)] = a[t];
+        for (int t = 0; t < X1; t++)
+          if (!Sph<NX, X2>::template cut<L>(t)) row[t * (X2 + L::EA)] = a[t];
       }
       __syncthreads();
       // split the two real rows, keep kx < nplane_x, conjugated, into P
@@ -579,19 +604,22 @@ constexpr int kPlaneThreads = 256;
 // us, identical results.  Padding grows the LDS a little, so a launch uses
 // PadLay only where that keeps the workgroups per CU of OldLay (FP64 at
 // 100^2 planes sits 8 bytes under half a CU's LDS with OldLay).
-template <int N>
-using OldLay = Lay<(N % 2 == 0) ? 1 : 0, 0, 0, 0, 0, false>;
-template <int N>
-using PadLay = Lay<(N % 2 == 0) ? 5 : 4, 0, 0, (N % 2 == 0) ? 5 : 4, 31, true>;
+template <int N, bool PR = false>
+using OldLay = Lay<(N % 2 == 0) ? 1 : 0, 0, 0, 0, 0, false, PR>;
+template <int N, bool PR = false>
+using PadLay = Lay<(N % 2 == 0) ? 5 : 4, 0, 0, (N % 2 == 0) ? 5 : 4, 31, true, PR>;
 
 // x rounds per k3_g2r workgroup: the rho accumulators are 2*X2 doubles per
-// round and thread; FP64 keeps one round, FP32 two.  When the LDS of one
+// round and thread; FP32 keeps two rounds, FP64 two when pruned to the
+// sphere (220 VGPRs at N = 100, no spills) and one otherwise, which makes
+// two row parts at N = 100 that each read zbuf and repeat the y-FFT of the
+// plane.  When the LDS of one
 // workgroup exceeds half a CU's (one workgroup per CU, four waves), the
 // register file has room for every round and one workgroup covers the whole
 // plane ("wide"), so the y-FFT of a plane is not repeated per row part.
-template <typename T>
+template <typename T, class L>
 constexpr int row_rounds() {
-  return sizeof(T) == 8 ? 1 : 2;
+  return sizeof(T) == 8 && !L::PR ? 1 : 2;
 }
 
 constexpr size_t kCuLds = 160 * 1024;
@@ -625,9 +653,9 @@ hipError_t plane3_info_l(int npx, int* rowparts, int* occ_g2r, int* occ_r2g, int
   constexpr bool VREG = sizeof(T) == 4 && S::ROUNDS * 2 * X2 <= 48;
   const size_t smem = plane3_smem_l<T, X1, X2, Y1, Y2, NT, L>(npx);
   const bool wide = g2r_wide<T, X1, X2, Y1, Y2, NT, L>(npx);
-  *rowparts = wide ? 1 : (S::ROUNDS + row_rounds<T>() - 1) / row_rounds<T>();
+  *rowparts = wide ? 1 : (S::ROUNDS + row_rounds<T, L>() - 1) / row_rounds<T, L>();
   const void* g = wide ? (const void*)k3_g2r<T, X1, X2, Y1, Y2, NT, S::ROUNDS, false, L>
-                       : (const void*)k3_g2r<T, X1, X2, Y1, Y2, NT, row_rounds<T>(), false, L>;
+                       : (const void*)k3_g2r<T, X1, X2, Y1, Y2, NT, row_rounds<T, L>(), false, L>;
   const void* r = (const void*)k3_r2g<T, X1, X2, Y1, Y2, NT, VREG, false, L>;
   *rho_ok = r2g_rho_ok<X1, X2, NT>() ? 1 : 0;
   hipError_t e;
@@ -642,7 +670,7 @@ template <typename T, int X1, int X2, int Y1, int Y2, class L>
 hipError_t launch_plane3_l(int kind, const FftCtx& c, hipStream_t st) {
   constexpr int NT = kPlaneThreads;
   using S = PS<X1, X2, Y1, Y2, NT, L>;
-  constexpr int kRowRounds = row_rounds<T>();
+  constexpr int kRowRounds = row_rounds<T, L>();
   constexpr int RPn = (S::ROUNDS + kRowRounds - 1) / kRowRounds;
   constexpr bool VREG = sizeof(T) == 4 && S::ROUNDS * 2 * X2 <= 48;
   const size_t smem = plane3_smem_l<T, X1, X2, Y1, Y2, NT, L>(c.g.nplane_x);
@@ -689,19 +717,31 @@ hipError_t launch_plane3_l(int kind, const FftCtx& c, hipStream_t st) {
 }
 
 template <typename T, int X1, int X2, int Y1, int Y2>
-hipError_t plane3_info(int npx, int* rowparts, int* occ_g2r, int* occ_r2g, int* rho_ok) {
-  return use_pad<T, X1, X2, Y1, Y2, kPlaneThreads>(npx)
-             ? plane3_info_l<T, X1, X2, Y1, Y2, PadLay<X1 * X2>>(npx, rowparts, occ_g2r, occ_r2g,
-                                                                 rho_ok)
-             : plane3_info_l<T, X1, X2, Y1, Y2, OldLay<X1 * X2>>(npx, rowparts, occ_g2r, occ_r2g,
-                                                                 rho_ok);
+hipError_t plane3_info(int npx, bool prune, int* rowparts, int* occ_g2r, int* occ_r2g,
+                       int* rho_ok) {
+  constexpr int N = X1 * X2;
+  const bool pad = use_pad<T, X1, X2, Y1, Y2, kPlaneThreads>(npx);
+  if (prune)
+    return pad ? plane3_info_l<T, X1, X2, Y1, Y2, PadLay<N, true>>(npx, rowparts, occ_g2r,
+                                                                   occ_r2g, rho_ok)
+               : plane3_info_l<T, X1, X2, Y1, Y2, OldLay<N, true>>(npx, rowparts, occ_g2r,
+                                                                   occ_r2g, rho_ok);
+  return pad ? plane3_info_l<T, X1, X2, Y1, Y2, PadLay<N>>(npx, rowparts, occ_g2r, occ_r2g, rho_ok)
+             : plane3_info_l<T, X1, X2, Y1, Y2, OldLay<N>>(npx, rowparts, occ_g2r, occ_r2g, rho_ok);
 }
 
+// c.prune (set by the host when every populated |kx|, |ky| <= (N - 1) / 4)
+// selects the variants pruned to the sphere
 template <typename T, int X1, int X2, int Y1, int Y2>
 hipError_t launch_plane3(int kind, const FftCtx& c, hipStream_t st) {
+  constexpr int N = X1 * X2;
+  if (c.prune)
+    return use_pad<T, X1, X2, Y1, Y2, kPlaneThreads>(c.g.nplane_x)
+               ? launch_plane3_l<T, X1, X2, Y1, Y2, PadLay<N, true>>(kind, c, st)
+               : launch_plane3_l<T, X1, X2, Y1, Y2, OldLay<N, true>>(kind, c, st);
   return use_pad<T, X1, X2, Y1, Y2, kPlaneThreads>(c.g.nplane_x)
-             ? launch_plane3_l<T, X1, X2, Y1, Y2, PadLay<X1 * X2>>(kind, c, st)
-             : launch_plane3_l<T, X1, X2, Y1, Y2, OldLay<X1 * X2>>(kind, c, st);
+             ? launch_plane3_l<T, X1, X2, Y1, Y2, PadLay<N>>(kind, c, st)
+             : launch_plane3_l<T, X1, X2, Y1, Y2, OldLay<N>>(kind, c, st);
 }
 
 }  // namespace oa3

@@ -26,9 +26,11 @@ size_t OA_CAT(plane3_smem_n, OA_FFT2_N)(int npx, bool fp32) {
               : plane3_smem<double, kX1, kX2, kY1, kY2, kPlaneThreads>(npx);
 }
 
-hipError_t OA_CAT(plane3_info_n, OA_FFT2_N)(int npx, bool fp32, int* rowparts, int* occ_g2r,
-                                           int* occ_r2g, int* rho_ok) {
-  return fp32 ? plane3_info<float, kX1, kX2, kY1, kY2>(npx, rowparts, occ_g2r, occ_r2g, rho_ok)
-              : plane3_info<double, kX1, kX2, kY1, kY2>(npx, rowparts, occ_g2r, occ_r2g, rho_ok);
+hipError_t OA_CAT(plane3_info_n, OA_FFT2_N)(int npx, bool fp32, bool prune, int* rowparts,
+                                           int* occ_g2r, int* occ_r2g, int* rho_ok) {
+  return fp32 ? plane3_info<float, kX1, kX2, kY1, kY2>(npx, prune, rowparts, occ_g2r, occ_r2g,
+                                                       rho_ok)
+              : plane3_info<double, kX1, kX2, kY1, kY2>(npx, prune, rowparts, occ_g2r, occ_r2g,
+                                                        rho_ok);
 }
 }  // namespace oa3

@@ -346,14 +346,20 @@ oa_status build_layout(oa_ctx* c) {
     hi[lkx[l]] = std::max(hi[lkx[l]], lky[l]);
     cnt[lkx[l]]++;
   }
+  int kabs = c->nplane_x - 1;
   for (int x = 0; x < c->nplane_x; x++) {
     if (cnt[x] == 0) {
       lo[x] = 1;
       hi[x] = 0;
     } else if (hi[x] - lo[x] + 1 != cnt[x]) {
       return fail(c, OA_EINVAL, "populated ky range of a kx column is not contiguous");
+    } else {
+      kabs = std::max(kabs, std::max(std::abs(lo[x]), std::abs(hi[x])));
     }
   }
+  // the plane passes skip the transforms' zero inputs and unkept outputs
+  // when every populated |kx|, |ky| is <= (N - 1) / 4
+  c->fc.prune = !c->opt.plane_full && 4 * kabs + 1 <= std::min(c->nx, c->ny);
   // redundant kx=0 points and their partners (CP_State_GSpacePlane.C:2862-3061)
   std::map<std::pair<int, int>, int> kx0idx;
   for (int i = 0; i < nkx0; i++) kx0idx[{ky[i], kz[i]}] = i;
@@ -418,7 +424,8 @@ oa_status build_layout(oa_ctx* c) {
     // State chunks of both plane passes from the resident-workgroup count
     // (every G->R chunk also writes one rho partial of N doubles)
     int rowparts = 1, occ_g = 1, occ_r = 1, rho_ok = 0, ncu = 0;
-    if (oa::plane3_info(c->nx, c->nplane_x, c->fp32, &rowparts, &occ_g, &occ_r, &rho_ok) !=
+    if (oa::plane3_info(c->nx, c->nplane_x, c->fp32, f.prune != 0, &rowparts, &occ_g, &occ_r,
+                        &rho_ok) !=
         hipSuccess)
       return fail(c, OA_EHIP, "plane pass occupancy query failed");
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) !=
@@ -1325,6 +1332,7 @@ void oa_default_options(oa_options* o) {
   o->autotune = 0;
   o->gemm_split = 1;
   o->ns_fp32 = 0;
+  o->plane_full = 0;
 }
 
 int oa_kvecs(double ecut_ry, const double hmat[9], int* kx, int* ky, int* kz) {
