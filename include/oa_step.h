@@ -113,11 +113,12 @@ typedef struct {
                           from that iterate to ns_tol, ns_max_iter iterations in all (two
                           launches per iteration; oa_step_out.ns_iters counts both).  T meets
                           the same tolerance as in FP64.  0 (default): FP64 throughout */
-  int plane_full;      /* 1: the fused plane passes run every x and y transform in full;
-                          0 (default): where every populated |kx|, |ky| is <= (N - 1) / 4
-                          (always so on a grid that holds the density sphere) they skip the
-                          radix-pass inputs that are zero outside the sphere and the outputs
-                          that are not kept (rounding-level differences only) */
+  int fft_full;        /* 1: the state FFT passes (two-factor z passes, fused planes) run
+                          every transform in full; 0 (default): along each axis where every
+                          stored |k| is <= (N - 1) / 4 (always so on a grid that holds the
+                          density sphere) they skip the radix-pass inputs that are zero
+                          outside the sphere and the outputs that are not kept
+                          (rounding-level differences only) */
 } oa_options;
 
 typedef struct {

@@ -229,4 +229,24 @@ __device__ __forceinline__ cpx<T> twiddle(const double2* __restrict__ tw, int m)
   return {(T)w.x, (T)(S < 0 ? w.y : -w.y)};
 }
 
+// Pruning to the sphere.  A grid that holds the density sphere |G|^2 <= 4
+// Ecut has N >= 2 kr + 1 with kr >= 2 kmax, so the wavefunction components
+// satisfy |k| <= KM = (N - 1) / 4 along every axis.  A radix pass reading
+// (or writing) index i = j + t*STRIDE, j < STRIDE, touches only zeros (or
+// unneeded outputs) for every j when KM < t*STRIDE and t*STRIDE + STRIDE - 1
+// < N - KM, i.e. for t in [T0, T1].  Pruned kernels set those inputs to zero
+// at compile time and do not store those outputs, so the compiler drops
+// their arithmetic (the engine's translation units are built with
+// -fno-signed-zeros, which lets x + 0 fold to x; only the sign of zero
+// results can differ).
+template <int N, int STRIDE>
+struct Sph {
+  static constexpr int KM = (N - 1) / 4;
+  static constexpr int T0 = KM / STRIDE + 1;
+  static constexpr int T1 = (N - KM - STRIDE) / STRIDE;
+  static __device__ __forceinline__ constexpr bool cut(bool on, int t) {
+    return on && t >= T0 && t <= T1;
+  }
+};
+
 }  // namespace oa2

@@ -33,7 +33,8 @@ __device__ __forceinline__ int signed_k(int i, int n) { return (i <= (n - 1) / 2
 
 // ------------------------------------------------------------------ z passes
 
-template <typename T, int R1, int R2>
+// PR: every stored |kz| <= (N - 1) / 4 (pruned pass 1, Sph in oa_fft2.h)
+template <typename T, int R1, int R2, bool PR>
 __global__ __launch_bounds__(kThreads) void k2_zfwd(FftCtx c) {
   using Sh = Shape<R1, R2>;
   constexpr int N = Sh::N, G = Sh::G, NTR = Sh::NTR;
@@ -54,6 +55,10 @@ __global__ __launch_bounds__(kThreads) void k2_zfwd(FftCtx c) {
       cpx<T> v[R1];
 #pragma unroll
       for (int t = 0; t < R1; t++) {
+        if (Sph<N, R2>::cut(PR, t)) {
+          v[t] = {(T)0, (T)0};
+          continue;
+        }
         const int d = signed_k(j + t * R2, N) - L.z;
         const double2 a = ps[min(max(d, 0), L.y - 1)];
         const bool in = d >= 0 && d < L.y;
@@ -82,7 +87,8 @@ __global__ __launch_bounds__(kThreads) void k2_zfwd(FftCtx c) {
   }
 }
 
-template <typename T, int R1, int R2>
+// PR: every stored |kz| <= (N - 1) / 4 (pass-2 outputs outside it dropped)
+template <typename T, int R1, int R2, bool PR>
 __global__ __launch_bounds__(kThreads) void k2_zbwd(FftCtx c) {
   using Sh = Shape<R1, R2>;
   constexpr int N = Sh::N, G = Sh::G, NTR = Sh::NTR;
@@ -153,7 +159,7 @@ __global__ __launch_bounds__(kThreads) void k2_zbwd(FftCtx c) {
 #pragma unroll
       for (int t = 0; t < R2; t++) {
         const int d = signed_k(j + t * R1, N) - L.z;
-        if (d >= 0 && d < L.y) {
+        if (!Sph<N, R1>::cut(PR, t) && d >= 0 && d < L.y) {
           ex[ob + d] = u[t];
           wt_l[ob + d] = w;
         }
@@ -445,12 +451,20 @@ hipError_t launch(int kind, const FftCtx& c, hipStream_t st) {
   const Geom& g = c.g;
   switch (kind) {
     case ZFWD:
-      hipLaunchKernelGGL((k2_zfwd<T, R1, R2>), dim3((g.nlines + NTR - 1) / NTR, g.nst),
-                         dim3(kThreads), 0, st, c);
+      if (c.prune_z)
+        hipLaunchKernelGGL((k2_zfwd<T, R1, R2, true>), dim3((g.nlines + NTR - 1) / NTR, g.nst),
+                           dim3(kThreads), 0, st, c);
+      else
+        hipLaunchKernelGGL((k2_zfwd<T, R1, R2, false>), dim3((g.nlines + NTR - 1) / NTR, g.nst),
+                           dim3(kThreads), 0, st, c);
       break;
     case ZBWD:
-      hipLaunchKernelGGL((k2_zbwd<T, R1, R2>), dim3((g.nlines + NTR - 1) / NTR, g.nst),
-                         dim3(kThreads), 0, st, c);
+      if (c.prune_z)
+        hipLaunchKernelGGL((k2_zbwd<T, R1, R2, true>), dim3((g.nlines + NTR - 1) / NTR, g.nst),
+                           dim3(kThreads), 0, st, c);
+      else
+        hipLaunchKernelGGL((k2_zbwd<T, R1, R2, false>), dim3((g.nlines + NTR - 1) / NTR, g.nst),
+                           dim3(kThreads), 0, st, c);
       break;
     case YFWD:
       hipLaunchKernelGGL((k2_yfwd<T, R1, R2>), dim3((g.nplane_x + NTR - 1) / NTR, g.nz, g.nst),

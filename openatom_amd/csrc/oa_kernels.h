@@ -50,6 +50,7 @@ struct FftCtx {
   int cs_r;               // ... and states per chunk
   int psir_state;         // state whose psi(r) the G->R plane pass writes to psir
   int prune;              // plane passes pruned to |kx|, |ky| <= (N - 1) / 4
+  int prune_z;            // z passes pruned to |kz| <= (nz - 1) / 4
 };
 
 // two-factor register FFT engine (oa_fft2_*.hip): kind 0 z fwd, 1 y fwd,

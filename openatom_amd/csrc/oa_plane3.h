@@ -63,7 +63,7 @@ __device__ __forceinline__ int signed_k(int i, int n) { return (i <= (n - 1) / 2
 //         4 x c2r pass 2 (and what follows it in registers: rho, V psi and
 //         the r2c pass 1), 8 r2c pass 2, 16 y_bwd pass 1
 //   TW2 : twiddle tables indexed [t][j] (else one table indexed t*j)
-//   PR  : prune the plane transforms to the sphere (Sph below): every
+//   PR  : prune the plane transforms to the sphere (Sph, oa_fft2.h): every
 //         populated kx and ky satisfies |k| <= (N - 1) / 4
 template <int CYP_, int YA_, int EA_, int REP_, int MAPS_, bool TW2_, bool PR_ = false>
 struct Lay {
@@ -71,25 +71,7 @@ struct Lay {
   static constexpr bool TW2 = TW2_, PR = PR_;
 };
 
-// Pruning to the sphere.  A grid that holds the density sphere |G|^2 <= 4
-// Ecut has N >= 2 kr + 1 with kr >= 2 kmax, so the wavefunction components
-// satisfy |k| <= KM = (N - 1) / 4.  A radix pass reading (or writing) index
-// i = j + t*STRIDE, j < STRIDE, touches only zeros (or unneeded outputs) for
-// every j when KM < t*STRIDE and t*STRIDE + STRIDE - 1 < N - KM, i.e. for t in
-// [T0, T1].  Those inputs are set to zero at compile time and those outputs
-// are not stored, so the compiler drops their arithmetic (the plane
-// translation units are built with -fno-signed-zeros, which lets x + 0 fold
-// to x; only the sign of zero results can differ).
-template <int N, int STRIDE>
-struct Sph {
-  static constexpr int KM = (N - 1) / 4;
-  static constexpr int T0 = KM / STRIDE + 1;
-  static constexpr int T1 = (N - KM - STRIDE) / STRIDE;
-  template <class L>
-  static __device__ __forceinline__ constexpr bool cut(int t) {
-    return L::PR && t >= T0 && t <= T1;
-  }
-};
+using oa2::Sph;  // the sphere windows of the pruned passes (oa_fft2.h)
 
 enum : int { kMapYf2 = 1, kMapXc1 = 2, kMapXc2 = 4, kMapRx2 = 8, kMapYb1 = 16 };
 
@@ -200,7 +182,7 @@ __device__ __forceinline__ void y_fwd(const cpx<T>* __restrict__ zs, cpx<T>* P, 
       for (int t = 0; t < Y1; t++) {
         const int y = j + t * Y2;
         const int kyv = signed_k(y, NY);
-        v[t] = (!Sph<NY, Y2>::template cut<L>(t) && kyv >= lo && kyv <= hi) ? src[y]
+        v[t] = (!Sph<NY, Y2>::cut(L::PR, t) && kyv >= lo && kyv <= hi) ? src[y]
                                                                           : cpx<T>{(T)0, (T)0};
       }
       Dft<T, Y1, -1>::run(v);
@@ -269,7 +251,7 @@ __device__ __forceinline__ void y_bwd(const cpx<T>* P, cpx<T>* E, const cpx<T>* 
       for (int t = 0; t < Y2; t++) {
         const int y = j + t * Y1;
         const int kyv = signed_k(y, NY);
-        if (!Sph<NY, Y1>::template cut<L>(t) && kyv >= lo && kyv <= hi) dst[y] = u[t];
+        if (!Sph<NY, Y1>::cut(L::PR, t) && kyv >= lo && kyv <= hi) dst[y] = u[t];
       }
     }
     __syncthreads();
@@ -317,7 +299,7 @@ __device__ __forceinline__ void x_c2r(const cpx<T>* P, cpx<T>* E, const cpx<T>* 
       cpx<T> Z = {(T)0, (T)0};
       int kk = -1;
       bool mirror = false;
-      if (Sph<NX, X2>::template cut<L>(t)) {
+      if (Sph<NX, X2>::cut(L::PR, t)) {
       } else if (k < npx) {
         kk = k;
       } else if (k > NX - npx) {
@@ -547,7 +529,7 @@ k3_r2g(FftCtx c) {
         Dft<T, X1, -1>::run(a);
 #pragma unroll
         for (int t = 0; t < X1; t++)
-          if (!Sph<NX, X2>::template cut<L>(t)) row[t * (X2 + L::EA)] = a[t];
+          if (!Sph<NX, X2>::cut(L::PR, t)) row[t * (X2 + L::EA)] = a[t];
       }
       __syncthreads();
       // split the two real rows, keep kx < nplane_x, conjugated, into P

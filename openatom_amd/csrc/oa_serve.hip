@@ -81,7 +81,7 @@ ShapeKey shape_key(const Job& j, int precision) {
   const oa_options& o = j.r.opt;
   k.ints = {sys.grid[0], sys.grid[1], sys.grid[2], sys.nstates, sys.npts, precision,
             o.ns_max_iter, o.use_lambda, o.use_cg, o.want_resid, o.fft_mode, o.use_graph,
-            o.ns_mode, o.vks_from_rho, o.autotune, o.gemm_split, o.ns_fp32, o.plane_full};
+            o.ns_mode, o.vks_from_rho, o.autotune, o.gemm_split, o.ns_fp32, o.fft_full};
   k.scalars.assign(sys.hmat, sys.hmat + 9);
   for (double v : {sys.ecut_ry, o.ns_tol, o.dt, o.tau_mass, o.ecut_mass}) k.scalars.push_back(v);
   k.kx = j.kx;

@@ -357,9 +357,12 @@ oa_status build_layout(oa_ctx* c) {
       kabs = std::max(kabs, std::max(std::abs(lo[x]), std::abs(hi[x])));
     }
   }
-  // the plane passes skip the transforms' zero inputs and unkept outputs
-  // when every populated |kx|, |ky| is <= (N - 1) / 4
-  c->fc.prune = !c->opt.plane_full && 4 * kabs + 1 <= std::min(c->nx, c->ny);
+  // the plane (z) passes skip the transforms' zero inputs and unkept
+  // outputs when every populated |kx|, |ky| (|kz|) is <= (N - 1) / 4
+  int kzabs = 0;
+  for (int i = 0; i < n; i++) kzabs = std::max(kzabs, std::abs(kz[i]));
+  c->fc.prune = !c->opt.fft_full && 4 * kabs + 1 <= std::min(c->nx, c->ny);
+  c->fc.prune_z = !c->opt.fft_full && 4 * kzabs + 1 <= c->nz;
   // redundant kx=0 points and their partners (CP_State_GSpacePlane.C:2862-3061)
   std::map<std::pair<int, int>, int> kx0idx;
   for (int i = 0; i < nkx0; i++) kx0idx[{ky[i], kz[i]}] = i;
@@ -1332,7 +1335,7 @@ void oa_default_options(oa_options* o) {
   o->autotune = 0;
   o->gemm_split = 1;
   o->ns_fp32 = 0;
-  o->plane_full = 0;
+  o->fft_full = 0;
 }
 
 int oa_kvecs(double ecut_ry, const double hmat[9], int* kx, int* ky, int* kz) {

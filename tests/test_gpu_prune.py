@@ -1,15 +1,15 @@
-"""Plane passes pruned to the sphere (plane_full = 0, the default) against
-the full transforms (plane_full = 1).
+"""State FFT passes pruned to the sphere (fft_full = 0, the default) against
+the full transforms (fft_full = 1).
 
-Where every populated |kx|, |ky| is <= (N - 1) / 4 the fused plane passes
-skip the radix-pass inputs that are zero outside the sphere and the outputs
-that are not kept.  Only additions of exact zeros and unused values go, so
-the density, forces and kinetic energy agree with the full transforms to
-rounding (1e-13 relative in FP64, 1e-6 mixed), and both agree with the
-oracle (1e-10 FP64, 1e-5 mixed).  The cases: water8 (48^3, |k| <= 10), a
-serving shape (5 H2O at 70 Ry, 54^3 = 9 x 6), and water8's sphere on a
-40^3 grid, which does not satisfy the bound (both settings then run the full
-transforms)."""
+Along each axis where every stored |k| is <= (N - 1) / 4, the two-factor z
+passes and the fused plane passes skip the radix-pass inputs that are zero
+outside the sphere and the outputs that are not kept.  Only additions of
+exact zeros and unused values go, so the density, forces and kinetic energy
+agree with the full transforms to rounding (1e-13 relative in FP64, 1e-6
+mixed), and both agree with the oracle (1e-10 FP64, 1e-5 mixed).  The
+cases: water8 (48^3, |k| <= 10), a serving shape (5 H2O at 70 Ry, 54^3 =
+9 x 6), and water8's sphere on a 40^3 grid, which does not satisfy the bound
+(both settings then run the full transforms)."""
 import numpy as np
 import pytest
 
@@ -37,9 +37,9 @@ def get_system(name):
     return _cache[name]
 
 
-def run(s, precision, plane_full):
+def run(s, precision, fft_full):
     with CPContext(s["grid"], s["hmat"], s["ecut_ry"], s["nstates"], s["kx"], s["ky"], s["kz"],
-                   precision=precision, plane_full=plane_full) as c:
+                   precision=precision, fft_full=fft_full) as c:
         c.set_psi(s["psi"])
         c.set_vks(s["vks"])
         c.fft_g2r()
@@ -53,7 +53,7 @@ def run(s, precision, plane_full):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_pruned_equals_full(gpu, name, precision):
     s = get_system(name)
-    kabs = max(int(np.max(s["kx"])), int(np.max(np.abs(s["ky"]))))
+    kabs = max(int(np.max(np.abs(s[k]))) for k in ("kx", "ky", "kz"))
     pruned = 4 * kabs + 1 <= min(s["grid"][:2])
     assert pruned == (name != "water8_grid40")
     rho_p, f_p, eke_p = run(s, precision, 0)
