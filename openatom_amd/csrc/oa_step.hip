@@ -139,8 +139,14 @@ struct oa_ctx {
       nred = 0;
   double hmat[9]{}, hmati[9]{}, vol = 0.0, ecut_ry = 0.0;
   std::vector<int> kx, ky, kz;
-  // device
+  // device: the allocations of oa_create come from one block (arena), so a
+  // context is released with one hipFree (each hipFree waits for the whole
+  // device, stalling every other stream); later ones are separate blocks
   std::vector<void*> allocs;
+  char* arena = nullptr;
+  size_t arena_size = 0, arena_used = 0;
+  char* stage = nullptr;  // staging block of set-up uploads, grown on demand
+  size_t stage_cap = 0;
   double2 *psi = nullptr, *psi2 = nullptr, *force = nullptr;
   int *blk_pt0 = nullptr, *pt_code = nullptr, *line_dst = nullptr, *kylo = nullptr,
       *kyhi = nullptr, *red = nullptr, *partner = nullptr,
@@ -234,9 +240,15 @@ template <typename P>
 oa_status dalloc(oa_ctx* c, P** p, size_t bytes) {
   void* q = nullptr;
   if (bytes == 0) bytes = 16;
-  c->bytes += (bytes + 255) & ~(size_t)255;
+  const size_t a = (bytes + 255) & ~(size_t)255;
+  c->bytes += a;
   if (c->dry) {  // sizing run: count only
     *p = nullptr;
+    return OA_OK;
+  }
+  if (c->arena && c->arena_used + a <= c->arena_size) {
+    *p = reinterpret_cast<P*>(c->arena + c->arena_used);
+    c->arena_used += a;
     return OA_OK;
   }
   if (hipMalloc(&q, bytes) != hipSuccess) {
@@ -253,6 +265,27 @@ oa_status dalloc(oa_ctx* c, P** p, size_t bytes) {
     oa_status s_ = (expr);            \
     if (s_ != OA_OK) return s_;       \
   } while (0)
+
+// the context's staging block, at least `bytes` long (kept until destroy:
+// a per-call hipMalloc/hipFree pair would wait for the whole device)
+oa_status stage_buf(oa_ctx* c, size_t bytes, char** out) {
+  if (bytes > c->stage_cap) {
+    if (c->stage) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      (void)hipFree(c->stage);
+      c->stage = nullptr;
+      c->stage_cap = 0;
+    }
+    if (hipMalloc((void**)&c->stage, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      c->stage = nullptr;
+      return fail(c, OA_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    c->stage_cap = bytes;
+  }
+  *out = c->stage;
+  return OA_OK;
+}
 
 oa_status upload(oa_ctx* c, void* dst, const void* src, size_t bytes) {
   if (c->dry) return OA_OK;
@@ -1406,6 +1439,19 @@ static oa_status create_impl(int device, const oa_system* sys, const oa_options*
     c->err = "stream creation failed";
     return bail(OA_EHIP);
   }
+  if (!dry) {
+    // the arena: the device bytes a sizing run of the same system counts
+    oa_ctx* d = nullptr;
+    if (create_impl(device, sys, opt, true, &d) == OA_OK && d) {
+      if (hipMalloc((void**)&c->arena, d->bytes) == hipSuccess) {
+        c->arena_size = d->bytes;
+      } else {  // separate blocks instead
+        (void)hipGetLastError();
+        c->arena = nullptr;
+      }
+    }
+    oa_destroy(d);
+  }
   oa_status s = build_layout(c);
   if (s != OA_OK) return bail(s);
   s = alloc_state(c, sys->occ);
@@ -1459,6 +1505,8 @@ oa_status oa_destroy(oa_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graphs(c);
   for (void* p : c->allocs) (void)hipFree(p);
+  if (c->arena) (void)hipFree(c->arena);
+  if (c->stage) (void)hipFree(c->stage);
   if (c->res_host) (void)hipHostFree(c->res_host);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -1570,7 +1618,8 @@ oa_status oa_local_pseudo_set(oa_ctx* c, const oa_local_pseudo* lp) {
   const size_t off_sp = (bpos + 255) & ~(size_t)255;
   const size_t off_vps = off_sp + ((bsp + 255) & ~(size_t)255);
   const size_t off_g0 = off_vps + ((bvps + 255) & ~(size_t)255);
-  HIPCHK(hipMalloc((void**)&tmp, off_g0 + bg0));
+  HIPCHK(hipStreamSynchronize(c->stream));  // the staging block may be in use
+  TRY(stage_buf(c, off_g0 + bg0, &tmp));
   oa::PseudoDev p{};
   p.natoms = lp->natoms;
   p.nspecies = lp->nspecies;
@@ -1587,7 +1636,6 @@ oa_status oa_local_pseudo_set(oa_ctx* c, const oa_local_pseudo* lp) {
   if (e == hipSuccess) e = hipMemcpy(tmp + off_g0, lp->vps_g0, bg0, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = oa::vloc_build(c->va, p, c->vloc, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  (void)hipFree(tmp);
   HIPCHK(e);
   return OA_OK;
 }
@@ -1652,7 +1700,7 @@ oa_status oa_nonlocal_set(oa_ctx* c, const oa_nonlocal* nl) {
   const size_t o_ky = up(bk), o_kz = o_ky + up(bk), o_pos = o_kz + up(bk);
   const size_t o_sp = o_pos + up(bpos), o_proj = o_sp + up(bsp), o_g0 = o_proj + up(bproj);
   char* tmp = nullptr;
-  HIPCHK(hipMalloc((void**)&tmp, o_g0 + bg0));
+  TRY(stage_buf(c, o_g0 + bg0, &tmp));
   oa::NlDev d{};
   d.natoms = na;
   d.nspecies = nl->nspecies;
@@ -1683,7 +1731,6 @@ oa_status oa_nonlocal_set(oa_ctx* c, const oa_nonlocal* nl) {
     e = hipMemcpy(c->nl_vn, vn.data(), sizeof(double) * na, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = oa::nl_sf_build(d, c->nl_sf, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  (void)hipFree(tmp);
   HIPCHK(e);
   c->nl_natm = na;
   drop_graphs(c);  // captured steps lack (or hold the old) non-local stage
