@@ -217,3 +217,29 @@ def test_pooled_context_equals_fresh(gpu):
     assert res["x"]["status"] == 0 and res["x"]["steps"] == fresh["steps"]
     assert res["x"]["eelec"] == fresh["eelec"]
     assert np.array_equal(res["x"]["psi"], fresh["psi"])
+
+
+def test_mixed_switch_pools_both_contexts(gpu):
+    """At the FP64 switch the request's mixed context goes to the pool: a
+    second mixed request of the same shape takes it over at admission and
+    the FP64 context at its own switch, and gives the result of a fresh
+    server bit for bit."""
+    s = systems()[0]  # 45^3, 16 states
+    warm = dict(s, psi=np.ascontiguousarray(s["psi"][::-1]))  # same shape, states reversed
+    kw = dict(f_tol=1e-6, max_steps=800, precision="mixed", want_psi=True, **OPTS)
+    with serve.Server(max_active=1) as srv:
+        srv.submit(s, "x", **kw)
+        fresh = {r["tag"]: r for r in srv.drain()}["x"]
+    with serve.Server(max_active=1) as srv:
+        srv.submit(warm, "warm", **kw)
+        srv.submit(s, "x", **kw)
+        res = {r["tag"]: r for r in srv.drain()}
+    print(f"[serve mixed pool] fresh: {fresh['steps']} steps ({fresh['fp64_steps']} FP64) "
+          f"E {fresh['eelec']:.10f}; pooled: {res['x']['steps']} steps, reused "
+          f"{[res[k]['reused'] for k in ('warm', 'x')]}", flush=True)
+    assert fresh["fp64_steps"] > 0 and res["warm"]["fp64_steps"] > 0
+    assert res["warm"]["reused"] == 0 and res["x"]["reused"] == 1
+    assert res["x"]["status"] == 0 and res["x"]["steps"] == fresh["steps"]
+    assert res["x"]["fp64_steps"] == fresh["fp64_steps"]
+    assert res["x"]["eelec"] == fresh["eelec"]
+    assert np.array_equal(res["x"]["psi"], fresh["psi"])
