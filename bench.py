@@ -270,10 +270,14 @@ class ServingMix:
         self.ex.shutdown()
 
 
-def serving_run(mix, arrive, precision, f_tol, max_steps, budget, max_active, opts, ahead=64):
+def serving_run(mix, arrive, precision, f_tol, max_steps, budget, max_active, opts, ahead=64,
+                deadline=0.0):
     """Submit each request at its arrival time (or as soon as its system is
     built, when that is later), collect results as they finish.  Latency =
-    finish - arrival (queueing and any build delay included)."""
+    finish - arrival (queueing and any build delay included).  deadline > 0:
+    at that many seconds every request not finished yet is withdrawn (the
+    server's cancel; requests not yet submitted are not submitted), and the
+    summary covers the requests that finished within it."""
     import numpy as np
     from openatom_amd import serve
     n = len(mix)
@@ -284,12 +288,21 @@ def serving_run(mix, arrive, precision, f_tol, max_steps, budget, max_active, op
         t0 = time.perf_counter()
         i = 0
         t_log = 0.0
-        while len(res) < n:
-            for k in range(i, min(n, i + ahead)):
-                if k not in futs:
-                    futs[k] = mix.build(k)
+        cancelled_at = None
+        while len(res) < i or (cancelled_at is None and i < n):
             now = time.perf_counter() - t0
-            while i < n and arrive[i] <= now and futs[i].done():
+            if deadline > 0 and cancelled_at is None and now >= deadline:
+                srv.cancel()
+                cancelled_at = now
+                for f in futs.values():
+                    f.cancel()
+                log(f"[serve] {precision} deadline {deadline:g} s: {len(res)} finished, "
+                    f"{i} submitted, the rest withdrawn")
+            if cancelled_at is None:
+                for k in range(i, min(n, i + ahead)):
+                    if k not in futs:
+                        futs[k] = mix.build(k)
+            while cancelled_at is None and i < n and arrive[i] <= now and futs[i].done():
                 srv.submit(futs.pop(i).result(), i, f_tol=f_tol, max_steps=max_steps,
                            precision=precision, **opts)
                 lag[i] = time.perf_counter() - t0 - arrive[i]
@@ -303,24 +316,31 @@ def serving_run(mix, arrive, precision, f_tol, max_steps, budget, max_active, op
                     f"active {int(st['active'])} queued {int(st['queued'])} "
                     f"device {st['hbm_bytes'] / 2**30:.1f} GiB")
             time.sleep(5e-4)
-        wall = time.perf_counter() - t0
+        wall = time.perf_counter() - t0 if cancelled_at is None else cancelled_at
         st = srv.stats()
-    lat = np.array([res[k]["total_ms"] + lag[k] * 1e3 for k in range(n)])
-    steps = np.array([res[k]["steps"] for k in range(n)])
-    ok = [k for k in range(n) if res[k]["status"] == 0]
+    ECANCELED = 6
+    done = [k for k in res if res[k]["status"] != ECANCELED]  # finished, not withdrawn
+    lat = np.array([res[k]["total_ms"] + lag[k] * 1e3 for k in done])
+    steps = np.array([res[k]["steps"] for k in done])
+    ok = [k for k in done if res[k]["status"] == 0]
     for r in res.values():
         r.pop("psi", None)
-    return dict(res=res, summary={
-        "reused_contexts": int(sum(r["reused"] for r in res.values())),
+    summary = {
+        "reused_contexts": int(sum(res[k]["reused"] for k in done)),
         "solves_per_s": round(len(ok) / wall, 3),
         "p50_ms": round(float(np.percentile(lat, 50)), 2),
         "p99_ms": round(float(np.percentile(lat, 99)), 2),
         "max_ms": round(float(lat.max()), 2),
-        "mean_queue_ms": round(float(np.mean([res[k]["queue_ms"] for k in range(n)])), 2),
+        "mean_queue_ms": round(float(np.mean([res[k]["queue_ms"] for k in done])), 2),
         "converged": len(ok), "requests": n,
         "steps_mean": round(float(steps.mean()), 1), "steps_max": int(steps.max()),
         "wall_s": round(wall, 3), "peak_active": int(st["peak_active"]),
-        "peak_hbm_gib": round(st["peak_hbm_bytes"] / 2**30, 3)})
+        "peak_hbm_gib": round(st["peak_hbm_bytes"] / 2**30, 3)}
+    if cancelled_at is not None:
+        summary.update(deadline_s=deadline, finished_in_deadline=len(done),
+                       submitted_by_deadline=i, withdrawn=len(res) - len(done),
+                       not_submitted=n - i)
+    return dict(res=res, summary=summary)
 
 
 def serving_leg(args, mix, precisions):
@@ -342,7 +362,8 @@ def serving_leg(args, mix, precisions):
     for prec in precisions:
         log(f"[bench] serving ({prec}): {len(mix)} requests")
         r = serving_run(mix, arrive, prec, args.serve_ftol, args.serve_max_steps,
-                        int(args.serve_budget_gib * 2**30), args.serve_active, sopts)
+                        int(args.serve_budget_gib * 2**30), args.serve_active, sopts,
+                        deadline=args.serve_deadline)
         log(f"[bench] serving {prec}: {r['summary']}")
         serving[prec] = r["summary"]
         runs_s[prec] = r["res"]
@@ -410,6 +431,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-max-mol", type=int, default=64)
     ap.add_argument("--serve-ftol", type=float, default=1e-5)
+    ap.add_argument("--serve-deadline", type=float, default=0.0,
+                    help="seconds after which unfinished serving requests are withdrawn "
+                         "(0: run every request to the end)")
     ap.add_argument("--serve-dt", type=float, default=DEFAULT_SERVE_DT,
                     help="fictitious time step of the requests' CG (a.u.)")
     ap.add_argument("--serve-max-steps", type=int, default=1500)

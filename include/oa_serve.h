@@ -85,6 +85,12 @@ int oa_server_poll(oa_server *s, oa_result *res, int max);
 /* Block until every submitted request has finished (results stay queued
  * for poll). */
 oa_status oa_server_drain(oa_server *s);
+/* Withdraw every request not finished yet: queued ones finish at once,
+ * admitted ones after their step in flight, all with OA_ECANCELED (their
+ * results carry the steps run so far).  Requests submitted later run
+ * normally.  No counterpart in the reference (its jobs run to completion);
+ * for a caller that bounds a run by time. */
+oa_status oa_server_cancel(oa_server *s);
 /* [0] admitted now, [1] queued, [2] finished, [3] HBM bytes held,
  * [4] peak admitted, [5] peak HBM bytes, [6] steps run, [7] submitted */
 oa_status oa_server_stats(oa_server *s, double stats[8]);

@@ -44,7 +44,8 @@ typedef enum {
   OA_EHIP = 2,     /* HIP runtime error */
   OA_ENOMEM = 3,   /* device allocation failed */
   OA_ENOCONV = 4,  /* Newton-Schulz missed its tolerance (ortho.C:129-133) */
-  OA_ESTATE = 5    /* call out of order (the reference's "Flow of Control" checks) */
+  OA_ESTATE = 5,   /* call out of order (the reference's "Flow of Control" checks) */
+  OA_ECANCELED = 6 /* serving: the request was withdrawn by oa_server_cancel */
 } oa_status;
 
 typedef enum { OA_FP64 = 0, OA_MIXED = 1 } oa_precision;

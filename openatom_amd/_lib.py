@@ -11,10 +11,10 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboa_step.so")
 
-OA_OK, OA_EINVAL, OA_EHIP, OA_ENOMEM, OA_ENOCONV, OA_ESTATE = range(6)
+OA_OK, OA_EINVAL, OA_EHIP, OA_ENOMEM, OA_ENOCONV, OA_ESTATE, OA_ECANCELED = range(7)
 OA_FP64, OA_MIXED = 0, 1
 STATUS_NAMES = {0: "OA_OK", 1: "OA_EINVAL", 2: "OA_EHIP", 3: "OA_ENOMEM",
-                4: "OA_ENOCONV", 5: "OA_ESTATE"}
+                4: "OA_ENOCONV", 5: "OA_ESTATE", 6: "OA_ECANCELED"}
 
 # every symbol include/oa_step.h and include/oa_serve.h declare
 EXPORTS = (
@@ -31,7 +31,7 @@ EXPORTS = (
     "oa_state_read", "oa_state_write", "oa_states_read", "oa_states_write",
     # request loop (oa_serve.h)
     "oa_ctx_bytes", "oa_server_create", "oa_server_submit", "oa_server_poll", "oa_server_drain",
-    "oa_server_stats", "oa_server_destroy",
+    "oa_server_stats", "oa_server_destroy", "oa_server_cancel",
 )
 
 

@@ -119,7 +119,8 @@ struct Job {
   std::vector<int> lp_species;
   size_t bytes = 0;
   clk::time_point t_submit;
-  ShapeKey key;  // of the requested precision (set at submit)
+  ShapeKey key;       // of the requested precision (set at submit)
+  long long seq = 0;  // submission order
 };
 
 ShapeKey shape_key(const Job& j, int precision) {
@@ -194,6 +195,8 @@ struct oa_server {
   int n_active = 0, peak_active = 0;
   long long steps_run = 0, submitted = 0, finished = 0;
   bool stop = false;
+  std::atomic<long long> cancel_upto{-1};  // withdraw requests with seq <= this
+  long long seq = 0;                       // submission counter (under mu)
   std::thread th;
   // guarded by mu; its bytes are part of `used`
   std::list<Pooled> pool;
@@ -222,7 +225,7 @@ void finish(oa_server* s, Active* a, oa_status st) {
   res.reused = a->from_pool ? 1 : 0;
   if (a->ev) (void)hipEventDestroy(a->ev);
   // a context that ran without a device error goes back to the pool
-  const bool keep = a->ctx && (st == OA_OK || st == OA_ENOCONV);
+  const bool keep = a->ctx && (st == OA_OK || st == OA_ENOCONV || st == OA_ECANCELED);
   Pooled p;
   std::vector<oa_ctx*> release;
   if (keep) {
@@ -411,7 +414,9 @@ void scheduler(oa_server* s) {
     }
     // 2. issue the next step of every idle request
     for (Active* a : active) {
-      if (a->phase.load(std::memory_order_relaxed) != 0 || a->inflight) continue;
+      if (a->phase.load(std::memory_order_relaxed) != 0 || a->inflight ||
+          a->job.seq <= s->cancel_upto.load(std::memory_order_relaxed))
+        continue;
       const oa_status st = oa_step(a->ctx, nullptr);
       if (st != OA_OK || hipEventRecord(a->ev, oa_ctx_stream(a->ctx)) != hipSuccess) {
         a->issue_failed = true;  // retired below
@@ -428,6 +433,7 @@ void scheduler(oa_server* s) {
       }
       oa_status st = OA_OK;
       bool end = false;
+      const bool cancelled = a->job.seq <= s->cancel_upto.load(std::memory_order_relaxed);
       const StepPoll poll = a->issue_failed ? StepPoll::Failed
                             : a->inflight   ? classify_event(hipEventQuery(a->ev))
                                             : StepPoll::Running;
@@ -464,6 +470,10 @@ void scheduler(oa_server* s) {
             progressed = true;
           }
         }
+      }
+      if (!end && cancelled && !a->inflight && a->phase.load(std::memory_order_relaxed) == 0) {
+        st = OA_ECANCELED;  // withdrawn: ends once no step is in flight
+        end = true;
       }
       if (end) {
         finish(s, a, st);
@@ -555,6 +565,7 @@ oa_status oa_server_submit(oa_server* s, const oa_request* req) {
   j.t_submit = clk::now();
   {
     std::lock_guard<std::mutex> lk(s->mu);
+    j.seq = s->seq++;
     s->queue.push_back(std::move(j));
     s->submitted++;
   }
@@ -577,6 +588,27 @@ oa_status oa_server_drain(oa_server* s) {
   if (!s) return OA_EINVAL;
   std::unique_lock<std::mutex> lk(s->mu);
   s->cv_done.wait(lk, [&] { return s->finished == s->submitted; });
+  return OA_OK;
+}
+
+oa_status oa_server_cancel(oa_server* s) {
+  if (!s) return OA_EINVAL;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->cancel_upto.store(s->seq - 1, std::memory_order_relaxed);
+    const auto now = clk::now();
+    for (const Job& j : s->queue) {
+      oa_result r{};
+      r.tag = j.r.tag;
+      r.status = OA_ECANCELED;
+      r.queue_ms = r.total_ms = ms_since(j.t_submit, now);
+      s->done.push_back(r);
+      s->finished++;
+    }
+    s->queue.clear();
+  }
+  s->cv_done.notify_all();
+  s->cv_work.notify_all();
   return OA_OK;
 }
 

@@ -44,6 +44,8 @@ def _bind(L):
     L.oa_server_poll.restype = ctypes.c_int
     L.oa_server_drain.argtypes = [vp]
     L.oa_server_drain.restype = ctypes.c_int
+    L.oa_server_cancel.argtypes = [vp]
+    L.oa_server_cancel.restype = ctypes.c_int
     L.oa_server_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     L.oa_server_stats.restype = ctypes.c_int
     L.oa_server_destroy.argtypes = [vp]
@@ -167,6 +169,14 @@ class Server:
             if not r:
                 return res
             res += r
+
+    def cancel(self):
+        """Withdraw every request not finished yet (results with status
+        OA_ECANCELED: queued ones at once, admitted ones after their step in
+        flight)."""
+        st = self._L.oa_server_cancel(self.h)
+        if st != _lib.OA_OK:
+            raise OaError(st, "oa_server_cancel failed")
 
     def stats(self):
         a = (ctypes.c_double * 8)()

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import oracle
-from openatom_amd import serve, system
+from openatom_amd import _lib, serve, system
 from openatom_amd.step import CPContext
 from tests.helpers import small_system
 
@@ -243,3 +243,24 @@ def test_mixed_switch_pools_both_contexts(gpu):
     assert res["x"]["fp64_steps"] == fresh["fp64_steps"]
     assert res["x"]["eelec"] == fresh["eelec"]
     assert np.array_equal(res["x"]["psi"], fresh["psi"])
+
+
+def test_cancel_withdraws_unfinished_requests(gpu):
+    """oa_server_cancel: queued requests finish at once and the admitted one
+    after its step in flight, all with OA_ECANCELED; a request submitted
+    afterwards runs normally and equals one run on a fresh server."""
+    ss = systems()
+    with serve.Server(max_active=1) as srv:
+        for t in range(4):
+            srv.submit(ss[t], t, f_tol=F_TOL, max_steps=MAX_STEPS, **OPTS)
+        srv.cancel()
+        srv.submit(ss[0], "after", f_tol=F_TOL, max_steps=MAX_STEPS, want_psi=True, **OPTS)
+        res = {r["tag"]: r for r in srv.drain()}
+    with serve.Server(max_active=1) as srv:
+        fresh = run(srv, [("after", ss[0])])["after"]
+    print(f"[serve cancel] {[(t, res[t]['status'], res[t]['steps']) for t in range(4)]}; after: "
+          f"{res['after']['status']} {res['after']['steps']} steps", flush=True)
+    assert all(res[t]["status"] == _lib.OA_ECANCELED for t in range(4))
+    assert all(res[t]["steps"] < MAX_STEPS for t in range(4))
+    assert res["after"]["status"] == 0 and res["after"]["steps"] == fresh["steps"]
+    assert np.array_equal(res["after"]["psi"], fresh["psi"])
