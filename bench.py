@@ -368,9 +368,10 @@ def serving_leg(args, mix, precisions):
         serving[prec] = r["summary"]
         runs_s[prec] = r["res"]
     if "mixed" in runs_s and "fp64" in runs_s:
-        de = [abs(runs_s["mixed"][k]["eelec"] - runs_s["fp64"][k]["eelec"]) /
-              (3 * mix.nmol[k]) for k in range(len(mix))
-              if runs_s["mixed"][k]["status"] == 0 and runs_s["fp64"][k]["status"] == 0]
+        rm, r64 = runs_s["mixed"], runs_s["fp64"]
+        de = [abs(rm[k]["eelec"] - r64[k]["eelec"]) / (3 * mix.nmol[k])
+              for k in range(len(mix))
+              if k in rm and k in r64 and rm[k]["status"] == 0 and r64[k]["status"] == 0]
         serving["max_dE_per_atom_mixed_vs_fp64"] = max(de) if de else None
     return serving
 
