@@ -31,6 +31,22 @@ struct Shape {
 
 __device__ __forceinline__ int signed_k(int i, int n) { return (i <= (n - 1) / 2) ? i : i - n; }
 
+// XCD-aware block order of a 2D grid.  The chip hands workgroups to its 8
+// XCDs (each with its own L2) round-robin by linear block id, so blocks x
+// and x+1 would run on different XCDs.  This map gives each XCD a
+// contiguous range of logical blocks (x fastest): neighbouring line blocks,
+// which share the 128-byte lines at their edges, then meet in one L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ void xcd_block(int& bx, int& by) {
+  const int nx = gridDim.x, total = nx * gridDim.y;
+  const int lin = blockIdx.x + blockIdx.y * nx;
+  const int xcd = lin % kXcds, k = lin / kXcds;
+  const int q = total / kXcds, r = total % kXcds;
+  const int logical = xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
+  bx = logical % nx;
+  by = logical / nx;
+}
+
 // ------------------------------------------------------------------ z passes
 
 // PR: every stored |kz| <= (N - 1) / 4 (pruned pass 1, Sph in oa_fft2.h)
@@ -39,7 +55,9 @@ __global__ __launch_bounds__(kThreads) void k2_zfwd(FftCtx c) {
   using Sh = Shape<R1, R2>;
   constexpr int N = Sh::N, G = Sh::G, NTR = Sh::NTR;
   __shared__ cpx<T> ex[NTR * N];
-  const int s = blockIdx.y, l0 = blockIdx.x * NTR;
+  int bx, s;
+  xcd_block(bx, s);
+  const int l0 = bx * NTR;
   const int nl = min(NTR, c.g.nlines - l0);
   const int tid = threadIdx.x;
   // pass 2's zbuf column, loaded up front (behind pass 1)
@@ -96,7 +114,9 @@ __global__ __launch_bounds__(kThreads) void k2_zbwd(FftCtx c) {
   __shared__ cpx<T> ex[NTR * N];
   __shared__ unsigned char wt_l[NTR * N];  // per point: 1 on kx = 0 lines, 2 elsewhere
   __shared__ double red[kThreads / 64];
-  const int s = blockIdx.y, l0 = blockIdx.x * NTR;
+  int bx, s;
+  xcd_block(bx, s);
+  const int l0 = bx * NTR;
   const int nl = min(NTR, c.g.nlines - l0);
   const int tid = threadIdx.x;
   const size_t plane = (size_t)c.g.nplane_x * c.g.ny;
@@ -196,7 +216,7 @@ __global__ __launch_bounds__(kThreads) void k2_zbwd(FftCtx c) {
   if (tid == 0) {
     double tsum = 0.0;
     for (int w = 0; w < kThreads / 64; w++) tsum += red[w];
-    c.eke_part[(size_t)s * gridDim.x + blockIdx.x] = 0.5 * tsum;
+    c.eke_part[(size_t)s * gridDim.x + bx] = 0.5 * tsum;
   }
 }
 
