@@ -89,7 +89,9 @@ typedef struct {
                           2: one cooperative launch over 32 x 32 tiles,
                           3, 4, 5: one cooperative launch over 16-row blocks, each split
                           over 1, 2 or 4 workgroups by 16-column tiles (4 and 5 need
-                          n > 16 and n > 48); 2..5 give OA_EINVAL where they do not fit */
+                          n > 16 and n > 48); 2..5 give OA_EINVAL where they do not fit;
+                          6: no cooperative launch: one plain launch of one workgroup for
+                          n <= 64, else as 1 (for contexts that share the device) */
   int vks_from_rho;    /* 1: each step builds V(r) = V_KS[rho] from its own density
                           (PZ-LDA exchange-correlation, Hartree, local pseudopotential of
                           oa_local_pseudo_set) between rho and V psi, as the density chares

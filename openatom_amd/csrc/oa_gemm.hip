@@ -963,6 +963,59 @@ __global__ __launch_bounds__(256) void k_ns_coop(NsArgs a) {
   }
 }
 
+// One workgroup for the whole iteration, a plain launch (n <= kNsOneMax):
+// the recurrence of k_ns_coop with the workgroup visiting every 32 x 32 tile
+// in turn and workgroup barriers in place of the grid barriers.  For the
+// small matrices of small requests this replaces 2 max_iter + 2 launches per
+// step (most of them early exits) by one, with no cooperative launch.
+__global__ __launch_bounds__(256) void k_ns_one(NsArgs a) {
+  extern __shared__ __align__(16) double ns_smem[];
+  const int n = a.n, nt = (n + NT - 1) / NT, ntile = nt * nt;
+  const int lane = threadIdx.x & 63, l16 = lane & 15, g = lane >> 4, wave = threadIdx.x >> 6;
+  const int wr = (wave >> 1) * 16, wc = (wave & 1) * 16;
+  for (int e = threadIdx.x; e < n * n; e += 256) {
+    a.X0[e] = 0.5 * a.S[e];  // B = S/2 (ortho.C:236-239)
+    a.T0[e] = (e / n == e % n) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  double* X[2] = {a.X0, a.X1};
+  double* T[2] = {a.T0, a.T1};
+  int conv = 0;
+  double err = 0.0;
+  for (int it = 1; it <= a.max_iter; it++) {
+    const int o = (it - 1) & 1, w = it & 1;
+    v4d p1, p2;
+    for (int t = 0; t < ntile; t++) {  // C = 3I - T X
+      const int r0 = (t / nt) * NT, c0 = (t % nt) * NT;
+      ns_tile2(T[o], X[o], nullptr, nullptr, n, r0, c0, ns_smem, p1, p2);
+      blk_store(a.C, p1, n, r0 + wr, c0 + wc, -1.0, 3.0, nullptr);
+    }
+    __syncthreads();
+    double e = 0.0;
+    for (int t = 0; t < ntile; t++) {  // X' = X C / 2, T' = C T / 2
+      const int r0 = (t / nt) * NT, c0 = (t % nt) * NT;
+      ns_tile2(X[o], a.C, a.C, T[o], n, r0, c0, ns_smem, p1, p2);
+      blk_store(X[w], p1, n, r0 + wr, c0 + wc, 0.5, 0.0, nullptr);
+      e += blk_store(T[w], p2, n, r0 + wr, c0 + wc, 0.5, 0.0, T[o]);
+    }
+    err = sqrt(wg_sum(e) / ((double)n * n));  // wg_sum ends with a barrier
+    if (err <= a.tol) {
+      conv = it;
+      break;
+    }
+  }
+  const int last = conv ? conv : a.max_iter;
+  if (threadIdx.x == 0) {
+    a.res->ns_iter = conv ? conv : -a.max_iter;
+    a.res->ns_err = err;
+  }
+  const double* Tf = T[last & 1];
+  for (int e = threadIdx.x; e < n * n; e += 256) a.Tout[e] = Tf[e];
+  (void)lane;
+  (void)l16;
+  (void)g;
+}
+
 // ---- fallback: two launches per iteration with a device-side early exit
 
 // 16x16 block (row0, col0) of A B straight from global memory
@@ -1588,6 +1641,13 @@ bool ns_coop_possible(int n, int device) {
 
 hipError_t ns_run(const NsArgs& a, hipStream_t st) {
   if (!a.coop) return ns_run_launches(a, st);
+  if (a.coop == kNsOneKind) {
+    if (a.n > kNsOneMax) return ns_run_launches(a, st);
+    const hipError_t e = grant_lds((const void*)k_ns_one, kNsSmem);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_ns_one, dim3(1), dim3(256), kNsSmem, st, a);
+    return hipGetLastError();
+  }
   NsArgs args = a;
   void* params[] = {&args};
   hipError_t e;

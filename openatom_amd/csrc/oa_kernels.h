@@ -231,8 +231,11 @@ struct NsArgs {
   int coop;  // 0: two launches per iteration; 1: one cooperative launch over 32 x 32
              // tiles (two grid barriers per iteration); 2, 3, 4: one cooperative launch
              // over 16-row blocks, each split over 1, 2 or 4 workgroups by column
-             // tiles (one barrier per iteration)
+             // tiles (one barrier per iteration); kNsOneKind: one workgroup, a plain
+             // launch, for n <= kNsOneMax (two launches per iteration above)
 };
+constexpr int kNsOneKind = 5;
+constexpr int kNsOneMax = 64;
 hipError_t ns_run(const NsArgs& a, hipStream_t st);
 // FP32 work space of the mixed-precision Newton-Schulz: X and T (double
 // buffered) and C, n x n floats each

@@ -706,9 +706,12 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
       if (!c->dry && oa::ns_rows_possible(n, oa::ns_rows_split(kind), c->device))
         c->ns_kinds.push_back(kind);
     if (!c->dry && oa::ns_coop_possible(n, c->device)) c->ns_kinds.push_back(1);
+    if (n <= oa::kNsOneMax) c->ns_kinds.push_back(oa::kNsOneKind);
     c->ns_kinds.push_back(0);
   } else if (nsm == 1) {
     c->ns_kinds.push_back(0);
+  } else if (nsm == 6) {  // no cooperative launch
+    c->ns_kinds.push_back(n <= oa::kNsOneMax ? oa::kNsOneKind : 0);
   } else if (nsm == 2) {
     if (!c->dry && !oa::ns_coop_possible(n, c->device))
       return fail(c, OA_EINVAL, "ns_mode 2: the 32 x 32 tiles of n do not fit on the device at once");
@@ -722,7 +725,7 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
                       "by 16-column tiles) do not fit on the device at once");
     c->ns_kinds.push_back(kind);
   } else {
-    return fail(c, OA_EINVAL, "ns_mode must be 0..5");
+    return fail(c, OA_EINVAL, "ns_mode must be 0..6");
   }
   c->ns_kind = c->ns_kinds.front();
   c->sd_fused = false;  // measured slower than a separate SD pass (registers)
@@ -1319,7 +1322,9 @@ oa_status autotune(oa_ctx* c) {
     HIPCHK(hipMemcpy(c->S, two.data(), two.size() * sizeof(double), hipMemcpyHostToDevice));
     tune_knob(
         c, e0, e1,
-        "Newton-Schulz row blocks split over 4, 2, 1 workgroups (4, 3, 2), cooperative tiles (1) or launches (0)", c->ns_kinds,
+        "Newton-Schulz row blocks split over 4, 2, 1 workgroups (4, 3, 2), cooperative tiles (1), "
+        "one workgroup (5) or launches (0)",
+        c->ns_kinds,
         c->ns_kind, [&](int v) { c->ns_kind = v; },
         [&]() { return st_ns(c, c->opt.ns_tol, c->opt.ns_max_iter); });
   }

@@ -100,8 +100,9 @@ def test_r2g_force_eke(gpu, name, mode):
 
 
 # Newton-Schulz forms: 0 autotuned, 1 launches per iteration, 2 cooperative
-# 32 x 32 tiles, 3/4/5 cooperative 16-row blocks split over 1/2/4 workgroups
-@pytest.mark.parametrize("ns_mode", [0, 1, 2, 3, 4, 5])
+# 32 x 32 tiles, 3/4/5 cooperative 16-row blocks split over 1/2/4 workgroups,
+# 6 one workgroup in a plain launch (n <= 64; launches per iteration above)
+@pytest.mark.parametrize("ns_mode", [0, 1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("name", ["small", "ragged37", "water8", "states150"])
 def test_lambda_sd_overlap_ns_rotate(gpu, name, ns_mode):
     s = get_system(name)
