@@ -349,7 +349,7 @@ def serving_leg(args, mix, precisions):
     import numpy as np
     arrive = mix.arrive
     # fixed ions: V_KS = V_xc + V_H + V_loc rebuilt from every step's rho
-    sopts = dict(dt=args.serve_dt, use_cg=1, ns_mode=1, scf=True)
+    sopts = dict(dt=args.serve_dt, use_cg=1, ns_mode=6, scf=True)
     serving = {"workload": f"{len(mix)} independent fixed-ion minimisations of water clusters "
                            f"(4-{args.serve_max_mol} H2O log-uniform, 4 states/molecule, "
                            f"Ecut in {{50,60,70}} Ry, V_KS from each step's rho), Poisson "
