@@ -619,8 +619,25 @@ bool use_pad(int npx) {
   return kCuLds / sp >= kCuLds / so;
 }
 
+// Wide workgroups: FP64 planes of 108..128 points use more than half a CU's
+// LDS with 256 threads, so one workgroup (four waves) runs per CU.  With 512
+// threads the plane's state and exchange buffers still fit one CU, twice as
+// many transforms run per round, and the CU holds eight waves.
+constexpr int kWideThreads = 512;
+template <typename T, int X1, int X2, int Y1, int Y2>
+bool use_wide(int npx) {
+  if constexpr (sizeof(T) == 8 && X1 * X2 >= 96) {
+    using L = OldLay<X1 * X2>;
+    return 2 * plane3_smem_l<T, X1, X2, Y1, Y2, kPlaneThreads, L>(npx) > kCuLds &&
+           plane3_smem_l<T, X1, X2, Y1, Y2, kWideThreads, L>(npx) <= kCuLds;
+  }
+  return false;
+}
+
 template <typename T, int X1, int X2, int Y1, int Y2, int NT>
 size_t plane3_smem(int npx) {
+  if (use_wide<T, X1, X2, Y1, Y2>(npx))
+    return plane3_smem_l<T, X1, X2, Y1, Y2, kWideThreads, OldLay<X1 * X2>>(npx);
   return use_pad<T, X1, X2, Y1, Y2, NT>(npx)
              ? plane3_smem_l<T, X1, X2, Y1, Y2, NT, PadLay<X1 * X2>>(npx)
              : plane3_smem_l<T, X1, X2, Y1, Y2, NT, OldLay<X1 * X2>>(npx);
@@ -628,9 +645,8 @@ size_t plane3_smem(int npx) {
 
 // Launch facts of one size for the host's chunk choice: row parts of the
 // G->R grid and resident workgroups per CU of both kernels.
-template <typename T, int X1, int X2, int Y1, int Y2, class L>
+template <typename T, int X1, int X2, int Y1, int Y2, int NT, class L>
 hipError_t plane3_info_l(int npx, int* rowparts, int* occ_g2r, int* occ_r2g, int* rho_ok) {
-  constexpr int NT = kPlaneThreads;
   using S = PS<X1, X2, Y1, Y2, NT, L>;
   constexpr bool VREG = sizeof(T) == 4 && S::ROUNDS * 2 * X2 <= 48;
   const size_t smem = plane3_smem_l<T, X1, X2, Y1, Y2, NT, L>(npx);
@@ -639,7 +655,8 @@ hipError_t plane3_info_l(int npx, int* rowparts, int* occ_g2r, int* occ_r2g, int
   const void* g = wide ? (const void*)k3_g2r<T, X1, X2, Y1, Y2, NT, S::ROUNDS, false, L>
                        : (const void*)k3_g2r<T, X1, X2, Y1, Y2, NT, row_rounds<T, L>(), false, L>;
   const void* r = (const void*)k3_r2g<T, X1, X2, Y1, Y2, NT, VREG, false, L>;
-  *rho_ok = r2g_rho_ok<X1, X2, NT>() ? 1 : 0;
+  // the R->G pass with rho accumulators spills at 512 threads
+  *rho_ok = r2g_rho_ok<X1, X2, NT>() && NT == kPlaneThreads ? 1 : 0;
   hipError_t e;
   if ((e = oa::grant_lds(g, smem)) != hipSuccess || (e = oa::grant_lds(r, smem)) != hipSuccess)
     return e;
@@ -648,9 +665,8 @@ hipError_t plane3_info_l(int npx, int* rowparts, int* occ_g2r, int* occ_r2g, int
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ_r2g, r, NT, smem);
 }
 
-template <typename T, int X1, int X2, int Y1, int Y2, class L>
+template <typename T, int X1, int X2, int Y1, int Y2, int NT, class L>
 hipError_t launch_plane3_l(int kind, const FftCtx& c, hipStream_t st) {
-  constexpr int NT = kPlaneThreads;
   using S = PS<X1, X2, Y1, Y2, NT, L>;
   constexpr int kRowRounds = row_rounds<T, L>();
   constexpr int RPn = (S::ROUNDS + kRowRounds - 1) / kRowRounds;
@@ -686,8 +702,9 @@ hipError_t launch_plane3_l(int kind, const FftCtx& c, hipStream_t st) {
       break;
     }
     case R2G_RHO: {
-      if (!r2g_rho_ok<X1, X2, NT>()) return hipErrorInvalidValue;
-      auto fn = k3_r2g<T, X1, X2, Y1, Y2, NT, VREG, r2g_rho_ok<X1, X2, NT>(), L>;
+      constexpr bool kRho = r2g_rho_ok<X1, X2, NT>() && NT == kPlaneThreads;
+      if (!kRho) return hipErrorInvalidValue;
+      auto fn = k3_r2g<T, X1, X2, Y1, Y2, NT, VREG, kRho, L>;
       if ((e = grant((const void*)fn)) != hipSuccess) return e;
       hipLaunchKernelGGL(fn, dim3(c.g.nz, c.nchunk_r), dim3(NT), smem, st, c);
       break;
@@ -698,32 +715,49 @@ hipError_t launch_plane3_l(int kind, const FftCtx& c, hipStream_t st) {
   return hipGetLastError();
 }
 
+// the variant a launch takes: 512 threads (OldLay) for wide FP64 planes,
+// else 256 with PadLay where it keeps OldLay's workgroups per CU; PR: pruned
+template <typename T, int X1, int X2, int Y1, int Y2, bool PR>
+hipError_t plane3_info_pr(int npx, int* rowparts, int* occ_g2r, int* occ_r2g, int* rho_ok) {
+  constexpr int N = X1 * X2, NT = kPlaneThreads;
+  if constexpr (sizeof(T) == 8 && N >= 96) {
+    if (use_wide<T, X1, X2, Y1, Y2>(npx))
+      return plane3_info_l<T, X1, X2, Y1, Y2, kWideThreads, OldLay<N, PR>>(npx, rowparts, occ_g2r,
+                                                                            occ_r2g, rho_ok);
+  }
+  return use_pad<T, X1, X2, Y1, Y2, NT>(npx)
+             ? plane3_info_l<T, X1, X2, Y1, Y2, NT, PadLay<N, PR>>(npx, rowparts, occ_g2r, occ_r2g,
+                                                                   rho_ok)
+             : plane3_info_l<T, X1, X2, Y1, Y2, NT, OldLay<N, PR>>(npx, rowparts, occ_g2r, occ_r2g,
+                                                                   rho_ok);
+}
+
 template <typename T, int X1, int X2, int Y1, int Y2>
 hipError_t plane3_info(int npx, bool prune, int* rowparts, int* occ_g2r, int* occ_r2g,
                        int* rho_ok) {
-  constexpr int N = X1 * X2;
-  const bool pad = use_pad<T, X1, X2, Y1, Y2, kPlaneThreads>(npx);
-  if (prune)
-    return pad ? plane3_info_l<T, X1, X2, Y1, Y2, PadLay<N, true>>(npx, rowparts, occ_g2r,
-                                                                   occ_r2g, rho_ok)
-               : plane3_info_l<T, X1, X2, Y1, Y2, OldLay<N, true>>(npx, rowparts, occ_g2r,
-                                                                   occ_r2g, rho_ok);
-  return pad ? plane3_info_l<T, X1, X2, Y1, Y2, PadLay<N>>(npx, rowparts, occ_g2r, occ_r2g, rho_ok)
-             : plane3_info_l<T, X1, X2, Y1, Y2, OldLay<N>>(npx, rowparts, occ_g2r, occ_r2g, rho_ok);
+  return prune ? plane3_info_pr<T, X1, X2, Y1, Y2, true>(npx, rowparts, occ_g2r, occ_r2g, rho_ok)
+               : plane3_info_pr<T, X1, X2, Y1, Y2, false>(npx, rowparts, occ_g2r, occ_r2g, rho_ok);
+}
+
+template <typename T, int X1, int X2, int Y1, int Y2, bool PR>
+hipError_t launch_plane3_pr(int kind, const FftCtx& c, hipStream_t st) {
+  constexpr int N = X1 * X2, NT = kPlaneThreads;
+  const int npx = c.g.nplane_x;
+  if constexpr (sizeof(T) == 8 && N >= 96) {
+    if (use_wide<T, X1, X2, Y1, Y2>(npx))
+      return launch_plane3_l<T, X1, X2, Y1, Y2, kWideThreads, OldLay<N, PR>>(kind, c, st);
+  }
+  return use_pad<T, X1, X2, Y1, Y2, NT>(npx)
+             ? launch_plane3_l<T, X1, X2, Y1, Y2, NT, PadLay<N, PR>>(kind, c, st)
+             : launch_plane3_l<T, X1, X2, Y1, Y2, NT, OldLay<N, PR>>(kind, c, st);
 }
 
 // c.prune (set by the host when every populated |kx|, |ky| <= (N - 1) / 4)
 // selects the variants pruned to the sphere
 template <typename T, int X1, int X2, int Y1, int Y2>
 hipError_t launch_plane3(int kind, const FftCtx& c, hipStream_t st) {
-  constexpr int N = X1 * X2;
-  if (c.prune)
-    return use_pad<T, X1, X2, Y1, Y2, kPlaneThreads>(c.g.nplane_x)
-               ? launch_plane3_l<T, X1, X2, Y1, Y2, PadLay<N, true>>(kind, c, st)
-               : launch_plane3_l<T, X1, X2, Y1, Y2, OldLay<N, true>>(kind, c, st);
-  return use_pad<T, X1, X2, Y1, Y2, kPlaneThreads>(c.g.nplane_x)
-             ? launch_plane3_l<T, X1, X2, Y1, Y2, PadLay<N>>(kind, c, st)
-             : launch_plane3_l<T, X1, X2, Y1, Y2, OldLay<N>>(kind, c, st);
+  return c.prune ? launch_plane3_pr<T, X1, X2, Y1, Y2, true>(kind, c, st)
+                 : launch_plane3_pr<T, X1, X2, Y1, Y2, false>(kind, c, st);
 }
 
 }  // namespace oa3

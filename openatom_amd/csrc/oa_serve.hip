@@ -24,16 +24,13 @@
 // Steps of different requests overlap on the device; no request waits for
 // another one to finish.
 //
-// Two helper threads keep slow host calls off the scheduler: one creates and
-// loads contexts (admission, the FP64 switch) while the scheduler keeps the
-// other requests' steps in flight; one releases contexts (every hipFree
-// waits for the whole device).
+// Every HIP call of the loop is made on the scheduler thread (contexts are
+// created, loaded and released there too).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <atomic>
 #include <chrono>
-#include <functional>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -42,6 +39,7 @@
 #include <thread>
 #include <vector>
 
+#include "oa_ctx_info.h"
 #include "oa_internal.h"
 #include "oa_serve.h"
 #include "oa_serve_poll.h"
@@ -49,46 +47,6 @@
 namespace {
 
 using clk = std::chrono::steady_clock;
-
-// A thread that runs posted tasks in order, with the device current.
-struct Worker {
-  std::thread th;
-  std::mutex mu;
-  std::condition_variable cv;
-  std::deque<std::function<void()>> tasks;
-  bool stop = false;
-  void begin(int device) {
-    th = std::thread([this, device] {
-      (void)hipSetDevice(device);
-      for (;;) {
-        std::function<void()> f;
-        {
-          std::unique_lock<std::mutex> lk(mu);
-          cv.wait(lk, [&] { return stop || !tasks.empty(); });
-          if (tasks.empty()) return;  // stop, and everything posted has run
-          f = std::move(tasks.front());
-          tasks.pop_front();
-        }
-        f();
-      }
-    });
-  }
-  void post(std::function<void()> f) {
-    {
-      std::lock_guard<std::mutex> lk(mu);
-      tasks.push_back(std::move(f));
-    }
-    cv.notify_one();
-  }
-  void end() {  // runs what is posted, then joins
-    {
-      std::lock_guard<std::mutex> lk(mu);
-      stop = true;
-    }
-    cv.notify_one();
-    if (th.joinable()) th.join();
-  }
-};
 
 double ms_since(clk::time_point a, clk::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
@@ -160,10 +118,6 @@ using oa_serve::StepPoll;
 
 struct Active {
   Job job;
-  // 0: the scheduler owns the request; 1: the set-up thread is creating or
-  // loading its context; 2: set-up done with status setup_st
-  std::atomic<int> phase{0};
-  oa_status setup_st = OA_OK;
   oa_ctx* ctx = nullptr;
   bool from_pool = false;  // ctx was taken over from the pool at admission
   hipEvent_t ev = nullptr;
@@ -201,8 +155,6 @@ struct oa_server {
   // guarded by mu; its bytes are part of `used`
   std::list<Pooled> pool;
   long long pool_hits = 0;
-  Worker setup;   // oa_create, oa_psi_set, potential (admission, FP64 switch)
-  Worker reaper;  // oa_destroy
 };
 
 namespace {
@@ -252,7 +204,7 @@ void finish(oa_server* s, Active* a, oa_status st) {
     s->done.push_back(res);
     s->cv_done.notify_all();
   }
-  for (oa_ctx* c : release) s->reaper.post([c] { oa_destroy(c); });
+  for (oa_ctx* c : release) oa_destroy(c);
 }
 
 // Take a pooled context of this shape (caller holds s->mu), or nullptr.
@@ -292,7 +244,7 @@ oa_status load_potential(Active* a) {
   return oa_vks_set(a->ctx, a->job.vks.data());
 }
 
-// create and load one admitted request's context (set-up thread)
+// create and load one admitted request's context
 oa_status start(oa_server* s, Active* a) {
   oa_system sys = job_system(a->job);
   oa_status st = a->ctx ? OA_OK : oa_create(s->device, &sys, &a->job.r.opt, &a->ctx);
@@ -303,34 +255,34 @@ oa_status start(oa_server* s, Active* a) {
   return st;
 }
 
-// hand a request to the set-up thread; the scheduler skips it until done
-template <class F>
-void set_up(oa_server* s, Active* a, F f) {
-  a->phase.store(1, std::memory_order_relaxed);
-  s->setup.post([a, f] {
-    a->setup_st = f();
-    a->phase.store(2, std::memory_order_release);
-  });
-}
-
 // Continue a mixed-precision request in FP64: its coefficients go through
 // the host copy of psi0 into an FP64 context of its shape (pooled, else
-// new).  The mixed context goes to the pool for the next request of its
-// shape; the request's reservation (the larger of the two precisions) covers
-// the FP64 one, and pooled contexts of other shapes are released, least
-// recently pooled first, while the pool holds more than the budget allows.
+// new), which takes over the mixed context's CG direction and gamma history
+// (the minimisation continues rather than restarting).  The mixed context
+// then goes to the pool for the next request of its shape; the request's
+// reservation (the larger of the two precisions) covers the FP64 one, and
+// pooled contexts of other shapes are released, least recently pooled
+// first, while the pool holds more than the budget allows.
 oa_status to_fp64(oa_server* s, Active* a) {
   oa_status st = oa_psi_get(a->ctx, a->job.psi0.data());
   if (st != OA_OK) return st;
+  oa_ctx* mixed = a->ctx;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    a->job.r.opt.precision = OA_FP64;
+    a->ctx = pool_take(s, with_precision(a->job.key, OA_FP64));
+  }
+  oa_system sys = job_system(a->job);
+  st = a->ctx ? OA_OK : oa_create(s->device, &sys, &a->job.r.opt, &a->ctx);
+  if (st == OA_OK) st = oa_psi_set(a->ctx, a->job.psi0.data());
+  if (st == OA_OK) st = load_potential(a);
+  if (st == OA_OK && a->job.r.opt.use_cg) st = oa_ctx_copy_cg(a->ctx, mixed);
   std::vector<oa_ctx*> release;
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    // a pooled FP64 context of this shape: the request's reservation already
-    // covers one (admission charged the larger of the two precisions)
-    oa_ctx* fp64 = pool_take(s, with_precision(a->job.key, OA_FP64));
     Pooled p;
-    p.ctx = a->ctx;
-    p.bytes = oa_ctx_device_bytes(a->ctx);
+    p.ctx = mixed;
+    p.bytes = oa_ctx_device_bytes(mixed);
     p.key = a->job.key;
     s->used += p.bytes;
     s->pool.push_back(std::move(p));
@@ -339,14 +291,8 @@ oa_status to_fp64(oa_server* s, Active* a) {
       release.push_back(s->pool.front().ctx);
       s->pool.pop_front();
     }
-    a->ctx = fp64;
-    a->job.r.opt.precision = OA_FP64;
   }
-  for (oa_ctx* c : release) s->reaper.post([c] { oa_destroy(c); });
-  oa_system sys = job_system(a->job);
-  st = a->ctx ? OA_OK : oa_create(s->device, &sys, &a->job.r.opt, &a->ctx);
-  if (st == OA_OK) st = oa_psi_set(a->ctx, a->job.psi0.data());
-  if (st == OA_OK) st = load_potential(a);
+  for (oa_ctx* c : release) oa_destroy(c);
   a->in_fp64 = true;
   return st;
 }
@@ -389,34 +335,22 @@ void scheduler(oa_server* s) {
         }
       }
     }
-    for (oa_ctx* c : release) s->reaper.post([c] { oa_destroy(c); });
+    for (oa_ctx* c : release) oa_destroy(c);
     release.clear();
     for (Active* a : admitted) {
       a->t_admit = clk::now();
-      set_up(s, a, [s, a] { return start(s, a); });
-      active.push_back(a);
+      const oa_status st = start(s, a);
+      if (st == OA_OK) {
+        active.push_back(a);
+      } else {
+        finish(s, a, st);
+        delete a;
+      }
     }
     bool progressed = !admitted.empty();
-    // requests whose set-up finished: back to the scheduler, or done on error
-    for (auto it = active.begin(); it != active.end();) {
-      Active* a = *it;
-      if (a->phase.load(std::memory_order_acquire) == 2) {
-        a->phase.store(0, std::memory_order_relaxed);
-        progressed = true;
-        if (a->setup_st != OA_OK) {
-          finish(s, a, a->setup_st);
-          delete a;
-          it = active.erase(it);
-          continue;
-        }
-      }
-      ++it;
-    }
     // 2. issue the next step of every idle request
     for (Active* a : active) {
-      if (a->phase.load(std::memory_order_relaxed) != 0 || a->inflight ||
-          a->job.seq <= s->cancel_upto.load(std::memory_order_relaxed))
-        continue;
+      if (a->inflight || a->job.seq <= s->cancel_upto.load(std::memory_order_relaxed)) continue;
       const oa_status st = oa_step(a->ctx, nullptr);
       if (st != OA_OK || hipEventRecord(a->ev, oa_ctx_stream(a->ctx)) != hipSuccess) {
         a->issue_failed = true;  // retired below
@@ -427,10 +361,6 @@ void scheduler(oa_server* s) {
     // 3. retire finished steps
     for (auto it = active.begin(); it != active.end();) {
       Active* a = *it;
-      if (a->phase.load(std::memory_order_relaxed) != 0) {
-        ++it;
-        continue;
-      }
       oa_status st = OA_OK;
       bool end = false;
       const bool cancelled = a->job.seq <= s->cancel_upto.load(std::memory_order_relaxed);
@@ -466,12 +396,12 @@ void scheduler(oa_server* s) {
             a->best_step = a->steps;
           }
           if (a->last.fnorm <= a->job.r.fp64_below || a->steps - a->best_step >= kStallSteps) {
-            set_up(s, a, [s, a] { return to_fp64(s, a); });
-            progressed = true;
+            st = to_fp64(s, a);
+            if (st != OA_OK) end = true;
           }
         }
       }
-      if (!end && cancelled && !a->inflight && a->phase.load(std::memory_order_relaxed) == 0) {
+      if (!end && cancelled && !a->inflight) {
         st = OA_ECANCELED;  // withdrawn: ends once no step is in flight
         end = true;
       }
@@ -492,7 +422,7 @@ void scheduler(oa_server* s) {
     pool.swap(s->pool);
     for (const Pooled& p : pool) s->used -= p.bytes;
   }
-  for (Pooled& p : pool) s->reaper.post([c = p.ctx] { oa_destroy(c); });
+  for (Pooled& p : pool) oa_destroy(p.ctx);
 }
 
 }  // namespace
@@ -509,8 +439,6 @@ oa_status oa_server_create(int device, size_t hbm_budget_bytes, int max_active,
   s->device = device;
   s->budget = hbm_budget_bytes;
   s->max_active = max_active;
-  s->setup.begin(device);
-  s->reaper.begin(device);
   s->th = std::thread(scheduler, s);
   *out = s;
   return OA_OK;
@@ -640,8 +568,6 @@ oa_status oa_server_destroy(oa_server* s) {
   }
   s->cv_work.notify_all();
   if (s->th.joinable()) s->th.join();
-  s->setup.end();  // idle: the scheduler only ends with no request in set-up
-  s->reaper.end();
   delete s;
   return OA_OK;
 }

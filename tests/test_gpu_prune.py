@@ -8,8 +8,9 @@ exact zeros and unused values go, so the density, forces and kinetic energy
 agree with the full transforms to rounding (1e-13 relative in FP64, 1e-6
 mixed), and both agree with the oracle (1e-10 FP64, 1e-5 mixed).  The
 cases: water8 (48^3, |k| <= 10), a serving shape (5 H2O at 70 Ry, 54^3 =
-9 x 6), and water8's sphere on a 40^3 grid, which does not satisfy the bound
-(both settings then run the full transforms)."""
+9 x 6), water8's sphere on a 40^3 grid, which does not satisfy the bound
+(both settings then run the full transforms), and a 108^3 serving shape
+whose FP64 plane passes run in 512-thread workgroups."""
 import numpy as np
 import pytest
 
@@ -30,6 +31,9 @@ def get_system(name):
         elif name == "serve5":
             s = system.build(system.serving_workload(5, 70.0), seed=8,
                              kvec_source=oracle.stored_kvecs)
+        elif name == "wide108":  # 40 H2O at 70 Ry: 108^3, FP64 planes in 512-thread workgroups
+            s = system.build(system.serving_workload(40, 70.0), seed=9, nstates=24,
+                             kvec_source=oracle.stored_kvecs)
         else:  # water8's sphere on a 40^3 grid (4 kmax + 1 = 41 > 40)
             s = system.build(dict(system.WORKLOADS["water8"], grid=40), seed=21,
                              kvec_source=oracle.stored_kvecs)
@@ -49,7 +53,7 @@ def run(s, precision, fft_full):
     return rho, f, eke
 
 
-@pytest.mark.parametrize("name", ["water8", "serve5", "water8_grid40"])
+@pytest.mark.parametrize("name", ["water8", "serve5", "water8_grid40", "wide108"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_pruned_equals_full(gpu, name, precision):
     s = get_system(name)
