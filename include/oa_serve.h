@@ -48,9 +48,9 @@ typedef struct {
                            until the request's result is returned */
   long long tag;        /* caller's id, returned with the result */
   double fp64_below;    /* mixed precision only: continue in FP64, from the current
-                           coefficients and CG direction, once |F| <= fp64_below or |F| has
-                           not improved for 50 steps (the FP32 FFT chain sets a floor on
-                           |F|); 0 disables the switch */
+                           coefficients with a fresh CG direction, once |F| <= fp64_below or
+                           |F| has not improved for 50 steps (the FP32 FFT chain sets a
+                           floor on |F|); 0 disables the switch */
   const oa_local_pseudo *pseudo; /* with vks NULL: the fixed ions (copied); every step
                            rebuilds V = V_xc + V_H + V_loc from its own rho, as the
                            density chares do (CP_Rho_RealSpacePlane.C:739-864) */

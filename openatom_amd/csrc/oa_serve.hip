@@ -39,7 +39,6 @@
 #include <thread>
 #include <vector>
 
-#include "oa_ctx_info.h"
 #include "oa_internal.h"
 #include "oa_serve.h"
 #include "oa_serve_poll.h"
@@ -257,32 +256,22 @@ oa_status start(oa_server* s, Active* a) {
 
 // Continue a mixed-precision request in FP64: its coefficients go through
 // the host copy of psi0 into an FP64 context of its shape (pooled, else
-// new), which takes over the mixed context's CG direction and gamma history
-// (the minimisation continues rather than restarting).  The mixed context
-// then goes to the pool for the next request of its shape; the request's
-// reservation (the larger of the two precisions) covers the FP64 one, and
-// pooled contexts of other shapes are released, least recently pooled
-// first, while the pool holds more than the budget allows.
+// new).  The mixed context goes to the pool for the next request of its
+// shape; the request's reservation (the larger of the two precisions) covers
+// the FP64 one, and pooled contexts of other shapes are released, least
+// recently pooled first, while the pool holds more than the budget allows.
 oa_status to_fp64(oa_server* s, Active* a) {
   oa_status st = oa_psi_get(a->ctx, a->job.psi0.data());
   if (st != OA_OK) return st;
-  oa_ctx* mixed = a->ctx;
-  {
-    std::lock_guard<std::mutex> lk(s->mu);
-    a->job.r.opt.precision = OA_FP64;
-    a->ctx = pool_take(s, with_precision(a->job.key, OA_FP64));
-  }
-  oa_system sys = job_system(a->job);
-  st = a->ctx ? OA_OK : oa_create(s->device, &sys, &a->job.r.opt, &a->ctx);
-  if (st == OA_OK) st = oa_psi_set(a->ctx, a->job.psi0.data());
-  if (st == OA_OK) st = load_potential(a);
-  if (st == OA_OK && a->job.r.opt.use_cg) st = oa_ctx_copy_cg(a->ctx, mixed);
   std::vector<oa_ctx*> release;
   {
     std::lock_guard<std::mutex> lk(s->mu);
+    // a pooled FP64 context of this shape: the request's reservation already
+    // covers one (admission charged the larger of the two precisions)
+    oa_ctx* fp64 = pool_take(s, with_precision(a->job.key, OA_FP64));
     Pooled p;
-    p.ctx = mixed;
-    p.bytes = oa_ctx_device_bytes(mixed);
+    p.ctx = a->ctx;
+    p.bytes = oa_ctx_device_bytes(a->ctx);
     p.key = a->job.key;
     s->used += p.bytes;
     s->pool.push_back(std::move(p));
@@ -291,8 +280,14 @@ oa_status to_fp64(oa_server* s, Active* a) {
       release.push_back(s->pool.front().ctx);
       s->pool.pop_front();
     }
+    a->ctx = fp64;
+    a->job.r.opt.precision = OA_FP64;
   }
   for (oa_ctx* c : release) oa_destroy(c);
+  oa_system sys = job_system(a->job);
+  st = a->ctx ? OA_OK : oa_create(s->device, &sys, &a->job.r.opt, &a->ctx);
+  if (st == OA_OK) st = oa_psi_set(a->ctx, a->job.psi0.data());
+  if (st == OA_OK) st = load_potential(a);
   a->in_fp64 = true;
   return st;
 }

@@ -1496,20 +1496,6 @@ size_t oa_ctx_device_bytes(const oa_ctx* c) { return c ? c->bytes : 0; }
 
 hipStream_t oa_ctx_stream(oa_ctx* c) { return c ? c->stream : nullptr; }
 int oa_ctx_nstates(const oa_ctx* c) { return c ? c->nst : 0; }
-oa_status oa_ctx_copy_cg(oa_ctx* dst, oa_ctx* src) {
-  if (!dst || !src || !dst->opt.use_cg || !src->opt.use_cg || dst->nst != src->nst ||
-      dst->ld != src->ld || dst->npts != src->npts || dst->device != src->device)
-    return OA_EINVAL;
-  oa_ctx* c = dst;  // for HIPCHK
-  HIPCHK(hipSetDevice(c->device));
-  HIPCHK(hipStreamSynchronize(src->stream));
-  HIPCHK(hipMemcpyAsync(dst->cgdir, src->cgdir, (size_t)dst->nst * dst->ld * sizeof(double2),
-                        hipMemcpyDeviceToDevice, dst->stream));
-  HIPCHK(hipMemcpyAsync(dst->cgs, src->cgs, sizeof(oa::CgDev), hipMemcpyDeviceToDevice,
-                        dst->stream));
-  HIPCHK(hipStreamSynchronize(dst->stream));
-  return OA_OK;
-}
 void oa_ctx_grid(const oa_ctx* c, int grid[3]) {
   grid[0] = c ? c->nx : 0;
   grid[1] = c ? c->ny : 0;
