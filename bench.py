@@ -447,6 +447,9 @@ def main():
     ap.add_argument("--ns-mode", type=int, default=None,
                     help="force the Newton-Schulz kernel (ns_mode option; default: autotuned). "
                          "1 = one launch per product, no cooperative launch")
+    ap.add_argument("--psir-keep", type=int, default=None,
+                    help="psir_keep option: 1 keeps psi(r) between the plane passes, 0 "
+                    "recomputes it in the R->G pass (default: the library's, 1)")
     ap.add_argument("--traffic-from", nargs="+", default=None,
                     help="rocprofv3 --pmc output directories (request-size counters): write "
                          "profiles/traffic_<workload>.json from them and exit (no GPU work)")
@@ -458,6 +461,8 @@ def main():
 
     if args.ns_mode is not None:
         RUN_OPTS["ns_mode"] = args.ns_mode
+    if args.psir_keep is not None:
+        RUN_OPTS["psir_keep"] = args.psir_keep
     if args.traffic_from:
         def prec(name):
             return "mixed" if "<float" in name else ("fp64" if "<double" in name else None)

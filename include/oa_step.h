@@ -122,6 +122,16 @@ typedef struct {
                           density sphere) they skip the radix-pass inputs that are zero
                           outside the sphere and the outputs that are not kept
                           (rounding-level differences only) */
+  int psir_keep;       /* fused plane engine with separate G->R and R->G passes (V built
+                          from rho, or rho not fused into the R->G pass): 1: the G->R pass
+                          also writes every state's psi(r) to the device (FP32 or FP64,
+                          about nstates x N reals) and the R->G pass reads it back, as the
+                          reference's RealSpacePlane chares hold psi(r) between the density
+                          and V psi (CP_State_RealSpacePlane.C:349-355, 530-597); 0
+                          (default): the R->G pass recomputes psi(r) from the z-pass output
+                          (no psi(r) buffer; measured faster on MI355X at 100^3: the extra
+                          write and read of psi(r) cost more than the y-FFT and x c2r they
+                          replace).  Bit-identical results either way. */
 } oa_options;
 
 typedef struct {

@@ -51,7 +51,8 @@ class OaOptions(ctypes.Structure):
                 ("fft_mode", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("ns_mode", ctypes.c_int), ("vks_from_rho", ctypes.c_int),
                 ("autotune", ctypes.c_int), ("gemm_split", ctypes.c_int),
-                ("ns_fp32", ctypes.c_int), ("fft_full", ctypes.c_int)]
+                ("ns_fp32", ctypes.c_int), ("fft_full", ctypes.c_int),
+                ("psir_keep", ctypes.c_int)]
 
 
 class OaStepOut(ctypes.Structure):

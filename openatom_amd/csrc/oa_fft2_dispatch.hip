@@ -13,6 +13,7 @@ namespace oa3 {
 #define OA_DECL(n, r1, r2)                                                                  \
   hipError_t plane3_n##n(int kind, bool fp32, const oa::FftCtx& c, hipStream_t st);        \
   size_t plane3_smem_n##n(int npx, bool fp32);                                              \
+  size_t plane3_kper_n##n(int npx, bool fp32);                                              \
   hipError_t plane3_info_n##n(int npx, bool fp32, bool prune, int* rowparts, int* occ_g2r,    \
                               int* occ_r2g, int* rho_ok);
 OA_FFT2_SIZES(OA_DECL)
@@ -50,6 +51,18 @@ size_t plane3_smem(int n, int npx, bool fp32) {
 #define OA_CASE(nn, r1, r2) \
   case nn:                  \
     return oa3::plane3_smem_n##nn(npx, fp32);
+    OA_FFT2_SIZES(OA_CASE)
+#undef OA_CASE
+    default:
+      return 0;
+  }
+}
+
+size_t plane3_kper(int n, int npx, bool fp32) {
+  switch (n) {
+#define OA_CASE(nn, r1, r2) \
+  case nn:                  \
+    return oa3::plane3_kper_n##nn(npx, fp32);
     OA_FFT2_SIZES(OA_CASE)
 #undef OA_CASE
     default:

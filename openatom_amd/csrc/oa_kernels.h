@@ -51,6 +51,10 @@ struct FftCtx {
   int psir_state;         // state whose psi(r) the G->R plane pass writes to psir
   int prune;              // plane passes pruned to |kx|, |ky| <= (N - 1) / 4
   int prune_z;            // z passes pruned to |kz| <= (nz - 1) / 4
+  // plane engine: psi(r) written by the G->R pass and read back by the R->G
+  // pass, [nst][nz][plane3_kper] complex T in the x pass's lane order; null:
+  // the R->G pass recomputes psi(r) from zbuf
+  void* psk;
 };
 
 // two-factor register FFT engine (oa_fft2_*.hip): kind 0 z fwd, 1 y fwd,
@@ -65,6 +69,8 @@ int fft2_ntr(int n);  // transforms per workgroup for length n, 0 if not compile
 // leaves the rho partials of its c.nchunk_r chunks (V independent of rho)
 hipError_t plane3_launch(int kind, int n, bool fp32, const FftCtx& c, hipStream_t st);
 size_t plane3_smem(int n, int nplane_x, bool fp32);  // 0 if n is not compiled
+// complex elements per state and z-plane of the kept psi(r) (FftCtx::psk)
+size_t plane3_kper(int n, int nplane_x, bool fp32);  // 0 if n is not compiled
 // row parts of the G->R grid and resident workgroups per CU of both passes
 hipError_t plane3_info(int n, int nplane_x, bool fp32, bool prune, int* rowparts, int* occ_g2r,
                        int* occ_r2g, int* rho_ok);

@@ -5,7 +5,7 @@
 // it reads the populated columns of the z-transformed plane (zbuf), runs the
 // y-FFT of the nplane_x columns into an LDS plane buffer P[kx][y], then the
 // x passes in rounds of pair-rows (two real rows per complex transform) with
-// an LDS exchange buffer E.  Nothing of psi(r) goes to HBM:
+// an LDS exchange buffer E.  By default nothing of psi(r) goes to HBM:
 //
 //   k3_g2r : y-FFT (FFTW_FORWARD) + Hermitian x-FFT -> rho partials of the
 //            chunk, accumulated in registers over the chunk's states
@@ -16,8 +16,11 @@
 //
 // Recomputing the G->R plane in k3_r2g costs one y-FFT and one x-FFT per
 // plane; it replaces a write and a read of psi(r) (Nx*Ny*Nz reals per state)
-// with a second read of the populated z-lines.  rho is complete before V
-// psi is formed, as in the reference, so V may depend on rho.
+// with a second read of the populated z-lines.  The KEEP variants take the
+// other side of that trade: the G->R pass also writes psi(r) (c.psk, in the
+// x pass's lane order) and the R->G pass reads it back instead of repeating
+// the two transforms.  rho is complete before V psi is formed, as in the
+// reference, so V may depend on rho.
 //
 // Template parameters: transform lengths Nx = X1*X2 and Ny = Y1*Y2 (the x
 // pass-2 writes X2 outputs per thread, so the smaller factor goes second),
@@ -96,6 +99,10 @@ struct PS {
   static constexpr int NTWB = X1 == X2 ? 0 : X1 * X2;
   static constexpr int NTW1 = (X1 - 1) * (X2 - 1) + 1;
   static constexpr int NTW = L::TW2 ? NTWA + NTWB : NTW1;
+  // kept psi(r) (KEEP variants): per state and plane, ROUNDS x X2 blocks of
+  // KSLOT pair-row values, one per active pass-2 lane of the x round
+  static constexpr int KSLOT = NTRX * X1;
+  static constexpr int KPER = ROUNDS * X2 * KSLOT;
   static_assert(NX == NY && X1 == Y2 && X2 == Y1,
                 "the fused plane passes need square x-y planes, y as R1 x R2, x as R2 x R1");
 };
@@ -280,6 +287,15 @@ __device__ __forceinline__ XLanes x_lanes(int r) {
   return x;
 }
 
+// Slot of an active pass-2 lane (x.tr < NTRX, x.j < X1) in a kept psi(r)
+// block: the lane's own thread index under the transform-minor map, so a
+// wave's loads and stores of one t run along consecutive elements.  The
+// slot does not depend on the round.
+template <int X1, int NTRX, class L>
+__device__ __forceinline__ int kslot(const XLanes& x) {
+  return (L::MAPS & kMapXc2) ? x.j * NTRX + x.tr : x.tr * X1 + x.j;
+}
+
 // One round of Hermitian x-FFTs (c2r by the pair-row trick) from the rows of
 // P: pair q carries rows y1 = 2q (real part) and y2 = 2q+1 (imaginary part).
 // On return thread (x.tr, x.j < X1) holds psi(r) at x = j + t*X1 in u[t].
@@ -334,7 +350,12 @@ __device__ __forceinline__ void x_c2r(const cpx<T>* P, cpx<T>* E, const cpx<T>* 
 // G->R plane pass.  Grid (nz, row parts, state chunks).  Row part rp covers
 // x rounds [rp*RPW, rp*RPW + RPW).  PSIR: write psi(r) of state
 // c.psir_state into c.psir ([Nz][Ny][Nx] of T) instead of rho partials.
-template <typename T, int X1, int X2, int Y1, int Y2, int NT, int RPW, bool PSIR, class L>
+// KEEP: also write every state's psi(r) to c.psk ([nst][nz][KPER] complex T,
+// pair rows as real and imaginary parts) for the R->G pass to read back
+// (the real-space plane the reference's RealSpacePlane chare holds between
+// the density and V psi).
+template <typename T, int X1, int X2, int Y1, int Y2, int NT, int RPW, bool PSIR, class L,
+          bool KEEP = false>
 __global__ __launch_bounds__(NT) void k3_g2r(FftCtx c) {
   using S = PS<X1, X2, Y1, Y2, NT, L>;
   constexpr int NX = S::NX, NY = S::NY;
@@ -376,6 +397,13 @@ __global__ __launch_bounds__(NT) void k3_g2r(FftCtx c) {
         x_c2r<T, X1, X2, Y1, Y2, NT, L>(P, E, TW, npx, x, u);
         if (x.act && x.j < X1) {
           const int y1 = 2 * x.q, y2 = y1 + 1;
+          if (KEEP) {
+            cpx<T>* kp = (cpx<T>*)c.psk +
+                         (((size_t)s * c.g.nz + z) * S::ROUNDS + r) * (X2 * S::KSLOT) +
+                         kslot<X1, S::NTRX, L>(x);
+#pragma unroll
+            for (int t = 0; t < X2; t++) kp[t * S::KSLOT] = u[t];
+          }
           if (PSIR) {
             T* pr = (T*)c.psir + (size_t)z * NP;
 #pragma unroll
@@ -418,7 +446,11 @@ __global__ __launch_bounds__(NT) void k3_g2r(FftCtx c) {
 // waves per SIMD (measured faster than two; larger planes would spill).  RHO: also accumulate rho
 // partials of the chunk from the recomputed psi(r), for a V(r) that does
 // not depend on this step's rho (then no separate G->R pass is needed).
-template <typename T, int X1, int X2, int Y1, int Y2, int NT, bool VREG, bool RHO, class L>
+// KEEP: psi(r) is read back from c.psk, as the G->R pass with KEEP wrote it,
+// instead of being recomputed from zbuf (no y-FFT and no x c2r here); each
+// round's values are loaded one round ahead.
+template <typename T, int X1, int X2, int Y1, int Y2, int NT, bool VREG, bool RHO, class L,
+          bool KEEP = false>
 __global__ __launch_bounds__(NT, (sizeof(T) == 4 && !RHO) ? (X1 * X2 <= 100 ? 3 : 2) : 1) void
 k3_r2g(FftCtx c) {
   using S = PS<X1, X2, Y1, Y2, NT, L>;
@@ -466,9 +498,25 @@ k3_r2g(FftCtx c) {
   }
   int trf, jf;  // r2c pass-2 lanes
   lane_map<(L::MAPS & kMapRx2) != 0, S::GX, NTRX>(tid, trf, jf);
+  // KEEP: this thread's slot in the kept blocks (slot 0 for lanes without
+  // pass-2 work, whose values are never used), and the next round's values
+  const cpx<T>* kbase = nullptr;
+  cpx<T> uk[KEEP ? X2 : 1];
+  if (KEEP) {
+    const XLanes x0 = x_lanes<X1, X2, Y1, Y2, NT, L>(0);
+    const int slot = (x0.tr < NTRX && x0.j < X1) ? kslot<X1, NTRX, L>(x0) : 0;
+    kbase = (const cpx<T>*)c.psk + (size_t)z * S::KPER + slot;
+    if (st0 < st1) {
+      const cpx<T>* kp = kbase + (size_t)st0 * c.g.nz * S::KPER;
+#pragma unroll
+      for (int t = 0; t < X2; t++) uk[KEEP ? t : 0] = kp[t * S::KSLOT];
+    }
+  }
   for (int s = st0; s < st1; s++) {
-    const cpx<T>* zs = (const cpx<T>*)c.zbuf + ((size_t)s * c.g.nz + z) * plane;
-    y_fwd<T, Y1, Y2, NT, L>(zs, P, TW, KY, npx);
+    if (!KEEP) {
+      const cpx<T>* zs = (const cpx<T>*)c.zbuf + ((size_t)s * c.g.nz + z) * plane;
+      y_fwd<T, Y1, Y2, NT, L>(zs, P, TW, KY, npx);
+    }
     const T oc = (T)c.occ[s];
 #pragma unroll
     for (int r = 0; r < R; r++) {
@@ -485,8 +533,21 @@ k3_r2g(FftCtx c) {
         }
       }
       cpx<T> u[X2];
-      x_c2r<T, X1, X2, Y1, Y2, NT, L>(P, E, TW, npx, x, u);
-      __syncthreads();  // E is rewritten below
+      if (KEEP) {
+#pragma unroll
+        for (int t = 0; t < X2; t++) u[t] = uk[KEEP ? t : 0];
+        // the following round, or round 0 of the next state (uniform branch)
+        const int rn = r + 1 < R ? r + 1 : 0, sn = r + 1 < R ? s : s + 1;
+        if (sn < st1) {
+          const cpx<T>* kp =
+              kbase + ((size_t)sn * c.g.nz * S::ROUNDS + rn) * (X2 * S::KSLOT);
+#pragma unroll
+          for (int t = 0; t < X2; t++) uk[KEEP ? t : 0] = kp[t * S::KSLOT];
+        }
+      } else {
+        x_c2r<T, X1, X2, Y1, Y2, NT, L>(P, E, TW, npx, x, u);
+        __syncthreads();  // E is rewritten below
+      }
       if (x.act && x.j < X1) {
         const int y2 = 2 * x.q + 1;
         if (RHO) {
@@ -678,15 +739,14 @@ hipError_t launch_plane3_l(int kind, const FftCtx& c, hipStream_t st) {
   hipError_t e = hipSuccess;
   switch (kind) {
     case G2R: {
-      if (g2r_wide<T, X1, X2, Y1, Y2, NT, L>(c.g.nplane_x)) {
-        auto fn = k3_g2r<T, X1, X2, Y1, Y2, NT, S::ROUNDS, false, L>;
-        if ((e = grant((const void*)fn)) != hipSuccess) return e;
-        hipLaunchKernelGGL(fn, dim3(c.g.nz, 1, c.nchunk), dim3(NT), smem, st, c);
-      } else {
-        auto fn = k3_g2r<T, X1, X2, Y1, Y2, NT, kRowRounds, false, L>;
-        if ((e = grant((const void*)fn)) != hipSuccess) return e;
-        hipLaunchKernelGGL(fn, dim3(c.g.nz, RPn, c.nchunk), dim3(NT), smem, st, c);
-      }
+      // c.psk set: the KEEP variant also writes psi(r) for the R->G pass
+      const bool wide = g2r_wide<T, X1, X2, Y1, Y2, NT, L>(c.g.nplane_x);
+      auto fn = wide ? (c.psk ? k3_g2r<T, X1, X2, Y1, Y2, NT, S::ROUNDS, false, L, true>
+                              : k3_g2r<T, X1, X2, Y1, Y2, NT, S::ROUNDS, false, L>)
+                     : (c.psk ? k3_g2r<T, X1, X2, Y1, Y2, NT, kRowRounds, false, L, true>
+                              : k3_g2r<T, X1, X2, Y1, Y2, NT, kRowRounds, false, L>);
+      if ((e = grant((const void*)fn)) != hipSuccess) return e;
+      hipLaunchKernelGGL(fn, dim3(c.g.nz, wide ? 1 : RPn, c.nchunk), dim3(NT), smem, st, c);
       break;
     }
     case G2R_PSIR: {
@@ -696,7 +756,9 @@ hipError_t launch_plane3_l(int kind, const FftCtx& c, hipStream_t st) {
       break;
     }
     case R2G: {
-      auto fn = k3_r2g<T, X1, X2, Y1, Y2, NT, VREG, false, L>;
+      // c.psk set: psi(r) read back from the G->R pass's KEEP output
+      auto fn = c.psk ? k3_r2g<T, X1, X2, Y1, Y2, NT, VREG, false, L, true>
+                      : k3_r2g<T, X1, X2, Y1, Y2, NT, VREG, false, L>;
       if ((e = grant((const void*)fn)) != hipSuccess) return e;
       hipLaunchKernelGGL(fn, dim3(c.g.nz, c.nchunk_r), dim3(NT), smem, st, c);
       break;
@@ -713,6 +775,16 @@ hipError_t launch_plane3_l(int kind, const FftCtx& c, hipStream_t st) {
       return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+// complex elements per state and plane of the kept psi(r) (c.psk) of the
+// variant a launch takes (the block layout depends only on the threads)
+template <typename T, int X1, int X2, int Y1, int Y2>
+size_t plane3_kper(int npx) {
+  constexpr int N = X1 * X2;
+  if (use_wide<T, X1, X2, Y1, Y2>(npx))
+    return PS<X1, X2, Y1, Y2, kWideThreads, OldLay<N>>::KPER;
+  return PS<X1, X2, Y1, Y2, kPlaneThreads, OldLay<N>>::KPER;
 }
 
 // the variant a launch takes: 512 threads (OldLay) for wide FP64 planes,

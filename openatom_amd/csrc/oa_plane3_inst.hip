@@ -26,6 +26,11 @@ size_t OA_CAT(plane3_smem_n, OA_FFT2_N)(int npx, bool fp32) {
               : plane3_smem<double, kX1, kX2, kY1, kY2, kPlaneThreads>(npx);
 }
 
+size_t OA_CAT(plane3_kper_n, OA_FFT2_N)(int npx, bool fp32) {
+  return fp32 ? plane3_kper<float, kX1, kX2, kY1, kY2>(npx)
+              : plane3_kper<double, kX1, kX2, kY1, kY2>(npx);
+}
+
 hipError_t OA_CAT(plane3_info_n, OA_FFT2_N)(int npx, bool fp32, bool prune, int* rowparts,
                                            int* occ_g2r, int* occ_r2g, int* rho_ok) {
   return fp32 ? plane3_info<float, kX1, kX2, kY1, kY2>(npx, prune, rowparts, occ_g2r, occ_r2g,

@@ -154,6 +154,7 @@ struct oa_ctx {
   double2 *twx = nullptr, *twy = nullptr, *twz = nullptr;
   double *g2 = nullptr, *occ = nullptr, *dtm = nullptr, *vks = nullptr, *rho = nullptr;
   void *zbuf = nullptr, *xbuf = nullptr, *psir = nullptr, *zout = nullptr;
+  void* psk = nullptr;  // plane engine, opt.psir_keep: psi(r) between the plane passes
   double* eke_part = nullptr;
   int nblk = 0;
   double *S = nullptr, *T = nullptr, *lam = nullptr, *gpart = nullptr, *S2 = nullptr;
@@ -639,9 +640,15 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
     TRY(dalloc(c, &c->fc.rho_part, (size_t)np * N * sizeof(double)));
   }
   if (c->engine == kEngPlane) {
-    // psi(r) stays on chip; one state's worth for oa_psir_get
+    // one state's psi(r) for oa_psir_get; every state's when the passes
+    // run separately and opt.psir_keep asks to keep it between them
     TRY(dalloc(c, &c->zout, zsz));
     TRY(dalloc(c, &c->psir, N * rsz));
+    if (c->opt.psir_keep && !c->fuse_rho) {
+      const size_t kper = oa::plane3_kper(c->nx, c->nplane_x, c->fp32);
+      if (kper == 0) return fail(c, OA_EINVAL, "plane size not compiled");
+      TRY(dalloc(c, &c->psk, (size_t)c->nst * c->nz * kper * csz));
+    }
   } else {
     if (c->engine != kEngFused)
       TRY(dalloc(c, &c->xbuf, (size_t)c->nst * c->nz * c->ny * c->nplane_x * csz));
@@ -770,6 +777,7 @@ oa_status alloc_state(oa_ctx* c, const double* occ) {
   f.zout = c->zout;
   f.xbuf = c->xbuf;
   f.psir = c->psir;
+  f.psk = c->psk;
   f.rho = c->rho;
   f.eke_part = c->eke_part;
   f.lines = c->lines;
@@ -797,6 +805,9 @@ FftCtx range_ctx(oa_ctx* c, int s0, int ns) {
   f.force = c->force + (size_t)s0 * c->ld;
   f.zbuf = static_cast<unsigned char*>(c->zbuf) + (size_t)s0 * zst;
   f.zout = static_cast<unsigned char*>(c->zout) + (size_t)s0 * zst;
+  if (c->psk)
+    f.psk = static_cast<unsigned char*>(c->psk) +
+            (size_t)s0 * c->nz * oa::plane3_kper(c->nx, c->nplane_x, c->fp32) * (c->fp32 ? 8 : 16);
   f.occ = c->occ + s0;
   f.eke_part = c->eke_part + (size_t)s0 * c->nzblk;
   f.nchunk = (ns + f.cs - 1) / f.cs;
@@ -1374,6 +1385,7 @@ void oa_default_options(oa_options* o) {
   o->gemm_split = 1;
   o->ns_fp32 = 0;
   o->fft_full = 0;
+  o->psir_keep = 0;
 }
 
 int oa_kvecs(double ecut_ry, const double hmat[9], int* kx, int* ky, int* kz) {
