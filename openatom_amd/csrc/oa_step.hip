@@ -203,6 +203,7 @@ struct oa_ctx {
   // autotuner (opt.autotune): candidate launch shapes (buffers are sized for
   // the largest) and what was measured
   std::vector<int> tune_nchunk, tune_nchunk_r, tune_nslice_sym, tune_nslice_full;
+  std::vector<int> tune_vks;  // V_KS rows / columns / lines per workgroup
   std::string tune_log;
   // non-local KB pseudopotential (oa_nonlocal_set): projector table, the
   // projections Z, their scaled transpose and the gram partials
@@ -603,12 +604,26 @@ oa_status alloc_vks(oa_ctx* c) {
   a.ecut4 = ecut4;
   if (oa::vks_smem(a) > 64 * 1024)
     return fail(c, OA_EINVAL, "V_KS pass needs more than 64 KiB of LDS");
+  // autotune: row pairs (x passes), columns (y passes) and z lines per
+  // workgroup among 2, 4, 8, 16 where the LDS fits; the partial-sum arrays
+  // are sized for the most workgroups
+  int vmin = a.pr;
+  if (c->opt.autotune) {
+    for (int v : {2, 4, 8, 16}) {
+      oa::VksArgs t = a;
+      t.pr = t.cb = t.lb = v;
+      if (oa::vks_smem(t) <= 64 * 1024) c->tune_vks.push_back(v);
+    }
+    vmin = c->tune_vks.front();
+  }
+  oa::VksArgs amin = a;
+  amin.pr = amin.cb = amin.lb = vmin;
   const size_t nbox = (size_t)a.nkx * a.nky * c->nz;
   TRY(dalloc(c, &a.hx, (size_t)c->nz * a.nkx * c->ny * sizeof(double2)));
   TRY(dalloc(c, &a.hl, nbox * sizeof(double2)));
   TRY(dalloc(c, &c->vloc, nbox * sizeof(double2)));
-  TRY(dalloc(c, &a.part_x, (size_t)oa::vks_nbx(a) * 3 * sizeof(double)));
-  TRY(dalloc(c, &a.part_l, (size_t)oa::vks_nbl(a) * 2 * sizeof(double)));
+  TRY(dalloc(c, &a.part_x, (size_t)oa::vks_nbx(amin) * 3 * sizeof(double)));
+  TRY(dalloc(c, &a.part_l, (size_t)oa::vks_nbl(amin) * 2 * sizeof(double)));
   a.vloc = c->vloc;
   return OA_OK;
 }
@@ -1320,6 +1335,20 @@ oa_status autotune(oa_ctx* c) {
         return st_gram(c, c->psi, c->psi, true, 1.0 / std::sqrt(2.0), 1.0, c->S,
                        split_on(c, kSplitS));
       });
+  if (c->opt.vks_from_rho && c->tune_vks.size() > 1) {
+    // V_KS from the rho of the trial passes above; the energy partials are
+    // summed in workgroup order, so the choice moves E_H and E_xc by rounding
+    oa::VksArgs& va = c->va;
+    tune_knob(
+        c, e0, e1, "V_KS row pairs per x-pass workgroup", c->tune_vks, va.pr,
+        [&](int v) { va.pr = v; }, [&]() { return st_vks(c); });
+    tune_knob(
+        c, e0, e1, "V_KS columns per y-pass workgroup", c->tune_vks, va.cb,
+        [&](int v) { va.cb = v; }, [&]() { return st_vks(c); });
+    tune_knob(
+        c, e0, e1, "V_KS z lines per workgroup", c->tune_vks, va.lb,
+        [&](int v) { va.lb = v; }, [&]() { return st_vks(c); });
+  }
   tune_knob(
       c, e0, e1, "Lambda K slices", c->tune_nslice_full, c->nslice_full,
       [&](int v) { c->nslice_full = v; },
@@ -1358,6 +1387,8 @@ oa_status autotune(oa_ctx* c) {
   HIPCHK(hipStreamSynchronize(c->stream));
   // leave no state from the trial runs behind
   HIPCHK(oa::fill_zero(c->res, sizeof(StepDev), c->stream));
+  if (c->opt.vks_from_rho)  // V_KS of the trials' (zero) density
+    HIPCHK(oa::fill_zero(c->vks, (size_t)c->nx * c->ny * c->nz * sizeof(double), c->stream));
   c->have_S = c->have_T = false;
   HIPCHK(hipStreamSynchronize(c->stream));
   return OA_OK;
