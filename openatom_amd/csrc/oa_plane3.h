@@ -498,6 +498,10 @@ k3_r2g(FftCtx c) {
   }
   int trf, jf;  // r2c pass-2 lanes
   lane_map<(L::MAPS & kMapRx2) != 0, S::GX, NTRX>(tid, trf, jf);
+  // split loop: element idx = qq npx + k of a round, stepped by NT without a
+  // division per element (idx + NT = (qq + sdq) npx + k + sdk, k + sdk < 2 npx)
+  const int sq0 = tid / npx, sk0 = tid - sq0 * npx;
+  const int sdq = NT / npx, sdk = NT - sdq * npx;
   // KEEP: this thread's slot in the kept blocks (slot 0 for lanes without
   // pass-2 work, whose values are never used), and the next round's values
   const cpx<T>* kbase = nullptr;
@@ -595,8 +599,8 @@ k3_r2g(FftCtx c) {
       __syncthreads();
       // split the two real rows, keep kx < nplane_x, conjugated, into P
       const int nq = min(NTRX, S::NPAIR - r * NTRX);
-      for (int idx = tid; idx < nq * npx; idx += NT) {
-        const int qq = idx / npx, k = idx - qq * npx;
+      for (int idx = tid, qq = sq0, k = sk0; idx < nq * npx;
+           idx += NT, qq += sdq + (k + sdk >= npx), k += sdk - (k + sdk >= npx ? npx : 0)) {
         const int r1 = 2 * (r * NTRX + qq), r2 = r1 + 1;
         const cpx<T> Zk = E[qq * RE + pe<X2, L>(k)];
         const cpx<T> Zm = E[qq * RE + pe<X2, L>(k == 0 ? 0 : NX - k)];

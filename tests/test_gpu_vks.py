@@ -92,6 +92,37 @@ def test_vks_steps_match_oracle(gpu, name):
     assert e_psi < TOL
 
 
+@pytest.mark.parametrize("name", ["square27", "water8"])
+def test_vks_autotuned_tiling_matches_oracle(gpu, name):
+    """opt.autotune also times the V_KS passes' row pairs, columns and z
+    lines per workgroup (2, 4, 8, 16 where the LDS fits); whatever it picks,
+    V_KS, its energies and three steps match the oracle at 1e-10."""
+    s = get_system(name)
+    o, vl = oracle_vloc(s)
+    rho = o.density(o.g2r_all(s["psi"]))
+    v_ref, e_ref = o.vks(rho, vl)
+    psi_o = s["psi"].copy()
+    with make_ctx(s, autotune=1) as c:
+        log = c.tuning_log()
+        print(f"[{name} autotune]\n{log}", flush=True)
+        for knob in ("V_KS row pairs", "V_KS columns", "V_KS z lines"):
+            assert knob in log
+        c.set_psi(s["psi"])
+        c.fft_g2r()
+        v = c.get_vks()
+        r = c.result()
+        errs = dict(v=rel(v, v_ref), ehart=rel(r["ehart"], e_ref["ehart"]),
+                    exc=rel(r["exc"], e_ref["exc"]))
+        c.set_psi(s["psi"])
+        for k in range(3):
+            g = c.step()
+            ro = o.step(None, psi_o, dt=5.0, ns_tol=1e-15, ns_max_iter=30, vloc=vl)
+            errs[f"eelec{k}"] = rel(g["eelec"], ro["eelec"])
+        errs["psi"] = rel(c.get_psi(), psi_o)
+    report(f"[{name} autotuned V_KS]", **errs)
+    assert all(e < TOL for e in errs.values())
+
+
 def test_vks_mixed_precision_error(gpu):
     """FP32 FFT chain, V_KS built in FP64 from the FP32-derived rho: the
     energies after three steps stay within 1e-6 Ha/atom of the oracle."""
