@@ -3,7 +3,9 @@ plane waves per state) on the GPU, checked through size-independent
 properties because the oracle would take minutes per step at this size:
 orthonormality of the rotated states, the density sum rule, the energy
 decrease of steepest descent with V_KS rebuilt from rho, and the agreement
-of the mixed-precision path with the FP64 path."""
+of the mixed-precision path with the FP64 path; and the workload as its
+configuration runs it, autotuned and replayed from HIP graphs, against eager
+launches."""
 import numpy as np
 import pytest
 
@@ -22,10 +24,10 @@ def water128():
     return _cache["s"]
 
 
-def run(s, precision, nsteps):
+def run(s, precision, nsteps, **kw):
     t = s["pseudo"]
     with CPContext(s["grid"], s["hmat"], s["ecut_ry"], s["nstates"], s["kx"], s["ky"], s["kz"],
-                   precision=precision, vks_from_rho=1, want_resid=1) as c:
+                   precision=precision, vks_from_rho=1, want_resid=1, **kw) as c:
         c.set_psi(s["psi"])
         c.set_local_pseudo(s["pos"], s["species"], t["gmin"], t["dg"], t["vps"], t["vps_g0"])
         outs = [c.step() for _ in range(nsteps)]
@@ -57,3 +59,23 @@ def test_water128_fp64_and_mixed(gpu):
           flush=True)
     assert de < 1e-6
     assert omx[-1]["resid_max"] < 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_water128_autotuned_graph_replay(gpu, precision):
+    """Autotuned launch shapes and the step replayed from HIP graphs (captured
+    on the second step, replayed from the third) give the eager path's
+    energies: the two contexts may tune different state chunkings, which
+    reorders the rho sums, so the bar is rounding (1e-12 relative in FP64,
+    1e-7 mixed), and the graph steps keep the orthonormality bar."""
+    s = water128()
+    n = 4
+    og, _, _ = run(s, precision, n, autotune=1, use_graph=1)
+    oe, _, _ = run(s, precision, n, autotune=1, use_graph=0)
+    bar = 1e-12 if precision == "fp64" else 1e-7
+    for k, (g, e) in enumerate(zip(og, oe)):
+        d = abs(g["eelec"] - e["eelec"]) / abs(e["eelec"])
+        print(f"[water128 {precision} graph] step {k}: E {g['eelec']:.10f} eager "
+              f"{e['eelec']:.10f} rel {d:.1e} resid {g['resid_max']:.2e}", flush=True)
+        assert d < bar
+        assert g["resid_max"] < 1e-10
